@@ -1,0 +1,227 @@
+"""Benchmark: LSRK54 time stepping of the 3D rising-bubble Euler problem,
+32^3 hexahedra, N = 4 (4.1M nodes x 5 states), FP64, one MI355X.
+
+A step is one LSRK54 step (5 stages).  Each stage is one launch of the fused
+single-pass kernel (volume + faces + update, Q ping-pong); the three-kernel
+drop-in path (volumerhs, facerhs, updatesolution) is timed beside it.
+
+Metric: GDOF-stage/s = nelem * Np * nstate * stages / seconds / 1e9, with
+all arrays resident in HBM before the timed region.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def load_package():
+    if "canary_amd" in sys.modules:
+        return sys.modules["canary_amd"]
+    pkg = os.path.join(ROOT, "canary.jl_amd")
+    spec = importlib.util.spec_from_file_location("canary_amd", os.path.join(pkg, "__init__.py"),
+                                                  submodule_search_locations=[pkg])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["canary_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def algorithmic_bytes(dim, N, nstate=None, word=8):
+    """Bytes one element moves per stage when every array the three reference
+    kernels touch is counted once: Q r+w, dQ r+w, the metric terms + MJ + MJI,
+    normals + sMJ, vmapM + vmapP (Int64) and elemtobndy."""
+    Nq = N + 1
+    Np, Nfp, nface = Nq ** dim, Nq ** (dim - 1), 2 * dim
+    ns = nstate if nstate is not None else dim + 2
+    nvol = dim * dim + 2
+    nsurf = dim + 1
+    return word * (4 * ns * Np + nvol * Np + nface * Nfp * nsurf) + 2 * nface * Nfp * 8 + nface * 8
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--ne", type=int, default=32)
+    ap.add_argument("--N", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-elems", type=int, default=32768)
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build measures the single-GPU kernel; "
+              "multi-GPU sharding is described, not run (got --gpus %d)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    pkg = load_package()
+    if not pkg.built():
+        raise SystemExit("libcanary_dg.so is not built; run python canary.jl_amd/build.py")
+    import canary_amd.kernels as K
+    import canary_amd.problems as P
+
+    t0 = time.time()
+    s = P.bubble3d(Ne=args.ne, N=args.N, gravity=9.81, dtheta=2.0, jitter=1e-3, seed=1)
+    setup_s = time.time() - t0
+    print("setup %s: %d elements, %.1f s" % (s.label, s.nelem, setup_s), file=sys.stderr, flush=True)
+    dev = torch.device("cuda:0")
+
+    def d(a):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to(dev)
+
+    vgeo, sgeo, D = d(s.vgeo), d(s.sgeo), d(s.D)
+    vM, vP, bnd = d(s.vmapM), d(s.vmapP), d(s.elemtobndy)
+    Q0 = d(s.Q)
+    A, B, _ = P.rk_coefs()
+    dt = 0.01
+    stream = torch.cuda.current_stream()
+
+    def run_fused(q, q2, rhs, nsteps):
+        for _ in range(nsteps):
+            for st in range(5):
+                K.stage_fused(3, s.N, rhs, q, q2, vgeo, sgeo, D, s.nelem, vM, vP, bnd,
+                              A[(st + 1) % 5], B[st], dt, phys=s.phys)
+                q, q2 = q2, q
+        return q, q2
+
+    def run_plain(q, rhs, nsteps):
+        for _ in range(nsteps):
+            for st in range(5):
+                K.volumerhs(3, s.N, rhs, q, vgeo, D, s.nelem, phys=s.phys)
+                K.facerhs(3, s.N, rhs, q, sgeo, s.nelem, vM, vP, bnd, phys=s.phys)
+                K.updatesolution(3, s.N, rhs, q, vgeo, s.nelem, A[(st + 1) % 5], B[st], dt)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t_wall = time.perf_counter()
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3, time.perf_counter() - t_wall
+
+    # fused path (the measured value)
+    q, q2 = Q0.clone(), torch.empty_like(Q0)
+    rhs = torch.zeros_like(Q0)
+    q, q2 = run_fused(q, q2, rhs, args.warmup)
+    K.check_errors()
+    holder = {}
+
+    def go():
+        holder["q"] = run_fused(q, q2, rhs, args.steps)
+
+    t_fused, wall_fused = timed(go)
+    K.check_errors()
+    qf = holder["q"][0]
+    if not bool(torch.isfinite(qf).all()):
+        raise SystemExit("non-finite state after the timed steps")
+    stages = 5 * args.steps
+    dof = s.nelem * s.Np * 5
+    value = dof * stages / t_fused / 1e9
+    launch_s = t_fused / stages
+    bytes_launch = algorithmic_bytes(3, s.N) * s.nelem
+    achieved = bytes_launch / launch_s / 1e9
+
+    extra = {}
+    if not args.no_plain:
+        qp, rp = Q0.clone(), torch.zeros_like(Q0)
+        run_plain(qp, rp, 1)
+        t_plain, _ = timed(lambda: run_plain(qp, rp, max(1, args.steps // 2)))
+        K.check_errors()
+        st_plain = 5 * max(1, args.steps // 2)
+        extra["plain_three_kernel_gdof_stage_s"] = round(dof * st_plain / t_plain / 1e9, 3)
+        extra["plain_ms_per_stage"] = round(t_plain / st_plain * 1e3, 4)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(s, args.cpu_sample_elems)
+
+    traffic = measured_traffic("euler3d_N%d_%d" % (s.N, args.ne))
+    line = {
+        "metric": "GDOF-stage/s 3D Euler N=4 FP64 (LSRK54 stages, fused single-pass kernel)",
+        "value": round(value, 3),
+        "unit": "GDOF-stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_fused / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded rising thermal bubble initial state",
+        "config": {"workload": "3D Euler set2c rising bubble, [0,1000]^3 m, %d^3 hexes, N=%d, "
+                               "no-flux walls, g=9.81, +2 K bubble, 1e-3 K jitter, LSRK54"
+                               % (args.ne, s.N),
+                   "nelem": s.nelem, "nodes": s.nelem * s.Np, "dof": dof,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "kernel": "stage_kernel<Euler3D,5,1> (fused volume+face+update)",
+                     "bytes_per_element_stage": algorithmic_bytes(3, s.N),
+                     "avg_launch_us": round(launch_s * 1e6, 2)},
+        "cpu_baseline": cpu,
+        "wall_s_timed_region": round(wall_fused, 4),
+        **extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def measured_traffic(tag):
+    """HBM bytes per launch from the committed rocprofv3 counter summary, if any."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % tag)
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    return None
+
+
+def cpu_baseline(s, sample_elems):
+    """The oracle (a C restatement of the reference's Julia CPU kernels) on the
+    host: one LSRK stage (volumerhs!, facerhs!, updatesolution!) over the first
+    `sample_elems` elements of the same mesh, single thread, timed like the
+    reference (best of 3 after a warm-up)."""
+    import numpy as np
+    from oracle import oracle as O
+    n = min(sample_elems, s.nelem)
+    Q = np.asfortranarray(s.Q.copy())
+    rhs = s.zeros_state()
+    A, B, _ = __import__("canary_amd.problems", fromlist=["rk_coefs"]).rk_coefs()
+    def stage_time(threads):
+        t0 = time.perf_counter()
+        O.volumerhs(s, rhs, Q, nthreads=threads, nelem=n)
+        O.facerhs(s, rhs, Q, nthreads=threads, nelem=n)
+        O.lib().oracle_updatesolution(s.Np, 5, 14, 10, O._p(rhs), O._p(Q), O._p(s.vgeo), n,
+                                      A[1], B[0], 0.01, threads)
+        return time.perf_counter() - t0
+
+    stage_time(1)  # warm-up (page-in, first touch)
+    t1 = sorted(stage_time(1) for _ in range(5))[2]
+    ncpu = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 8)
+    stage_time(ncpu)
+    tn = sorted(stage_time(ncpu) for _ in range(5))[2]
+    return {"value": round(n * s.Np * 5 / t1 / 1e9, 5), "unit": "GDOF-stage/s", "cores": 1,
+            "kind": "port",
+            "sample": "LSRK stages (volumerhs!, facerhs!, updatesolution!) over %d of %d "
+                      "elements, median of 5 after a warm-up, single thread; C restatement of "
+                      "the Julia CPU kernels (gcc -O2 -ffp-contract=off)" % (n, s.nelem),
+            "all_cores": {"value": round(n * s.Np * 5 / tn / 1e9, 5), "cores": ncpu}}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,128 @@
+"""ctypes binding of libcanary_dg.so (declarations mirror include/canary_dg.h).
+
+The library is loaded from this package directory; a missing library is an
+error, never a silent fallback.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libcanary_dg.so")
+
+# error codes (canary_dg.h)
+OK, ERR_ARG, ERR_UNSUPPORTED, ERR_LAUNCH, ERR_BC, ERR_VMAP = 0, 1, 2, 3, 4, 5
+ERR_NOCONVERGENCE, ERR_LOBATTO_N1, ERR_ORDERING, ERR_ORIENTATION = 6, 7, 8, 9
+ERR_NOMEM, ERR_CAPACITY = 10, 11
+EULER, SWE = 0, 1
+FP64, FP32 = 0, 1
+ENDPT_NEITHER, ENDPT_LEFT, ENDPT_RIGHT, ENDPT_BOTH = 0, 1, 2, 3
+
+ERRNAMES = {
+    ERR_ARG: "bad argument", ERR_UNSUPPORTED: "unsupported (eqn, dim, N, prec)",
+    ERR_LAUNCH: "HIP launch/runtime error", ERR_BC: "invalid boundary condition",
+    ERR_VMAP: "vmap index out of range", ERR_NOCONVERGENCE: "QL iteration did not converge",
+    ERR_LOBATTO_N1: "Must have at least two points for both ends.",
+    ERR_ORDERING: "Problem finding vertex ordering", ERR_ORIENTATION: "unsupported face orientation",
+    ERR_NOMEM: "out of memory", ERR_CAPACITY: "request exceeds a fixed capacity",
+}
+
+
+class CanaryError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        super().__init__("%s failed: %s (code %d)" % (what, ERRNAMES.get(code, "error"), code))
+
+
+class Phys(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("gamma", "p0", "R_gas", "c_p", "c_v", "gravity", "swe_gravity", "delta_nl")]
+
+
+class Bubble(C.Structure):
+    _fields_ = [("theta0", C.c_double), ("dtheta", C.c_double), ("rc", C.c_double),
+                ("xc", C.c_double), ("yc", C.c_double), ("zc", C.c_double),
+                ("spherical", C.c_int), ("jitter", C.c_double), ("seed", C.c_uint64)]
+
+
+class Vortex(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("beta", "x0", "y0", "u0", "v0", "rho_inf", "p_inf", "gamma", "Lx", "Ly")]
+
+
+_lib = None
+
+P = C.c_void_p
+I64 = C.c_int64
+D = C.c_double
+INT = C.c_int
+
+_SIGS = {
+    "canary_phys_default": (None, [C.POINTER(Phys)]),
+    "canary_volumerhs": (INT, [INT, INT, INT, INT, P, P, P, P, I64, C.POINTER(Phys), P]),
+    "canary_facerhs": (INT, [INT, INT, INT, INT, P, P, P, I64, I64, P, P, P, C.POINTER(Phys), P]),
+    "canary_updatesolution": (INT, [INT, INT, INT, INT, P, P, P, I64, D, D, D, INT, P]),
+    "canary_stage_fused": (INT, [INT, INT, INT, INT, P, P, P, P, P, P, I64, I64, P, P, P,
+                                 D, D, D, C.POINTER(Phys), P]),
+    "canary_check_errors": (INT, [P, P]),
+    "canary_device_count": (INT, []),
+    "canary_gauss_rule": (INT, [INT, INT, INT, P, P]),
+    "canary_lglpoints": (INT, [INT, P, P]),
+    "canary_lgpoints": (INT, [INT, P, P]),
+    "canary_baryweights": (INT, [INT, P, P]),
+    "canary_spectralderivative": (INT, [INT, P, P]),
+    "canary_interpolationmatrix": (INT, [INT, P, INT, P, P]),
+    "canary_hilbertcode": (INT, [INT, P, INT, P]),
+    "canary_centroidtocode": (INT, [INT, INT, I64, P, P]),
+    "canary_vertsortandorder": (INT, [INT, P, P, P]),
+    "canary_brickmesh": (P, [INT, P, P, P, I64, I64, P, P]),
+    "canary_mesh_from_arrays": (P, [INT, I64, P, P, P, I64, P, P]),
+    "canary_mesh_partition": (INT, [P]),
+    "canary_mesh_connect": (INT, [P]),
+    "canary_mesh_size": (I64, [P, INT]),
+    "canary_mesh_get": (INT, [P, C.c_char_p, P]),
+    "canary_mesh_free": (None, [P]),
+    "canary_mappings": (INT, [INT, INT, I64, P, P, P, P, P]),
+    "canary_creategrid": (INT, [INT, INT, I64, P, P, P]),
+    "canary_computemetric": (INT, [INT, INT, I64, P, P, P, P]),
+    "canary_computegeometry": (INT, [INT, INT, I64, P, P, P, P]),
+    "canary_hash_jitter": (D, [C.c_uint64, D, D, D]),
+    "canary_ic_bubble": (INT, [INT, INT, I64, P, C.POINTER(Bubble), C.POINTER(Phys), P]),
+    "canary_ic_vortex2d": (INT, [INT, I64, P, C.POINTER(Vortex), D, P]),
+    "canary_ic_swe_bumps": (INT, [INT, I64, P, D, INT, P, D, D, P]),
+    "canary_cfl": (D, [INT, INT, INT, P, P, I64, C.POINTER(Phys)]),
+    "canary_l2energysquared": (D, [INT, INT, INT, INT, P, P, I64]),
+}
+
+
+def lib():
+    """Load libcanary_dg.so (built by build.py); raise if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError("libcanary_dg.so is not built: run `python canary.jl_amd/build.py`")
+        L = C.CDLL(LIBPATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def check(rc, what):
+    if rc != 0:
+        raise CanaryError(rc, what)
+
+
+def default_phys(**kw):
+    p = Phys()
+    lib().canary_phys_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, float(v))
+    return p

@@ -1,0 +1,459 @@
+// Generic CDNA4 kernels of the nodal-DG right-hand side and LSRK update.
+//
+// Templated on the physics policy (phys.h), the number of 1-D nodes NQ = N+1
+// and EPW, the number of elements one workgroup owns.  Threads map to
+// (element slot, volume node) for volume work and to (element slot, face,
+// face node) for face work; a workgroup is sized to a multiple of the
+// 64-lane wavefront.
+//
+// Reference kernels these replace (behaviour, not structure):
+//   knl_volumerhs!      examples/3d_kernels/euler3d_set2c.jl:366-486
+//   knl_facerhs!        examples/3d_kernels/euler3d_set2c.jl:490-620
+//   knl_updatesolution! examples/3d_kernels/euler3d_set2c.jl:624-639
+//   (2D: euler2d_set2c.jl:318-524; SWE: shallow_water.jl:303-490)
+//
+// Evaluation order.  Each per-node result is accumulated in the order of
+// the reference CPU kernels (euler3d_set2c.jl:144-358): rhs, then the
+// buoyancy source, then the xi sweep n = 1..Nq, the eta sweep, the zeta
+// sweep; then the face contributions in ascending face number; then the
+// update Q + ((rkb*dt)*rhs)*MJI and rhs*rka.  With -ffp-contract=off the
+// only differences from the CPU oracle come from pow() (OCML vs libm).
+//
+// Face contributions are scattered into a staged copy of the element's rhs
+// in LDS in `DIM` rounds, faces (2d, 2d+1) per round: the two faces of one
+// direction share no node, so the owner computes without atomics, and edge
+// and corner nodes receive their face terms in ascending face order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "phys.h"
+
+namespace canary {
+
+constexpr int round64(int x) { return (x + 63) / 64 * 64; }
+constexpr int ipow(int a, int b) { return b == 0 ? 1 : a * ipow(a, b - 1); }
+
+template <class Phys, int NQ>
+struct Shape {
+  static constexpr int DIM = Phys::DIM;
+  static constexpr int Np = ipow(NQ, DIM);
+  static constexpr int Nfp = ipow(NQ, DIM - 1);
+  static constexpr int NFACE = 2 * DIM;
+  static constexpr int NFN = NFACE * Nfp;
+};
+
+// Elements per workgroup: smallest EPW whose round64(EPW*Np) keeps >= 90% of
+// the lanes busy, else the best utilisation with at most 256 threads.
+template <int Np>
+constexpr int default_epw() {
+  int best = 1;
+  double bu = 0.0;
+  for (int e = 1; e * Np <= 256 || e == 1; ++e) {
+    double u = double(e * Np) / double(round64(e * Np));
+    if (u >= 0.9) return e;
+    if (u > bu + 1e-12) {
+      bu = u;
+      best = e;
+    }
+    if (e * Np > 256) break;
+  }
+  return best;
+}
+
+// Device-side error record of this translation unit:
+// [code, face (1-based), element (1-based), offending value].
+static __device__ unsigned long long g_status[4];
+
+__device__ __forceinline__ void record_error(unsigned long long code, long long face,
+                                             long long elem, long long value) {
+  if (atomicCAS(&g_status[0], 0ull, code) == 0ull) {
+    g_status[1] = (unsigned long long)face;
+    g_status[2] = (unsigned long long)elem;
+    g_status[3] = (unsigned long long)value;
+  }
+}
+
+// Split a 1-based global node id into (element, node), 32-bit when it fits.
+template <int Np>
+__device__ __forceinline__ void split_id(int64_t id1, int64_t& elem, int& node) {
+  const uint64_t id = (uint64_t)(id1 - 1);
+  if ((id >> 32) == 0) {
+    const uint32_t i32 = (uint32_t)id;
+    elem = (int64_t)(i32 / (uint32_t)Np);
+    node = (int)(i32 % (uint32_t)Np);
+  } else {
+    elem = (int64_t)(id / (uint64_t)Np);
+    node = (int)(id % (uint64_t)Np);
+  }
+}
+
+// XCD-aware block order: workgroups are dispatched round-robin over the 8
+// XCDs, so give XCD x the contiguous (Hilbert-ordered) chunk of element
+// blocks [x*chunk, (x+1)*chunk); face neighbours then share an L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t chunk) {
+  return (b & 7) * chunk + (b >> 3);
+}
+
+// ------------------------------------------------------------------ sweeps
+// r[s] += sum_m D[m, i] F[m, j, k, s] (xi), then eta, then zeta, with the
+// reference CPU order.  sF[d] points at the flux of direction d,
+// laid out [s][node].
+template <int DIM, int NQ, int NSR>
+__device__ __forceinline__ void volume_sweeps(double* r, const double* sD, const double* sF0,
+                                              const double* sF1, const double* sF2, int n) {
+  constexpr int Np = ipow(NQ, DIM);
+  const int i = n % NQ;
+  const int j = (n / NQ) % NQ;
+  const int k = DIM == 3 ? n / (NQ * NQ) : 0;
+  const int base_xi = n - i;                      // (0, j, k)
+  const int base_eta = n - j * NQ;                // (i, 0, k)
+  const int base_zeta = n - k * NQ * NQ;          // (i, j, 0)
+#pragma unroll
+  for (int s = 0; s < NSR; ++s) {
+    double a = r[s];
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) a = a + sD[m + NQ * i] * sF0[s * Np + base_xi + m];
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) a = a + sD[m + NQ * j] * sF1[s * Np + base_eta + m * NQ];
+    if (DIM == 3) {
+#pragma unroll
+      for (int m = 0; m < NQ; ++m) a = a + sD[m + NQ * k] * sF2[s * Np + base_zeta + m * NQ * NQ];
+    }
+    r[s] = a;
+  }
+}
+
+// ------------------------------------------------------------ face terms
+// Contribution sMJ * Rusanov flux of face node `fi` (f*Nfp + fn) of element
+// e; M-side state read through `qm_at(s, vid)`, P side gathered from Q.
+// Returns vidM (0-based node of e), or -1 after recording an error.
+template <class Phys, int NQ, class QM, class PMF>
+__device__ __forceinline__ int face_contribution(const double* __restrict__ Q,
+                                                 const double* __restrict__ sgeo,
+                                                 const int64_t* __restrict__ vmapM,
+                                                 const int64_t* __restrict__ vmapP,
+                                                 const int64_t* __restrict__ bndy, int64_t e,
+                                                 int fi, int64_t ntotal, const PhysParams& p,
+                                                 QM qm_at, PMF pm_at, double* out) {
+  using S = Shape<Phys, NQ>;
+  constexpr int Np = S::Np, NS = Phys::NS, DIM = Phys::DIM, NSG = Phys::NSGEO;
+  const int f = fi / S::Nfp;
+  const size_t fo = (size_t)fi + (size_t)S::NFN * (size_t)e;
+  const double* sg = sgeo + (size_t)NSG * fo;
+  double nrm[DIM];
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) nrm[c] = sg[c];
+  const double sMJ = sg[DIM];
+  const int64_t idM = vmapM[fo];
+  const int64_t idP = vmapP[fo];
+  const int64_t bc = bndy[f + S::NFACE * e];
+  const int64_t nodes = ntotal * (int64_t)Np;
+  if (idM < 1 || idM > nodes) {
+    record_error(5 /*CANARY_ERR_VMAP*/, f + 1, e + 1, idM);
+    return -1;
+  }
+  int64_t eMdummy;
+  int vidM;
+  split_id<Np>(idM, eMdummy, vidM);
+  double qM[NS], qP[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) qM[s] = qm_at(s, vidM);
+  const double PM = pm_at(vidM, qM);
+  double PP = 0.0;
+  if (bc == 0) {
+    if (idP < 1 || idP > nodes) {
+      record_error(5 /*CANARY_ERR_VMAP*/, f + 1, e + 1, idP);
+      return -1;
+    }
+    int64_t eP;
+    int vidP;
+    split_id<Np>(idP, eP, vidP);
+    const double* qe = Q + (size_t)NS * Np * (size_t)eP;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) qP[s] = qe[vidP + Np * s];
+    if (Phys::B_FROM_M) qP[NS - 1] = qm_at(NS - 1, vidP);
+    PP = Phys::HAS_P ? Phys::pressure(qP, p) : 0.0;
+  } else if (bc == 1) {
+    Phys::reflect(qM, PM, nrm, qP, PP);
+  } else {
+    record_error(4 /*CANARY_ERR_BC*/, f + 1, e + 1, bc);
+    return -1;
+  }
+  Phys::face_flux(qM, PM, qP, PP, nrm, sMJ, p, out);
+  return vidM;
+}
+
+// ================================================================ kernels
+template <class Phys, int NQ, int EPW>
+struct VolumeCfg {
+  using S = Shape<Phys, NQ>;
+  static constexpr int NT = round64(EPW * S::Np);
+};
+
+template <class Phys, int NQ, int EPW>
+__global__ __launch_bounds__((VolumeCfg<Phys, NQ, EPW>::NT)) void volume_kernel(
+    double* __restrict__ rhs, const double* __restrict__ Q, const double* __restrict__ vgeo,
+    const double* __restrict__ D, int64_t nreal, PhysParams p) {
+  using S = Shape<Phys, NQ>;
+  constexpr int Np = S::Np, DIM = S::DIM, NS = Phys::NS, NSR = Phys::NSR, NV = Phys::NVGEO;
+  __shared__ double sD[NQ * NQ];
+  __shared__ double sF[DIM][EPW][NSR * Np];
+  const int t = threadIdx.x;
+  const int slot = t / Np, n = t - slot * Np;
+  const int64_t e = (int64_t)blockIdx.x * EPW + slot;
+  const bool valid = slot < EPW && e < nreal;
+  if (t < NQ * NQ) sD[t] = D[t];
+  double r[NSR];
+  if (valid) {
+    double q[NS];
+    const double* qe = Q + (size_t)NS * Np * (size_t)e;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) q[s] = qe[n + Np * s];
+    const double* vg = vgeo + (size_t)NV * Np * (size_t)e;
+    double met[DIM * DIM];
+#pragma unroll
+    for (int c = 0; c < DIM * DIM; ++c) met[c] = vg[n + Np * c];
+    const double MJ = vg[n + Np * Phys::iMJ];
+    const double P = Phys::HAS_P ? Phys::pressure(q, p) : 0.0;
+    double F[DIM][NSR];
+    Phys::volume_flux(q, P, met, MJ, p, F);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+#pragma unroll
+      for (int s = 0; s < NSR; ++s) sF[d][slot][s * Np + n] = F[d][s];
+    const double* re = rhs + (size_t)NS * Np * (size_t)e;
+#pragma unroll
+    for (int s = 0; s < NSR; ++s) r[s] = re[n + Np * s];
+    Phys::source(q, MJ, p, r);
+  }
+  __syncthreads();
+  if (valid) {
+    volume_sweeps<DIM, NQ, NSR>(r, sD, sF[0][slot], sF[DIM > 1 ? 1 : 0][slot],
+                                sF[DIM > 2 ? 2 : 0][slot], n);
+    double* re = rhs + (size_t)NS * Np * (size_t)e;
+#pragma unroll
+    for (int s = 0; s < NSR; ++s) re[n + Np * s] = r[s];
+  }
+}
+
+template <class Phys, int NQ, int EPW>
+struct FaceCfg {
+  using S = Shape<Phys, NQ>;
+  static constexpr int NT = round64(EPW * S::NFN) > 1024 ? 1024 : round64(EPW * S::NFN);
+};
+
+template <class Phys, int NQ, int EPW>
+__global__ __launch_bounds__((FaceCfg<Phys, NQ, EPW>::NT)) void face_kernel(
+    double* __restrict__ rhs, const double* __restrict__ Q, const double* __restrict__ sgeo,
+    int64_t nreal, int64_t ntotal, const int64_t* __restrict__ vmapM,
+    const int64_t* __restrict__ vmapP, const int64_t* __restrict__ bndy, PhysParams p) {
+  using S = Shape<Phys, NQ>;
+  constexpr int Np = S::Np, NS = Phys::NS, NSR = Phys::NSR, NFN = S::NFN, Nfp = S::Nfp;
+  constexpr int NT = FaceCfg<Phys, NQ, EPW>::NT;
+  __shared__ double sR[EPW][NSR * Np];
+  __shared__ double sC[EPW][NFN * NSR];
+  __shared__ int sV[EPW][NFN];
+  const int t = threadIdx.x;
+  const int64_t e0 = (int64_t)blockIdx.x * EPW;
+  for (int idx = t; idx < EPW * NSR * Np; idx += NT) {
+    const int slot = idx / (NSR * Np), rem = idx - slot * (NSR * Np);
+    const int64_t e = e0 + slot;
+    if (e < nreal) sR[slot][rem] = rhs[(size_t)NS * Np * (size_t)e + rem];
+  }
+  for (int item = t; item < EPW * NFN; item += NT) {
+    const int slot = item / NFN, fi = item - slot * NFN;
+    const int64_t e = e0 + slot;
+    if (e >= nreal) continue;
+    const double* qe = Q + (size_t)NS * Np * (size_t)e;
+    auto qm_at = [&](int s, int v) { return qe[v + Np * s]; };
+    auto pm_at = [&](int, const double* qm) { return Phys::HAS_P ? Phys::pressure(qm, p) : 0.0; };
+    double c[NSR];
+    const int vid = face_contribution<Phys, NQ>(Q, sgeo, vmapM, vmapP, bndy, e, fi, ntotal, p,
+                                                qm_at, pm_at, c);
+    sV[slot][fi] = vid;
+#pragma unroll
+    for (int s = 0; s < NSR; ++s) sC[slot][fi * NSR + s] = vid >= 0 ? c[s] : __builtin_nan("");
+  }
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < S::DIM; ++d) {
+    for (int it = t; it < EPW * 2 * Nfp; it += NT) {
+      const int slot = it / (2 * Nfp), fi = 2 * d * Nfp + (it - slot * 2 * Nfp);
+      const int64_t e = e0 + slot;
+      const int v = sV[slot][fi];
+      if (e < nreal && v >= 0) {
+#pragma unroll
+        for (int s = 0; s < NSR; ++s) sR[slot][s * Np + v] = sR[slot][s * Np + v] - sC[slot][fi * NSR + s];
+      }
+    }
+    __syncthreads();
+  }
+  for (int idx = t; idx < EPW * NSR * Np; idx += NT) {
+    const int slot = idx / (NSR * Np), rem = idx - slot * (NSR * Np);
+    const int64_t e = e0 + slot;
+    if (e < nreal) rhs[(size_t)NS * Np * (size_t)e + rem] = sR[slot][rem];
+  }
+}
+
+template <class Phys, int NQ>
+__global__ __launch_bounds__(256) void update_kernel(double* __restrict__ rhs,
+                                                     double* __restrict__ Q,
+                                                     const double* __restrict__ vgeo,
+                                                     int64_t nreal, double rka, double rkb,
+                                                     double dt, int advection) {
+  using S = Shape<Phys, NQ>;
+  constexpr int Np = S::Np, NS = Phys::NS, NSR = Phys::NSR, NV = Phys::NVGEO;
+  const int64_t total = nreal * Np;
+  const double bdt = rkb * dt;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = g / Np;
+    const int n = (int)(g - e * Np);
+    const double MJI = vgeo[n + Np * (Phys::iMJI + (size_t)NV * e)];
+    double* qe = Q + (size_t)NS * Np * e;
+    double* re = rhs + (size_t)NS * Np * e;
+    double uu = 0.0, vv = 0.0;
+    if (Phys::HAS_ADVECTION && advection) {
+      const double Ht = qe[n + Np * 2] + qe[n + Np * 3];
+      uu = qe[n] / Ht;
+      vv = qe[n + Np] / Ht;
+    }
+#pragma unroll
+    for (int s = 0; s < NSR; ++s) {
+      const double r = re[n + Np * s];
+      qe[n + Np * s] = qe[n + Np * s] + bdt * r * MJI;
+      re[n + Np * s] = r * rka;
+    }
+    if (Phys::HAS_ADVECTION && advection) {
+      const double Ht = qe[n + Np * 2] + qe[n + Np * 3];
+      qe[n] = Ht * uu;
+      qe[n + Np] = Ht * vv;
+    }
+  }
+}
+
+// --------------------------------------------------------- fused stage
+// One LSRK stage of EPW elements per workgroup: volume term, face terms and
+// update in one pass over HBM.  Reads Qin (own + neighbour traces), writes
+// Qout; rhs in place.
+template <class Phys, int NQ, int EPW>
+struct FusedCfg {
+  using S = Shape<Phys, NQ>;
+  static constexpr int NT = round64(EPW * S::Np);
+  static constexpr int FLUX = Phys::DIM * S::Np;               // per state
+  static constexpr int POST = S::NFN + S::Np;                   // sC + sR per state
+  static constexpr int BUF = (FLUX > POST ? FLUX : POST) * Phys::NSR;
+};
+
+template <class Phys, int NQ, int EPW>
+__global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW>::NT)) void stage_kernel(
+    double* __restrict__ rhs, const double* __restrict__ Qin, double* __restrict__ Qout,
+    const double* __restrict__ vgeo, const double* __restrict__ sgeo,
+    const double* __restrict__ D, int64_t nreal, int64_t ntotal,
+    const int64_t* __restrict__ vmapM, const int64_t* __restrict__ vmapP,
+    const int64_t* __restrict__ bndy, double rka, double rkb, double dt, int64_t chunk,
+    PhysParams p) {
+  using S = Shape<Phys, NQ>;
+  using C = FusedCfg<Phys, NQ, EPW>;
+  constexpr int Np = S::Np, DIM = S::DIM, NS = Phys::NS, NSR = Phys::NSR, NV = Phys::NVGEO;
+  constexpr int NFN = S::NFN, Nfp = S::Nfp, NT = C::NT;
+  __shared__ double sD[NQ * NQ];
+  __shared__ double sQ[EPW][NS * Np];
+  __shared__ double sP[EPW][Phys::HAS_P ? Np : 1];
+  __shared__ double sB[EPW][C::BUF];  // fluxes, then (contributions | staged rhs)
+  __shared__ int sV[EPW][NFN];
+
+  const int64_t blk = xcd_block(blockIdx.x, chunk);
+  const int64_t e0 = blk * EPW;
+  if (e0 >= nreal) return;  // whole workgroup leaves together
+  const int t = threadIdx.x;
+  const int slot = t / Np, n = t - slot * Np;
+  const int64_t e = e0 + slot;
+  const bool valid = slot < EPW && e < nreal;
+  if (t < NQ * NQ) sD[t] = D[t];
+
+  double q[NS], r[NSR], MJI = 0.0;
+  if (valid) {
+    const double* qe = Qin + (size_t)NS * Np * (size_t)e;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      q[s] = qe[n + Np * s];
+      sQ[slot][s * Np + n] = q[s];
+    }
+    const double* vg = vgeo + (size_t)NV * Np * (size_t)e;
+    double met[DIM * DIM];
+#pragma unroll
+    for (int c = 0; c < DIM * DIM; ++c) met[c] = vg[n + Np * c];
+    const double MJ = vg[n + Np * Phys::iMJ];
+    MJI = vg[n + Np * Phys::iMJI];
+    const double P = Phys::HAS_P ? Phys::pressure(q, p) : 0.0;
+    if (Phys::HAS_P) sP[slot][n] = P;
+    double F[DIM][NSR];
+    Phys::volume_flux(q, P, met, MJ, p, F);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+#pragma unroll
+      for (int s = 0; s < NSR; ++s) sB[slot][(d * NSR + s) * Np + n] = F[d][s];
+    const double* re = rhs + (size_t)NS * Np * (size_t)e;
+#pragma unroll
+    for (int s = 0; s < NSR; ++s) r[s] = re[n + Np * s];
+    Phys::source(q, MJ, p, r);
+  }
+  __syncthreads();
+  if (valid) {
+    const double* b = sB[slot];
+    volume_sweeps<DIM, NQ, NSR>(r, sD, b, b + (DIM > 1 ? NSR * Np : 0),
+                                b + (DIM > 2 ? 2 * NSR * Np : 0), n);
+  }
+  __syncthreads();  // fluxes consumed; sB is reused below
+  double* const sC = &sB[0][0];  // per slot: [NFN * NSR] contributions, then [NSR * Np] rhs
+  if (valid) {
+#pragma unroll
+    for (int s = 0; s < NSR; ++s) sB[slot][NFN * NSR + s * Np + n] = r[s];
+  }
+  for (int item = t; item < EPW * NFN; item += NT) {
+    const int sl = item / NFN, fi = item - sl * NFN;
+    const int64_t ee = e0 + sl;
+    if (ee >= nreal) continue;
+    auto qm_at = [&](int s, int v) { return sQ[sl][s * Np + v]; };
+    auto pm_at = [&](int v, const double*) { return Phys::HAS_P ? sP[sl][Phys::HAS_P ? v : 0] : 0.0; };
+    double c[NSR];
+    const int vid = face_contribution<Phys, NQ>(Qin, sgeo, vmapM, vmapP, bndy, ee, fi, ntotal,
+                                                p, qm_at, pm_at, c);
+    sV[sl][fi] = vid;
+#pragma unroll
+    for (int s = 0; s < NSR; ++s) sB[sl][fi * NSR + s] = vid >= 0 ? c[s] : __builtin_nan("");
+  }
+  (void)sC;
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    for (int it = t; it < EPW * 2 * Nfp; it += NT) {
+      const int sl = it / (2 * Nfp), fi = 2 * d * Nfp + (it - sl * 2 * Nfp);
+      const int64_t ee = e0 + sl;
+      const int v = sV[sl][fi];
+      if (ee < nreal && v >= 0) {
+        double* rr = &sB[sl][NFN * NSR];
+#pragma unroll
+        for (int s = 0; s < NSR; ++s) rr[s * Np + v] = rr[s * Np + v] - sB[sl][fi * NSR + s];
+      }
+    }
+    __syncthreads();
+  }
+  if (valid) {
+    const double bdt = rkb * dt;
+    double* qo = Qout + (size_t)NS * Np * (size_t)e;
+    double* re = rhs + (size_t)NS * Np * (size_t)e;
+#pragma unroll
+    for (int s = 0; s < NSR; ++s) {
+      const double rs = sB[slot][NFN * NSR + s * Np + n];
+      qo[n + Np * s] = q[s] + bdt * rs * MJI;
+      re[n + Np * s] = rs * rka;
+    }
+#pragma unroll
+    for (int s = NSR; s < NS; ++s) qo[n + Np * s] = q[s];  // read-only states (SWE b)
+  }
+}
+
+}  // namespace canary

@@ -1,0 +1,117 @@
+// Host-side launch plumbing shared by the per-equation translation units:
+// argument block, grid shapes and the N -> NQ template dispatch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dg_kernels.h"
+
+namespace canary {
+
+enum Op { OP_VOLUME = 0, OP_FACE = 1, OP_UPDATE = 2, OP_STAGE = 3 };
+
+struct KArgs {
+  void* rhs;
+  const void* Q;
+  void* Qout;
+  const void* vgeo;
+  const void* sgeo;
+  const double* D;
+  int64_t nreal, ntotal;
+  const int64_t* vmapM;
+  const int64_t* vmapP;
+  const int64_t* bndy;
+  double rka, rkb, dt;
+  int advection;
+  PhysParams phys;
+  hipStream_t stream;
+};
+
+constexpr int kMaxN = 7;  // N = 1..7 compiled in
+
+inline int launch_status() {
+  hipError_t err = hipGetLastError();
+  return err == hipSuccess ? 0 : 3 /*CANARY_ERR_LAUNCH*/;
+}
+
+template <class Phys, int NQ>
+int launch(int op, const KArgs& a) {
+  using S = Shape<Phys, NQ>;
+  constexpr int EPW = default_epw<S::Np>();
+  if (a.nreal <= 0) return 0;
+  const int64_t nb = (a.nreal + EPW - 1) / EPW;
+  double* rhs = (double*)a.rhs;
+  const double* Q = (const double*)a.Q;
+  switch (op) {
+    case OP_VOLUME: {
+      constexpr int NT = VolumeCfg<Phys, NQ, EPW>::NT;
+      volume_kernel<Phys, NQ, EPW><<<dim3((unsigned)nb), dim3(NT), 0, a.stream>>>(
+          rhs, Q, (const double*)a.vgeo, a.D, a.nreal, a.phys);
+      break;
+    }
+    case OP_FACE: {
+      constexpr int NT = FaceCfg<Phys, NQ, EPW>::NT;
+      face_kernel<Phys, NQ, EPW><<<dim3((unsigned)nb), dim3(NT), 0, a.stream>>>(
+          rhs, Q, (const double*)a.sgeo, a.nreal, a.ntotal, a.vmapM, a.vmapP, a.bndy, a.phys);
+      break;
+    }
+    case OP_UPDATE: {
+      const int64_t total = a.nreal * S::Np;
+      int64_t blocks = (total + 255) / 256;
+      if (blocks > 65536) blocks = 65536;
+      update_kernel<Phys, NQ><<<dim3((unsigned)blocks), dim3(256), 0, a.stream>>>(
+          rhs, (double*)a.Qout, (const double*)a.vgeo, a.nreal, a.rka, a.rkb, a.dt, a.advection);
+      break;
+    }
+    case OP_STAGE: {
+      constexpr int NT = FusedCfg<Phys, NQ, EPW>::NT;
+      const int64_t chunk = (nb + 7) / 8;
+      stage_kernel<Phys, NQ, EPW><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
+          rhs, Q, (double*)a.Qout, (const double*)a.vgeo, (const double*)a.sgeo, a.D, a.nreal,
+          a.ntotal, a.vmapM, a.vmapP, a.bndy, a.rka, a.rkb, a.dt, chunk, a.phys);
+      break;
+    }
+    default:
+      return 1 /*CANARY_ERR_ARG*/;
+  }
+  return launch_status();
+}
+
+template <class Phys>
+int dispatch(int op, int N, const KArgs& a) {
+  switch (N) {
+    case 1: return launch<Phys, 2>(op, a);
+    case 2: return launch<Phys, 3>(op, a);
+    case 3: return launch<Phys, 4>(op, a);
+    case 4: return launch<Phys, 5>(op, a);
+    case 5: return launch<Phys, 6>(op, a);
+    case 6: return launch<Phys, 7>(op, a);
+    case 7: return launch<Phys, 8>(op, a);
+  }
+  return 2 /*CANARY_ERR_UNSUPPORTED*/;
+}
+
+// Read (and optionally clear) this translation unit's device error record.
+inline int read_status(int64_t* info, int clear) {
+  unsigned long long h[4] = {0, 0, 0, 0};
+  if (hipMemcpyFromSymbol(h, HIP_SYMBOL(g_status), sizeof(h), 0, hipMemcpyDeviceToHost) != hipSuccess)
+    return 3;
+  if (clear && h[0] != 0) {
+    unsigned long long z[4] = {0, 0, 0, 0};
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_status), z, sizeof(z), 0, hipMemcpyHostToDevice) != hipSuccess)
+      return 3;
+  }
+  if (info)
+    for (int i = 0; i < 4; ++i) info[i] = (int64_t)h[i];
+  return (int)h[0];
+}
+
+}  // namespace canary
+
+#define CANARY_DEFINE_EQUATION(NAME, PHYS)                                        \
+  extern "C" int canary_k_##NAME(int op, int N, const canary::KArgs* a) {        \
+    return canary::dispatch<PHYS>(op, N, *a);                                     \
+  }                                                                               \
+  extern "C" int canary_k_##NAME##_status(int64_t* info, int clear) {            \
+    return canary::read_status(info, clear);                                      \
+  }

@@ -1,0 +1,220 @@
+// Physics policies of the DG kernels: fluxes, sources and the Rusanov face
+// flux of the set2c compressible Euler equations (2D/3D) and of the
+// shallow-water equations, in FP64 with the reference's evaluation order.
+// The translation units that include this header are compiled with
+// -ffp-contract=off, so a*b+c stays two correctly rounded operations.
+//
+// Reference formulas (same association, left to right):
+//   3D Euler volume  examples/3d_kernels/euler3d_set2c.jl:175-215
+//   3D Euler face    examples/3d_kernels/euler3d_set2c.jl:262-339
+//   2D Euler volume  examples/2d_kernels/euler2d_set2c.jl:163-184
+//   2D Euler face    examples/2d_kernels/euler2d_set2c.jl:238-292
+//   SWE volume       examples/old_fxg_files/shallow_water.jl:140-160
+//   SWE face         examples/old_fxg_files/shallow_water.jl:196-255
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace canary {
+
+struct PhysParams {
+  double gamma, p0, R, cp, cv, g, kappa;  // kappa = cp / cv
+  double swe_g, dnl;
+};
+
+// Julia's max(a, b) for Float64: NaN if either is NaN, else the larger.
+__device__ __forceinline__ double jmax(double a, double b) {
+  if (isnan(a) || isnan(b)) return a + b;
+  return b > a ? b : a;
+}
+
+// ------------------------------------------------------------ 3D Euler set2c
+struct Euler3D {
+  static constexpr int DIM = 3, NS = 5, NSR = 5, NVGEO = 14, NSGEO = 5;
+  static constexpr int iMJ = 9, iMJI = 10;
+  static constexpr bool HAS_P = true, B_FROM_M = false, HAS_ADVECTION = false;
+  enum { U = 0, V = 1, W = 2, RHO = 3, E = 4 };
+
+  __device__ static double pressure(const double* q, const PhysParams& p) {
+    return p.p0 * pow(p.R * q[E] / p.p0, p.kappa);
+  }
+  // met[d + DIM*c] = d(xi_d)/d(x_c) in the vgeo field order.
+  __device__ static void volume_flux(const double* q, double P, const double* met, double MJ,
+                                     const PhysParams&, double F[DIM][NSR]) {
+    const double Uq = q[U], Vq = q[V], Wq = q[W], rho = q[RHO], Eq = q[E];
+    const double ri = 1 / rho;
+    const double frx = Uq, fUx = ri * Uq * Uq + P, fVx = ri * Vq * Uq, fWx = ri * Wq * Uq, fEx = Eq * ri * Uq;
+    const double fry = Vq, fUy = ri * Uq * Vq, fVy = ri * Vq * Vq + P, fWy = ri * Wq * Vq, fEy = Eq * ri * Vq;
+    const double frz = Wq, fUz = ri * Uq * Wq, fVz = ri * Vq * Wq, fWz = ri * Wq * Wq + P, fEz = Eq * ri * Wq;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const double mx = met[d], my = met[d + 3], mz = met[d + 6];
+      F[d][RHO] = MJ * (mx * frx + my * fry + mz * frz);
+      F[d][U] = MJ * (mx * fUx + my * fUy + mz * fUz);
+      F[d][V] = MJ * (mx * fVx + my * fVy + mz * fVz);
+      F[d][W] = MJ * (mx * fWx + my * fWy + mz * fWz);
+      F[d][E] = MJ * (mx * fEx + my * fEy + mz * fEz);
+    }
+  }
+  __device__ static void source(const double* q, double MJ, const PhysParams& p, double* r) {
+    r[W] -= MJ * q[RHO] * p.g;
+  }
+  // Wall state: momentum reflected, rho, rho*theta and P copied.
+  __device__ static void reflect(const double* qM, double PM, const double* n, double* qP, double& PP) {
+    const double UnM = n[0] * qM[U] + n[1] * qM[V] + n[2] * qM[W];
+    qP[U] = qM[U] - 2 * UnM * n[0];
+    qP[V] = qM[V] - 2 * UnM * n[1];
+    qP[W] = qM[W] - 2 * UnM * n[2];
+    qP[RHO] = qM[RHO];
+    qP[E] = qM[E];
+    PP = PM;
+  }
+  // out[s] = sMJ * (Rusanov flux)_s, the amount the reference subtracts.
+  __device__ static void face_flux(const double* qM, double PM, const double* qP, double PP,
+                                   const double* n, double sMJ, const PhysParams& p, double* out) {
+    const double nx = n[0], ny = n[1], nz = n[2];
+    const double UM = qM[U], VM = qM[V], WM = qM[W], rM = qM[RHO], EM = qM[E];
+    const double UP = qP[U], VP = qP[V], WP = qP[W], rP = qP[RHO], EP = qP[E];
+    const double rMi = 1 / rM;
+    const double frMx = UM, fUMx = rMi * UM * UM + PM, fVMx = rMi * VM * UM, fWMx = rMi * WM * UM, fEMx = rMi * UM * EM;
+    const double frMy = VM, fUMy = rMi * UM * VM, fVMy = rMi * VM * VM + PM, fWMy = rMi * WM * VM, fEMy = rMi * VM * EM;
+    const double frMz = WM, fUMz = rMi * UM * WM, fVMz = rMi * VM * WM, fWMz = rMi * WM * WM + PM, fEMz = rMi * WM * EM;
+    const double rPi = 1 / rP;
+    const double frPx = UP, fUPx = rPi * UP * UP + PP, fVPx = rPi * VP * UP, fWPx = rPi * WP * UP, fEPx = rPi * UP * EP;
+    const double frPy = VP, fUPy = rPi * UP * VP, fVPy = rPi * VP * VP + PP, fWPy = rPi * WP * VP, fEPy = rPi * VP * EP;
+    const double frPz = WP, fUPz = rPi * UP * WP, fVPz = rPi * VP * WP, fWPz = rPi * WP * WP + PP, fEPz = rPi * WP * EP;
+    const double lM = rMi * fabs(nx * UM + ny * VM + nz * WM) + sqrt(rMi * p.gamma * PM);
+    const double lP = rPi * fabs(nx * UP + ny * VP + nz * WP) + sqrt(rPi * p.gamma * PP);
+    const double l = jmax(lM, lP);
+    out[RHO] = sMJ * ((nx * (frMx + frPx) + ny * (frMy + frPy) + nz * (frMz + frPz) - l * (rP - rM)) / 2);
+    out[U] = sMJ * ((nx * (fUMx + fUPx) + ny * (fUMy + fUPy) + nz * (fUMz + fUPz) - l * (UP - UM)) / 2);
+    out[V] = sMJ * ((nx * (fVMx + fVPx) + ny * (fVMy + fVPy) + nz * (fVMz + fVPz) - l * (VP - VM)) / 2);
+    out[W] = sMJ * ((nx * (fWMx + fWPx) + ny * (fWMy + fWPy) + nz * (fWMz + fWPz) - l * (WP - WM)) / 2);
+    out[E] = sMJ * ((nx * (fEMx + fEPx) + ny * (fEMy + fEPy) + nz * (fEMz + fEPz) - l * (EP - EM)) / 2);
+  }
+};
+
+// ------------------------------------------------------------ 2D Euler set2c
+struct Euler2D {
+  static constexpr int DIM = 2, NS = 4, NSR = 4, NVGEO = 8, NSGEO = 4;
+  static constexpr int iMJ = 4, iMJI = 5;
+  static constexpr bool HAS_P = true, B_FROM_M = false, HAS_ADVECTION = false;
+  enum { U = 0, V = 1, RHO = 2, E = 3 };
+
+  __device__ static double pressure(const double* q, const PhysParams& p) {
+    return p.p0 * pow(p.R * q[E] / p.p0, p.kappa);
+  }
+  __device__ static void volume_flux(const double* q, double P, const double* met, double MJ,
+                                     const PhysParams&, double F[DIM][NSR]) {
+    const double Uq = q[U], Vq = q[V], rho = q[RHO], Eq = q[E];
+    const double ri = 1 / rho;
+    const double frx = Uq, fUx = ri * Uq * Uq + P, fVx = ri * Vq * Uq, fEx = ri * Uq * Eq;
+    const double fry = Vq, fUy = ri * Uq * Vq, fVy = ri * Vq * Vq + P, fEy = ri * Vq * Eq;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const double mx = met[d], my = met[d + 2];
+      F[d][RHO] = MJ * (mx * frx + my * fry);
+      F[d][U] = MJ * (mx * fUx + my * fUy);
+      F[d][V] = MJ * (mx * fVx + my * fVy);
+      F[d][E] = MJ * (mx * fEx + my * fEy);
+    }
+  }
+  __device__ static void source(const double* q, double MJ, const PhysParams& p, double* r) {
+    r[V] -= MJ * q[RHO] * p.g;
+  }
+  __device__ static void reflect(const double* qM, double PM, const double* n, double* qP, double& PP) {
+    const double UnM = n[0] * qM[U] + n[1] * qM[V];
+    qP[U] = qM[U] - 2 * UnM * n[0];
+    qP[V] = qM[V] - 2 * UnM * n[1];
+    qP[RHO] = qM[RHO];
+    qP[E] = qM[E];
+    PP = PM;
+  }
+  __device__ static void face_flux(const double* qM, double PM, const double* qP, double PP,
+                                   const double* n, double sMJ, const PhysParams& p, double* out) {
+    const double nx = n[0], ny = n[1];
+    const double UM = qM[U], VM = qM[V], rM = qM[RHO], EM = qM[E];
+    const double UP = qP[U], VP = qP[V], rP = qP[RHO], EP = qP[E];
+    const double rMi = 1 / rM;
+    const double frMx = UM, fUMx = rMi * UM * UM + PM, fVMx = rMi * VM * UM, fEMx = rMi * UM * EM;
+    const double frMy = VM, fUMy = rMi * UM * VM, fVMy = rMi * VM * VM + PM, fEMy = rMi * VM * EM;
+    const double rPi = 1 / rP;
+    const double frPx = UP, fUPx = rPi * UP * UP + PP, fVPx = rPi * VP * UP, fEPx = rPi * UP * EP;
+    const double frPy = VP, fUPy = rPi * UP * VP, fVPy = rPi * VP * VP + PP, fEPy = rPi * VP * EP;
+    const double lM = rMi * fabs(nx * UM + ny * VM) + sqrt(rMi * p.gamma * PM);
+    const double lP = rPi * fabs(nx * UP + ny * VP) + sqrt(rPi * p.gamma * PP);
+    const double l = jmax(lM, lP);
+    out[RHO] = sMJ * ((nx * (frMx + frPx) + ny * (frMy + frPy) + -(l * (rP - rM))) / 2);
+    out[U] = sMJ * ((nx * (fUMx + fUPx) + ny * (fUMy + fUPy) + -(l * (UP - UM))) / 2);
+    out[V] = sMJ * ((nx * (fVMx + fVPx) + ny * (fVMy + fVPy) + -(l * (VP - VM))) / 2);
+    out[E] = sMJ * ((nx * (fEMx + fEPx) + ny * (fEMy + fEPy) + -(l * (EP - EM))) / 2);
+  }
+};
+
+// ------------------------------------------------------------ shallow water
+// Q = (H u, H v, h, b); b (bathymetry / still depth) has no tendency.
+struct SWE2D {
+  static constexpr int DIM = 2, NS = 4, NSR = 3, NVGEO = 8, NSGEO = 4;
+  static constexpr int iMJ = 4, iMJI = 5;
+  // B_FROM_M: the reference face kernel reads the P-side b from element eM
+  // (shallow_water.jl:202); kept for parity, harmless for a continuous b.
+  static constexpr bool HAS_P = false, B_FROM_M = true, HAS_ADVECTION = true;
+  enum { U = 0, V = 1, H = 2, B = 3 };
+
+  __device__ static double pressure(const double*, const PhysParams&) { return 0.0; }
+  __device__ static void volume_flux(const double* q, double, const double* met, double MJ,
+                                     const PhysParams& p, double F[DIM][NSR]) {
+    const double g = p.swe_g, dnl = p.dnl;
+    const double Uq = q[U], Vq = q[V], h = q[H], b = q[B];
+    const double Ht = h + b;
+    const double u = Uq / Ht, v = Vq / Ht;
+    const double fhx = Uq, fhy = Vq;
+    const double fUx = (Ht * u * u + 0.5 * g * (h * h)) * dnl + g * h * b;
+    const double fUy = (Ht * u * v) * dnl;
+    const double fVx = (Ht * v * u) * dnl;
+    const double fVy = (Ht * v * v + 0.5 * g * (h * h)) * dnl + g * h * b;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const double mx = met[d], my = met[d + 2];
+      F[d][H] = MJ * (mx * fhx + my * fhy);
+      F[d][U] = MJ * (mx * fUx + my * fUy);
+      F[d][V] = MJ * (mx * fVx + my * fVy);
+    }
+  }
+  __device__ static void source(const double*, double, const PhysParams&, double*) {}
+  __device__ static void reflect(const double* qM, double, const double* n, double* qP, double& PP) {
+    const double UnM = n[0] * qM[U] + n[1] * qM[V];
+    qP[U] = qM[U] - 2 * UnM * n[0];
+    qP[V] = qM[V] - 2 * UnM * n[1];
+    qP[H] = qM[H];
+    qP[B] = qM[B];
+    PP = 0.0;
+  }
+  __device__ static void face_flux(const double* qM, double, const double* qP, double,
+                                   const double* n, double sMJ, const PhysParams& p, double* out) {
+    const double g = p.swe_g, dnl = p.dnl;
+    const double nx = n[0], ny = n[1];
+    const double hM = qM[H], bM = qM[B], UM = qM[U], VM = qM[V];
+    const double hP = qP[H], bP = qP[B], UP = qP[U], VP = qP[V];
+    const double HM = hM + bM, uM = UM / HM, vM = VM / HM;
+    const double HP = hP + bP, uP = UP / HP, vP = VP / HP;
+    const double fhMx = UM, fhMy = VM;
+    const double fUMx = (HM * uM * uM + 0.5 * g * (hM * hM)) * dnl + g * hM * bM;
+    const double fUMy = HM * uM * vM * dnl;
+    const double fVMx = HM * vM * uM * dnl;
+    const double fVMy = (HM * vM * vM + 0.5 * g * (hM * hM)) * dnl + g * hM * bM;
+    const double fhPx = UP, fhPy = VP;
+    const double fUPx = (HP * uP * uP + 0.5 * g * (hP * hP)) * dnl + g * hP * bP;
+    const double fUPy = HP * uP * vP * dnl;
+    const double fVPx = HP * vP * uP * dnl;
+    const double fVPy = (HP * vP * vP + 0.5 * g * (hP * hP)) * dnl + g * hP * bP;
+    const double lM = (fabs(nx * uM + ny * vM) + sqrt(g * HM)) * dnl + (sqrt(g * bM)) * (1.0 - dnl);
+    const double lP = (fabs(nx * uP + ny * vP) + sqrt(g * HP)) * dnl + (sqrt(g * bP)) * (1.0 - dnl);
+    const double l = jmax(lM, lP);
+    out[H] = sMJ * ((nx * (fhMx + fhPx) + ny * (fhMy + fhPy) + -(l * (hP - hM))) / 2);
+    out[U] = sMJ * ((nx * (fUMx + fUPx) + ny * (fUMy + fUPy) + -(l * (UP - UM))) / 2);
+    out[V] = sMJ * ((nx * (fVMx + fVPx) + ny * (fVMy + fVPy) + -(l * (VP - VM))) / 2);
+  }
+};
+
+}  // namespace canary

@@ -1,0 +1,184 @@
+// Public C ABI of the device path: argument checks, (equation, dim)
+// dispatch to the kernel translation units, device error reporting.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../../include/canary_dg.h"
+#include "launch.h"
+
+extern "C" {
+int canary_k_euler3d(int op, int N, const canary::KArgs* a);
+int canary_k_euler2d(int op, int N, const canary::KArgs* a);
+int canary_k_swe2d(int op, int N, const canary::KArgs* a);
+int canary_k_euler3d_status(int64_t* info, int clear);
+int canary_k_euler2d_status(int64_t* info, int clear);
+int canary_k_swe2d_status(int64_t* info, int clear);
+}
+
+namespace {
+
+using KFn = int (*)(int, int, const canary::KArgs*);
+
+KFn select(int eqn, int dim) {
+  if (eqn == CANARY_EULER && dim == 3) return canary_k_euler3d;
+  if (eqn == CANARY_EULER && dim == 2) return canary_k_euler2d;
+  if (eqn == CANARY_SWE && dim == 2) return canary_k_swe2d;
+  return nullptr;
+}
+
+canary::PhysParams to_params(const canary_phys* ph) {
+  canary_phys d;
+  if (!ph) {
+    canary_phys_default(&d);
+    ph = &d;
+  }
+  canary::PhysParams p;
+  p.gamma = ph->gamma;
+  p.p0 = ph->p0;
+  p.R = ph->R_gas;
+  p.cp = ph->c_p;
+  p.cv = ph->c_v;
+  p.g = ph->gravity;
+  p.kappa = ph->c_p / ph->c_v;
+  p.swe_g = ph->swe_gravity;
+  p.dnl = ph->delta_nl;
+  return p;
+}
+
+int check_common(int eqn, int dim, int N, int prec, int64_t nreal, KFn* fn) {
+  *fn = select(eqn, dim);
+  if (!*fn) return CANARY_ERR_UNSUPPORTED;
+  if (N < 1 || N > canary::kMaxN) return CANARY_ERR_UNSUPPORTED;
+  if (prec != CANARY_FP64) return CANARY_ERR_UNSUPPORTED;
+  if (nreal < 0) return CANARY_ERR_ARG;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int canary_volumerhs(int eqn, int dim, int N, int prec, void* rhs, const void* Q,
+                                const void* vgeo, const double* D, int64_t nreal,
+                                const canary_phys* phys, void* stream) {
+  KFn fn;
+  int rc = check_common(eqn, dim, N, prec, nreal, &fn);
+  if (rc) return rc;
+  if (nreal == 0) return 0;
+  if (!rhs || !Q || !vgeo || !D) return CANARY_ERR_ARG;
+  canary::KArgs a = {};
+  a.rhs = rhs;
+  a.Q = Q;
+  a.vgeo = vgeo;
+  a.D = D;
+  a.nreal = nreal;
+  a.ntotal = nreal;
+  a.phys = to_params(phys);
+  a.stream = (hipStream_t)stream;
+  return fn(canary::OP_VOLUME, N, &a);
+}
+
+extern "C" int canary_facerhs(int eqn, int dim, int N, int prec, void* rhs, const void* Q,
+                              const void* sgeo, int64_t nreal, int64_t nelem_total,
+                              const int64_t* vmapM, const int64_t* vmapP,
+                              const int64_t* elemtobndy, const canary_phys* phys, void* stream) {
+  KFn fn;
+  int rc = check_common(eqn, dim, N, prec, nreal, &fn);
+  if (rc) return rc;
+  if (nreal == 0) return 0;
+  if (nelem_total < nreal) return CANARY_ERR_ARG;
+  if (!rhs || !Q || !sgeo || !vmapM || !vmapP || !elemtobndy) return CANARY_ERR_ARG;
+  canary::KArgs a = {};
+  a.rhs = rhs;
+  a.Q = Q;
+  a.sgeo = sgeo;
+  a.nreal = nreal;
+  a.ntotal = nelem_total;
+  a.vmapM = vmapM;
+  a.vmapP = vmapP;
+  a.bndy = elemtobndy;
+  a.phys = to_params(phys);
+  a.stream = (hipStream_t)stream;
+  return fn(canary::OP_FACE, N, &a);
+}
+
+extern "C" int canary_updatesolution(int eqn, int dim, int N, int prec, void* rhs, void* Q,
+                                     const void* vgeo, int64_t nreal, double rka, double rkb,
+                                     double dt, int advection, void* stream) {
+  KFn fn;
+  int rc = check_common(eqn, dim, N, prec, nreal, &fn);
+  if (rc) return rc;
+  if (nreal == 0) return 0;
+  if (!rhs || !Q || !vgeo) return CANARY_ERR_ARG;
+  if (advection && eqn != CANARY_SWE) return CANARY_ERR_ARG;
+  canary::KArgs a = {};
+  a.rhs = rhs;
+  a.Q = Q;
+  a.Qout = Q;
+  a.vgeo = vgeo;
+  a.nreal = nreal;
+  a.ntotal = nreal;
+  a.rka = rka;
+  a.rkb = rkb;
+  a.dt = dt;
+  a.advection = advection;
+  a.stream = (hipStream_t)stream;
+  return fn(canary::OP_UPDATE, N, &a);
+}
+
+extern "C" int canary_stage_fused(int eqn, int dim, int N, int prec, void* rhs, const void* Qin,
+                                  void* Qout, const void* vgeo, const void* sgeo,
+                                  const double* D, int64_t nreal, int64_t nelem_total,
+                                  const int64_t* vmapM, const int64_t* vmapP,
+                                  const int64_t* elemtobndy, double rka, double rkb, double dt,
+                                  const canary_phys* phys, void* stream) {
+  KFn fn;
+  int rc = check_common(eqn, dim, N, prec, nreal, &fn);
+  if (rc) return rc;
+  if (nreal == 0) return 0;
+  if (nelem_total < nreal) return CANARY_ERR_ARG;
+  if (!rhs || !Qin || !Qout || !vgeo || !sgeo || !D || !vmapM || !vmapP || !elemtobndy)
+    return CANARY_ERR_ARG;
+  if (Qin == (const void*)Qout) return CANARY_ERR_ARG;
+  canary::KArgs a = {};
+  a.rhs = rhs;
+  a.Q = Qin;
+  a.Qout = Qout;
+  a.vgeo = vgeo;
+  a.sgeo = sgeo;
+  a.D = D;
+  a.nreal = nreal;
+  a.ntotal = nelem_total;
+  a.vmapM = vmapM;
+  a.vmapP = vmapP;
+  a.bndy = elemtobndy;
+  a.rka = rka;
+  a.rkb = rkb;
+  a.dt = dt;
+  a.phys = to_params(phys);
+  a.stream = (hipStream_t)stream;
+  return fn(canary::OP_STAGE, N, &a);
+}
+
+extern "C" int canary_check_errors(void* stream, int64_t* info) {
+  if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return CANARY_ERR_LAUNCH;
+  int64_t tmp[4];
+  int first = 0;
+  int (*readers[3])(int64_t*, int) = {canary_k_euler3d_status, canary_k_euler2d_status,
+                                      canary_k_swe2d_status};
+  for (auto rd : readers) {
+    int rc = rd(tmp, 1);
+    if (rc && !first) {
+      first = rc;
+      if (info)
+        for (int i = 0; i < 4; ++i) info[i] = tmp[i];
+    }
+  }
+  if (!first && info)
+    for (int i = 0; i < 4; ++i) info[i] = 0;
+  return first;
+}
+
+extern "C" int canary_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}

@@ -1,0 +1,141 @@
+"""Device kernels with the reference call surface (euler3d_set2c.jl:713-734):
+
+    volumerhs(dim, N, rhs, Q, vgeo, D, elems)
+    facerhs(dim, N, rhs, Q, sgeo, elems, vmapM, vmapP, elemtobndy)
+    updatesolution(dim, N, rhs, Q, vgeo, elems, rka, rkb, dt)
+
+plus the fused single-pass stage.  Arrays are torch tensors on the GPU in
+the reference layout (a C-order tensor of shape (nelem, nstate, Np) is
+Julia's (Np, nstate, nelem)).  Every call goes through the C ABI of
+libcanary_dg.so; there is no other code path.  Shapes are checked on the
+host before any launch.
+"""
+import ctypes as C
+
+import torch
+
+from . import _lib
+from ._lib import EULER, SWE, FP64, check, lib
+
+_NS = {(EULER, 3): 5, (EULER, 2): 4, (SWE, 2): 4}
+_NV = {3: 14, 2: 8}
+_NSG = {3: 5, 2: 4}
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    s = torch.cuda.current_stream() if stream is None else stream
+    return C.c_void_p(s.cuda_stream)
+
+
+def _nreal(elems):
+    if isinstance(elems, range):
+        if elems.start != 1 or elems.step != 1:
+            raise ValueError("elems must be 1:nreal (the device methods use length(elems))")
+        return len(elems)
+    return int(elems)
+
+
+def _need(t, name, numel, dtype=torch.float64):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError("%s must be a CUDA tensor" % name)
+    if t.dtype != dtype:
+        raise TypeError("%s must be %s, got %s" % (name, dtype, t.dtype))
+    if not t.is_contiguous():
+        raise ValueError("%s must be contiguous" % name)
+    if t.numel() < numel:
+        raise ValueError("%s has %d elements, needs at least %d" % (name, t.numel(), numel))
+
+
+def _phys(phys):
+    return C.byref(phys) if phys is not None else C.byref(_lib.default_phys())
+
+
+def volumerhs(dim, N, rhs, Q, vgeo, D, elems, eqn=EULER, phys=None, stream=None):
+    """volumerhs!(Val(dim), Val(N), rhs, Q, vgeo, D, elems): rhs += volume term."""
+    n = _nreal(elems)
+    Np = (N + 1) ** dim
+    ns = _NS[(eqn, dim)]
+    _need(rhs, "rhs", Np * ns * n)
+    _need(Q, "Q", Np * ns * n)
+    _need(vgeo, "vgeo", Np * _NV[dim] * n)
+    _need(D, "D", (N + 1) ** 2)
+    rc = lib().canary_volumerhs(eqn, dim, N, FP64, _ptr(rhs), _ptr(Q), _ptr(vgeo), _ptr(D), n,
+                                _phys(phys), _stream(stream))
+    check(rc, "volumerhs")
+
+
+def facerhs(dim, N, rhs, Q, sgeo, elems, vmapM, vmapP, elemtobndy, eqn=EULER, phys=None,
+            nelem_total=None, stream=None):
+    """facerhs!(Val(dim), Val(N), rhs, Q, sgeo, elems, vmapM, vmapP, elemtobndy)."""
+    n = _nreal(elems)
+    Np, Nfp, nf = (N + 1) ** dim, (N + 1) ** (dim - 1), 2 * dim
+    ns = _NS[(eqn, dim)]
+    ntot = n if nelem_total is None else int(nelem_total)
+    _need(rhs, "rhs", Np * ns * n)
+    _need(Q, "Q", Np * ns * ntot)
+    _need(sgeo, "sgeo", _NSG[dim] * Nfp * nf * n)
+    _need(vmapM, "vmapM", Nfp * nf * n, torch.int64)
+    _need(vmapP, "vmapP", Nfp * nf * n, torch.int64)
+    _need(elemtobndy, "elemtobndy", nf * n, torch.int64)
+    rc = lib().canary_facerhs(eqn, dim, N, FP64, _ptr(rhs), _ptr(Q), _ptr(sgeo), n, ntot,
+                              _ptr(vmapM), _ptr(vmapP), _ptr(elemtobndy), _phys(phys),
+                              _stream(stream))
+    check(rc, "facerhs")
+
+
+fluxrhs = facerhs  # the shallow-water example's name for the face kernel
+
+
+def updatesolution(dim, N, rhs, Q, vgeo, elems, rka, rkb, dt, eqn=EULER, advection=False,
+                   stream=None):
+    """updatesolution!(Val(dim), Val(N), rhs, Q, vgeo, elems, rka, rkb, dt)."""
+    n = _nreal(elems)
+    Np = (N + 1) ** dim
+    ns = _NS[(eqn, dim)]
+    _need(rhs, "rhs", Np * ns * n)
+    _need(Q, "Q", Np * ns * n)
+    _need(vgeo, "vgeo", Np * _NV[dim] * n)
+    rc = lib().canary_updatesolution(eqn, dim, N, FP64, _ptr(rhs), _ptr(Q), _ptr(vgeo), n,
+                                     float(rka), float(rkb), float(dt), 1 if advection else 0,
+                                     _stream(stream))
+    check(rc, "updatesolution")
+
+
+def stage_fused(dim, N, rhs, Qin, Qout, vgeo, sgeo, D, elems, vmapM, vmapP, elemtobndy,
+                rka, rkb, dt, eqn=EULER, phys=None, nelem_total=None, stream=None):
+    """One LSRK stage (volume + faces + update) in one pass; Qin -> Qout."""
+    n = _nreal(elems)
+    Np, Nfp, nf = (N + 1) ** dim, (N + 1) ** (dim - 1), 2 * dim
+    ns = _NS[(eqn, dim)]
+    ntot = n if nelem_total is None else int(nelem_total)
+    _need(rhs, "rhs", Np * ns * n)
+    _need(Qin, "Qin", Np * ns * ntot)
+    _need(Qout, "Qout", Np * ns * n)
+    _need(vgeo, "vgeo", Np * _NV[dim] * n)
+    _need(sgeo, "sgeo", _NSG[dim] * Nfp * nf * n)
+    _need(D, "D", (N + 1) ** 2)
+    _need(vmapM, "vmapM", Nfp * nf * n, torch.int64)
+    _need(vmapP, "vmapP", Nfp * nf * n, torch.int64)
+    _need(elemtobndy, "elemtobndy", nf * n, torch.int64)
+    if Qin.data_ptr() == Qout.data_ptr():
+        raise ValueError("stage_fused needs distinct Qin and Qout")
+    rc = lib().canary_stage_fused(eqn, dim, N, FP64, _ptr(rhs), _ptr(Qin), _ptr(Qout),
+                                  _ptr(vgeo), _ptr(sgeo), _ptr(D), n, ntot, _ptr(vmapM),
+                                  _ptr(vmapP), _ptr(elemtobndy), float(rka), float(rkb),
+                                  float(dt), _phys(phys), _stream(stream))
+    check(rc, "stage_fused")
+
+
+def check_errors(stream=None):
+    """Synchronise and raise if a kernel recorded a bad bc / vmap index."""
+    info = (C.c_int64 * 4)()
+    rc = lib().canary_check_errors(_stream(stream), info)
+    if rc:
+        err = _lib.CanaryError(rc, "device kernel (face %d, element %d, value %d)"
+                               % (info[1], info[2], info[3]))
+        err.info = tuple(info)
+        raise err

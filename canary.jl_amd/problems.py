@@ -1,0 +1,204 @@
+"""Problem setup: mesh -> maps -> geometry -> initial state, as the reference
+drivers do it (euler3d_set2c.jl:1049-1110, euler2d_set2c.jl:924-984,
+shallow_water.jl:751-800), with every physical constant a parameter.
+
+Everything here is host-side NumPy in the reference's column-major layout.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from . import mesh as M
+from ._lib import check, lib
+
+NSTATE = {(_lib.EULER, 3): 5, (_lib.EULER, 2): 4, (_lib.SWE, 2): 4}
+NVGEO = {3: 14, 2: 8}
+NSGEO = {3: 5, 2: 4}
+IMJ = {3: 9, 2: 4}
+IMJI = {3: 10, 2: 5}
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def linrange(a, b, n):
+    """Julia range(a, stop=b, length=n) for the endpoints used here:
+    element i is the correctly rounded (a*(n-1-i) + b*i)/(n-1) whenever the
+    numerator is exact (integer endpoints)."""
+    return np.array([(a * (n - 1 - i) + b * i) / (n - 1) for i in range(n)], dtype=np.float64)
+
+
+@dataclass
+class Setup:
+    eqn: int
+    dim: int
+    N: int
+    nelem: int
+    D: np.ndarray
+    r: np.ndarray
+    w: np.ndarray
+    vgeo: np.ndarray
+    sgeo: np.ndarray
+    vmapM: np.ndarray
+    vmapP: np.ndarray
+    elemtobndy: np.ndarray
+    mesh: dict = field(repr=False)
+    phys: object = None
+    Q: np.ndarray = None
+    label: str = ""
+
+    @property
+    def Nq(self):
+        return self.N + 1
+
+    @property
+    def Np(self):
+        return (self.N + 1) ** self.dim
+
+    @property
+    def nstate(self):
+        return NSTATE[(self.eqn, self.dim)]
+
+    @property
+    def nvgeo(self):
+        return NVGEO[self.dim]
+
+    def zeros_state(self):
+        return np.zeros((self.Np, self.nstate, self.nelem), order="F")
+
+    def coords(self):
+        """Node coordinates (Np, nelem) per dimension."""
+        ix = 11 if self.dim == 3 else 6
+        return tuple(self.vgeo[:, ix + c, :] for c in range(self.dim))
+
+    def ndof(self):
+        return self.nelem * self.Np * self.nstate
+
+
+def build(eqn, dim, N, x, periodic, boundary=None, phys=None, label=""):
+    """brickmesh -> partition -> connectmesh -> mappings -> lglpoints /
+    spectralderivative -> computegeometry (one rank)."""
+    etv, etc, etb, fcs = M.brickmesh(x, periodic, boundary=boundary)
+    etv, etc, etb, fcs = M.partition(etv, etc, etb, fcs)
+    mesh = M.connectmesh(etv, etc, etb, fcs)
+    vmapM, vmapP = M.mappings(N, mesh["elemtoelem"], mesh["elemtoface"], mesh["elemtoordr"])
+    r, w = M.lglpoints(N)
+    D = M.spectralderivative(r)
+    vgeo, sgeo = M.computegeometry(dim, N, mesh["elemtocoord"], vmapM)
+    return Setup(eqn=eqn, dim=dim, N=N, nelem=etc.shape[2], D=D, r=r, w=w, vgeo=vgeo, sgeo=sgeo,
+                 vmapM=vmapM, vmapP=vmapP, elemtobndy=mesh["elemtobndy"], mesh=mesh,
+                 phys=phys if phys is not None else _lib.default_phys(), label=label)
+
+
+# ------------------------------------------------------------- initial data
+def bubble_state(s, theta0=300.0, dtheta=0.5, rc=250.0, center=(500.0, 500.0, 350.0),
+                 spherical=False, jitter=0.0, seed=0):
+    b = _lib.Bubble(theta0=theta0, dtheta=dtheta, rc=rc, xc=center[0], yc=center[1],
+                    zc=center[2] if s.dim == 3 else center[-1], spherical=1 if spherical else 0,
+                    jitter=jitter, seed=seed)
+    Q = s.zeros_state()
+    check(lib().canary_ic_bubble(s.dim, s.N, s.nelem, _p(s.vgeo), C.byref(b), C.byref(s.phys),
+                                 _p(Q)), "ic_bubble")
+    return Q
+
+
+def vortex_params(beta=5.0, x0=0.0, y0=0.0, u0=1.0, v0=1.0, rho_inf=1.0, p_inf=1.0, gamma=1.4,
+                  Lx=10.0, Ly=10.0):
+    return _lib.Vortex(beta=beta, x0=x0, y0=y0, u0=u0, v0=v0, rho_inf=rho_inf, p_inf=p_inf,
+                       gamma=gamma, Lx=Lx, Ly=Ly)
+
+
+def vortex_state(s, t=0.0, params=None):
+    params = params or vortex_params()
+    Q = s.zeros_state()
+    check(lib().canary_ic_vortex2d(s.N, s.nelem, _p(s.vgeo), C.byref(params), float(t), _p(Q)),
+          "ic_vortex2d")
+    return Q
+
+
+def swe_state(s, bumps, depth=10.0, L=(1.0, 1.0)):
+    bumps = np.ascontiguousarray(np.asarray(bumps, dtype=np.float64).reshape(-1, 4))
+    Q = s.zeros_state()
+    check(lib().canary_ic_swe_bumps(s.N, s.nelem, _p(s.vgeo), float(depth), len(bumps),
+                                    _p(bumps), float(L[0]), float(L[1]), _p(Q)), "ic_swe_bumps")
+    return Q
+
+
+def cfl(s, Q):
+    """Reference cfl() value (before the `/ N^sqrt(2)`)."""
+    Qf = np.asfortranarray(Q)
+    return lib().canary_cfl(s.eqn, s.dim, s.N, _p(s.vgeo), _p(Qf), s.nelem, C.byref(s.phys))
+
+
+def l2energysquared(s, Q):
+    Qf = np.asfortranarray(Q)
+    return lib().canary_l2energysquared(s.Np, s.nstate, s.nvgeo, IMJ[s.dim], _p(Qf), _p(s.vgeo),
+                                        s.nelem)
+
+
+# -------------------------------------------------------- named workloads
+def bubble3d(Ne=32, N=4, L=1000.0, gravity=9.81, dtheta=2.0, jitter=1e-3, seed=1,
+             walls=True, spherical=True):
+    """3D rising thermal bubble: [0,L]^3, Ne^3 hexes, no-flux walls, g,
+    theta0 = 300 K, +dtheta K cosine bubble of radius 250 m at (500,500,350),
+    seeded theta jitter."""
+    phys = _lib.default_phys(gravity=gravity)
+    x = linrange(0.0, L, Ne + 1)
+    per = (False, False, False) if walls else (True, True, False)
+    s = build(_lib.EULER, 3, N, (x, x, x), per, phys=phys,
+              label="euler3d bubble %d^3 N=%d" % (Ne, N))
+    s.Q = bubble_state(s, dtheta=dtheta, spherical=spherical, jitter=jitter, seed=seed)
+    return s
+
+
+def bubble2d(Ne=10, N=4, L=1000.0, gravity=10.0, dtheta=0.5, jitter=0.0, seed=1):
+    """2D bubble of the reference main() (euler2d_set2c.jl:1046-1118)."""
+    phys = _lib.default_phys(gravity=gravity)
+    x = linrange(0.0, L, Ne + 1)
+    s = build(_lib.EULER, 2, N, (x, x), (True, False), phys=phys,
+              label="euler2d bubble %d^2 N=%d" % (Ne, N))
+    s.Q = bubble_state(s, dtheta=dtheta, center=(500.0, 350.0), jitter=jitter, seed=seed)
+    return s
+
+
+def vortex_phys(gamma=1.4):
+    """set2c constants for which P = (rho theta)^gamma: p0 = R = 1, c_p/c_v = gamma, g = 0."""
+    return _lib.default_phys(gamma=gamma, p0=1.0, R_gas=1.0, c_p=gamma, c_v=1.0, gravity=0.0)
+
+
+def vortex2d(Ne=10, N=4, half=5.0, params=None):
+    """Isentropic vortex on the periodic box [-half, half]^2."""
+    x = linrange(-half, half, Ne + 1)
+    params = params or vortex_params(Lx=2 * half, Ly=2 * half)
+    s = build(_lib.EULER, 2, N, (x, x), (True, True), phys=vortex_phys(params.gamma),
+              label="euler2d vortex %d^2 N=%d" % (Ne, N))
+    s.Q = vortex_state(s, 0.0, params)
+    s.vortex = params
+    return s
+
+
+def swe2d(Ne=10, N=4, depth=10.0, bumps=((0.5, 0.5, 0.05, 0.1),), periodic=True,
+          gravity=10.0):
+    """Shallow water at rest on [0,1]^2 with Gaussian bumps over depth b."""
+    phys = _lib.default_phys(swe_gravity=gravity, delta_nl=1.0)
+    x = linrange(0.0, 1.0, Ne + 1)
+    per = (True, True) if periodic else (False, False)
+    s = build(_lib.SWE, 2, N, (x, x), per, phys=phys, label="swe2d %d^2 N=%d" % (Ne, N))
+    s.Q = swe_state(s, bumps, depth=depth)
+    return s
+
+
+def rk_coefs():
+    """LSRK54 (Carpenter & Kennedy 1994) as DFloat(num)/DFloat(den):
+    euler3d_set2c.jl:772-788."""
+    A = (0.0, -567301805773 / 1357537059087, -2404267990393 / 2016746695238,
+         -3550918686646 / 2091501179385, -1275806237668 / 842570457699)
+    B = (1432997174477 / 9575080441755, 5161836677717 / 13612068292357,
+         1720146321549 / 2090206949498, 3134564353537 / 4481467310338,
+         2277821191437 / 14882151754819)
+    Cc = (0.0, 1432997174477 / 9575080441755, 2526269341429 / 6820363962896,
+          2006345519317 / 3224310063776, 2802321613138 / 2924317926251)
+    return A, B, Cc

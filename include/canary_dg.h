@@ -1,0 +1,248 @@
+/* canary_dg.h -- C ABI of the MI355X nodal-DG library (libcanary_dg.so).
+ *
+ * Drop-in replacement for the device methods of Canary.jl's Euler (set2c)
+ * and shallow-water examples.  All arrays use the reference (Julia,
+ * column-major) layouts, all maps hold the reference's 1-based Int64 values,
+ * and all device pointers are owned by the caller.
+ *
+ *   Q, rhs      (Np, nstate, nelem)          Q[n + Np*(s + nstate*e)]
+ *   vgeo        (Np, nvgeo, nelem)           3D fields: xi_x eta_x zeta_x xi_y
+ *                                            eta_y zeta_y xi_z eta_z zeta_z MJ
+ *                                            MJI x y z (14); 2D: xi_x eta_x xi_y
+ *                                            eta_y MJ MJI x y (8)
+ *   sgeo        (nsgeo, Nfp, nface, nelem)   3D: nx ny nz sMJ vMJI (5);
+ *                                            2D: nx ny sMJ vMJI (4)
+ *   vmapM/P     (Nfp, nface, nelem) int64    1-based global node ids
+ *   elemtobndy  (nface, nelem) int64         0 = connected, 1 = no-flux wall
+ *   D           (Nq, Nq) column-major        D[j + Nq*k] = l_k'(r_j)
+ *
+ * Node n of an element is n = i + Nq*j (+ Nq^2*k), 0-based here.
+ * Return values: 0 on success, a CANARY_ERR_* code otherwise.
+ */
+#ifndef CANARY_DG_H
+#define CANARY_DG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes -------------------------------------------------------- */
+#define CANARY_OK 0
+#define CANARY_ERR_ARG 1            /* bad argument (size, pointer, N)        */
+#define CANARY_ERR_UNSUPPORTED 2    /* (eqn, dim, N, prec) not compiled in    */
+#define CANARY_ERR_LAUNCH 3         /* HIP launch / runtime error             */
+#define CANARY_ERR_BC 4             /* elemtobndy value other than 0 or 1     */
+#define CANARY_ERR_VMAP 5           /* vmapP outside [1, nelem_total*Np]      */
+#define CANARY_ERR_NOCONVERGENCE 6  /* QL iteration did not converge          */
+#define CANARY_ERR_LOBATTO_N1 7     /* Lobatto rule asked for n = 1           */
+#define CANARY_ERR_ORDERING 8       /* face vertices with no known ordering   */
+#define CANARY_ERR_ORIENTATION 9    /* 3D face orientation other than 1       */
+#define CANARY_ERR_NOMEM 10         /* allocation failed                      */
+#define CANARY_ERR_CAPACITY 11      /* request exceeds a stated fixed limit   */
+
+/* ---- equation sets, precisions ----------------------------------------- */
+#define CANARY_EULER 0   /* set2c: (rho u, rho v[, rho w], rho, rho theta)  */
+#define CANARY_SWE 1     /* (H u, H v, h, b), b read only                    */
+#define CANARY_FP64 0
+#define CANARY_FP32 1
+
+/* Physical constants passed at run time (reference defaults in
+ * canary_phys_default: examples/3d_kernels/euler3d_set2c.jl:58-63 and
+ * examples/old_fxg_files/shallow_water.jl main()). */
+typedef struct canary_phys {
+  double gamma;    /* wave-speed gamma (reference: 14//10)               */
+  double p0;       /* reference pressure (100000)                        */
+  double R_gas;    /* 28717//100                                         */
+  double c_p;      /* 100467//100                                        */
+  double c_v;      /* 7175//10                                           */
+  double gravity;  /* Euler buoyancy g (reference 10; bubble config 9.81) */
+  double swe_gravity; /* SWE gravity (reference 10)                      */
+  double delta_nl;    /* SWE nonlinear switch (reference 1)              */
+} canary_phys;
+
+void canary_phys_default(canary_phys* p);
+
+/* ======================================================================
+ * Drop-in device kernels (examples/3d_kernels/euler3d_set2c.jl,
+ * examples/2d_kernels/euler2d_set2c.jl, examples/old_fxg_files/shallow_water.jl)
+ * nreal:       number of real elements (reference `length(elems)`, elements
+ *              1:nreal); ghost elements follow them in Q.
+ * nelem_total: size(Q, 3) including ghosts; bounds every vmapP gather.
+ * stream:      hipStream_t (NULL = default stream).  Launches are async.
+ * Errors detected on the device (bad bc, vmapP out of range) are recorded
+ * in a device-side status word and returned by canary_check_errors().
+ * ====================================================================== */
+
+/* volumerhs!(Val(dim), Val(N), rhs, Q, vgeo, D, elems[, gravity, δnl])
+ *   replaces euler3d_set2c.jl:713-718 (CPU :144-231),
+ *            euler2d_set2c.jl:598-603 (CPU :140-205),
+ *            shallow_water.jl:560-563 (CPU :121-173).
+ *   rhs += weak-form volume term (+ buoyancy for Euler). */
+int canary_volumerhs(int eqn, int dim, int N, int prec, void* rhs, const void* Q,
+                     const void* vgeo, const double* D, int64_t nreal,
+                     const canary_phys* phys, void* stream);
+
+/* facerhs!(Val(dim), Val(N), rhs, Q, sgeo, elems, vmapM, vmapP, elemtobndy)
+ *   replaces euler3d_set2c.jl:720-726 (CPU :234-348),
+ *            euler2d_set2c.jl:605-611 (CPU :208-300),
+ *            shallow_water.jl fluxrhs! :565-569 (CPU :178-263).
+ *   rhs -= sMJ * Rusanov flux, owner computes, no atomics. */
+int canary_facerhs(int eqn, int dim, int N, int prec, void* rhs, const void* Q,
+                   const void* sgeo, int64_t nreal, int64_t nelem_total,
+                   const int64_t* vmapM, const int64_t* vmapP,
+                   const int64_t* elemtobndy, const canary_phys* phys, void* stream);
+
+/* updatesolution!(Val(dim), Val(N), rhs, Q, vgeo, elems, rka, rkb, dt[, advection])
+ *   replaces euler3d_set2c.jl:728-734 (CPU :352-358),
+ *            euler2d_set2c.jl:613-619 (CPU :304-310),
+ *            shallow_water.jl:571-575 (CPU :267-298).
+ *   Q += rkb*dt*rhs*MJI; rhs *= rka.  `advection` (SWE only) freezes the
+ *   velocity as the SWE CPU method does. */
+int canary_updatesolution(int eqn, int dim, int N, int prec, void* rhs, void* Q,
+                          const void* vgeo, int64_t nreal, double rka, double rkb,
+                          double dt, int advection, void* stream);
+
+/* One LSRK stage in a single pass: volume + faces + update, fused.
+ *   Equivalent to volumerhs! ; facerhs! ; updatesolution! of one iteration of
+ *   the stage loop (euler3d_set2c.jl:840-854), except that Q is read from
+ *   Qin and the updated state is written to Qout (Qin != Qout; the caller
+ *   swaps them each stage), so neighbouring elements never read a Q that
+ *   is being overwritten.  Ghost elements (nreal..nelem_total-1) of Qin are
+ *   read, never written.  rhs is updated in place. */
+int canary_stage_fused(int eqn, int dim, int N, int prec, void* rhs, const void* Qin,
+                       void* Qout, const void* vgeo, const void* sgeo,
+                       const double* D, int64_t nreal, int64_t nelem_total,
+                       const int64_t* vmapM, const int64_t* vmapP,
+                       const int64_t* elemtobndy, double rka, double rkb, double dt,
+                       const canary_phys* phys, void* stream);
+
+/* Synchronises `stream`, returns the first recorded device error
+ * (CANARY_ERR_BC / CANARY_ERR_VMAP or 0), writes (code, face, element,
+ * value) of it to info[0..3] if info != NULL, and clears the record. */
+int canary_check_errors(void* stream, int64_t* info);
+
+/* Number of HIP devices visible (0 without a GPU); never faults. */
+int canary_device_count(void);
+
+/* ======================================================================
+ * Host preparation (the inputs of the kernels).  All outputs are written
+ * to caller-provided host arrays in the reference layouts.
+ * ====================================================================== */
+
+#define CANARY_ENDPT_NEITHER 0
+#define CANARY_ENDPT_LEFT 1
+#define CANARY_ENDPT_RIGHT 2
+#define CANARY_ENDPT_BOTH 3
+
+/* GaussQuadrature.legendre(T, n, endpt) with maxits (<=0: 30):
+ * src/GaussQuadrature/quadrature.jl:126-214 */
+int canary_gauss_rule(int n, int endpt, int maxits, double* x, double* w);
+/* lglpoints / lgpoints: src/operators.jl:10-25 */
+int canary_lglpoints(int N, double* r, double* w);
+int canary_lgpoints(int N, double* r, double* w);
+/* baryweights / spectralderivative / interpolationmatrix:
+ * src/operators.jl:56-69, 84-104, 118-137 */
+int canary_baryweights(int n, const double* r, double* wb);
+int canary_spectralderivative(int n, const double* r, double* D);
+int canary_interpolationmatrix(int nsrc, const double* rsrc, int ndst,
+                               const double* rdst, double* I);
+
+/* hilbertcode(Y; bits): src/mesh.jl:35-88 (X, H are n words) */
+int canary_hilbertcode(int n, const uint64_t* Y, int bits, uint64_t* H);
+/* centroidtocode(comm, elemtocoord) on one rank: src/mesh.jl:107-141.
+ * elemtocoord (d, nvert, nelem) -> code (d, nelem) */
+int canary_centroidtocode(int d, int nvert, int64_t nelem, const double* elemtocoord,
+                          uint64_t* code);
+/* vertsortandorder(v...) for n = 1..4 vertices: src/mesh.jl:595-719 */
+int canary_vertsortandorder(int n, const int64_t* v, int64_t* sorted, int* order);
+
+/* Opaque mesh built by brickmesh -> partition -> connectmesh on one rank. */
+typedef struct canary_mesh canary_mesh;
+
+/* brickmesh(x, periodic; part, numparts, boundary): src/mesh.jl:259-317.
+ * ncoord[j] = length(x[j]); coords = x[1], x[2], ... concatenated;
+ * boundary = (2, d) column-major or NULL for all ones. */
+canary_mesh* canary_brickmesh(int d, const int64_t* ncoord, const double* coords,
+                              const int* periodic, int64_t part, int64_t numparts,
+                              const int64_t* boundary, int* status);
+/* A mesh from the four arrays brickmesh returns (faceconnections as
+ * (2+2^(d-1), nfc) column-major), e.g. to partition a user-built mesh. */
+canary_mesh* canary_mesh_from_arrays(int d, int64_t nelem, const int64_t* elemtovert,
+                                     const double* elemtocoord, const int64_t* elemtobndy,
+                                     int64_t nfc, const int64_t* faceconnections, int* status);
+/* partition(comm, mesh...) on one rank (Hilbert order): src/mesh.jl:499-580 */
+int canary_mesh_partition(canary_mesh* m);
+/* connectmesh(comm, mesh...) on one rank: src/mesh.jl:751-956 */
+int canary_mesh_connect(canary_mesh* m);
+/* sizes: what = 0 nelem, 1 dim, 2 nfaceconnections, 3 connected(0/1) */
+int64_t canary_mesh_size(const canary_mesh* m, int what);
+/* copy an array out: name is one of "elemtovert" (nvert,nelem),
+ * "elemtocoord" (d,nvert,nelem, double), "elemtobndy" (nface,nelem),
+ * "elemtoelem", "elemtoface", "elemtoordr" (nface,nelem),
+ * "faceconnections" (2+2^(d-1), nfc) -- int64 unless stated. */
+int canary_mesh_get(const canary_mesh* m, const char* name, void* out);
+void canary_mesh_free(canary_mesh* m);
+
+/* mappings(N, elemtoelem, elemtoface, elemtoordr): src/mesh.jl:973-1001 */
+int canary_mappings(int N, int d, int64_t nelem, const int64_t* elemtoelem,
+                    const int64_t* elemtoface, const int64_t* elemtoordr,
+                    int64_t* vmapM, int64_t* vmapP);
+
+/* creategrid!(x[, y[, z]], elemtocoord, r): src/metric.jl:11-107.
+ * X = d arrays of Nq^d*nelem doubles, contiguous one after the other. */
+int canary_creategrid(int d, int Nq, int64_t nelem, const double* elemtocoord,
+                      const double* r, double* X);
+/* computemetric!(...): src/metric.jl:120-400.
+ * X: d coordinate arrays; vol: J then the d*d metric terms in the
+ * reference argument order (1D: xi_x; 2D: xi_x eta_x xi_y eta_y;
+ * 3D: xi_x eta_x zeta_x xi_y eta_y zeta_y xi_z eta_z zeta_z), each
+ * Nq^d*nelem; surf: sJ then n_x .. n_d, each Nq^(d-1)*2d*nelem. */
+int canary_computemetric(int d, int Nq, int64_t nelem, const double* X, const double* D,
+                         double* vol, double* surf);
+/* computegeometry(Val(dim), mesh, D, ξ, ω, identity, vmapM):
+ * euler3d_set2c.jl:101-139 / euler2d_set2c.jl:97-134, no mesh warp.
+ * Writes vgeo (Np, nvgeo, nelem) and sgeo (nsgeo, Nfp, nface, nelem). */
+int canary_computegeometry(int d, int N, int64_t nelem, const double* elemtocoord,
+                           const int64_t* vmapM, double* vgeo, double* sgeo);
+
+/* ---- initial conditions, dt, energy (host) ------------------------------ */
+
+/* Rising thermal bubble (euler3d_set2c.jl:1189-1218, euler2d_set2c.jl:1056-1083):
+ * theta = theta0 + 0.5*dtheta*(1+cos(pi r/rc)) for r <= rc (+ jitter). */
+typedef struct canary_bubble {
+  double theta0;     /* background potential temperature (300)            */
+  double dtheta;     /* theta_c: peak perturbation (reference 0.5)        */
+  double rc;         /* radius (250)                                      */
+  double xc, yc, zc; /* centre; zc is the vertical centre (350)           */
+  int spherical;     /* 3D only: 1 = sphere, 0 = the reference x-z cylinder */
+  double jitter;     /* amplitude of a seeded theta jitter (0 = none)     */
+  uint64_t seed;
+} canary_bubble;
+
+/* Isentropic vortex (no reference counterpart; exact solution). */
+typedef struct canary_vortex {
+  double beta, x0, y0, u0, v0, rho_inf, p_inf, gamma, Lx, Ly;
+} canary_vortex;
+
+double canary_hash_jitter(uint64_t seed, double x, double y, double z);
+int canary_ic_bubble(int dim, int N, int64_t nelem, const double* vgeo,
+                     const canary_bubble* b, const canary_phys* ph, double* Q);
+int canary_ic_vortex2d(int N, int64_t nelem, const double* vgeo, const canary_vortex* v,
+                       double t, double* Q);
+/* bumps: nbump records (xc, yc, amplitude, width) */
+int canary_ic_swe_bumps(int N, int64_t nelem, const double* vgeo, double depth, int nbump,
+                        const double* bumps, double Lx, double Ly, double* Q);
+/* cfl(...) before the `/ N^sqrt(2)` (euler3d_set2c.jl:67-97, euler2d_set2c.jl:66-93,
+ * shallow_water.jl:43-67) */
+double canary_cfl(int eqn, int dim, int N, const double* vgeo, const double* Q,
+                  int64_t nelem, const canary_phys* ph);
+/* L2energysquared (euler3d_set2c.jl:738-750) */
+double canary_l2energysquared(int Np, int nstate, int nvgeo, int iMJ, const double* Q,
+                              const double* vgeo, int64_t nelem);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CANARY_DG_H */

@@ -1,0 +1,79 @@
+"""Seeded test states and small meshes shared by the CPU and GPU tests."""
+import numpy as np
+
+import canary_amd.problems as P
+
+
+def smooth_euler_state(s, seed=0, noise=1e-3, vel=10.0):
+    """A generic (non-equilibrium) Euler state: smooth fields plus per-node
+    noise, so every term of the RHS is exercised without cancellation."""
+    rng = np.random.default_rng(seed)
+    X = s.coords()
+    ph = rng.uniform(0, 2 * np.pi, size=8)
+    L = [np.ptp(x) if np.ptp(x) > 0 else 1.0 for x in X]
+    arg = [2 * np.pi * X[c] / L[c] for c in range(s.dim)]
+    sm = lambda k: np.sin(arg[0] + ph[k]) * np.cos(arg[-1] - ph[k + 1])
+    Q = s.zeros_state()
+    rho = 1.0 + 0.1 * sm(0) + noise * rng.standard_normal(X[0].shape)
+    theta = 300.0 + 5.0 * sm(2) + noise * rng.standard_normal(X[0].shape)
+    for c in range(s.dim):
+        Q[:, c, :] = rho * vel * (sm(4 + c) + noise * rng.standard_normal(X[0].shape))
+    Q[:, s.dim, :] = rho
+    Q[:, s.dim + 1, :] = rho * theta
+    return Q
+
+
+def pow_free_euler_state(s, seed=0, vel=10.0):
+    """Euler state with rho*theta chosen so that (R rho theta)/p0 == 1
+    exactly: pressure is p0 with no pow() rounding, so the GPU and the
+    oracle must agree bit for bit."""
+    Q = smooth_euler_state(s, seed=seed, vel=vel)
+    E = s.phys.p0 / s.phys.R_gas
+    # nudge E until (R*E)/p0 rounds to exactly 1.0
+    for _ in range(8):
+        if (s.phys.R_gas * E) / s.phys.p0 == 1.0:
+            break
+        E = np.nextafter(E, np.inf if (s.phys.R_gas * E) / s.phys.p0 < 1.0 else -np.inf)
+    assert (s.phys.R_gas * E) / s.phys.p0 == 1.0
+    Q[:, s.dim + 1, :] = E
+    return Q
+
+
+def smooth_swe_state(s, seed=0, depth=10.0, noise=1e-4):
+    rng = np.random.default_rng(seed)
+    X = s.coords()
+    ph = rng.uniform(0, 2 * np.pi, size=4)
+    Q = s.zeros_state()
+    Q[:, 3, :] = depth
+    Q[:, 2, :] = 0.05 * np.sin(2 * np.pi * X[0] + ph[0]) * np.cos(2 * np.pi * X[1] + ph[1])
+    Q[:, 2, :] += noise * rng.standard_normal(X[0].shape)
+    H = Q[:, 2, :] + Q[:, 3, :]
+    Q[:, 0, :] = H * 0.3 * np.sin(2 * np.pi * X[1] + ph[2])
+    Q[:, 1, :] = H * 0.3 * np.cos(2 * np.pi * X[0] + ph[3])
+    return Q
+
+
+def small_setup(eqn, dim, N, Ne=3, periodic=None, L=1000.0, gravity=9.81):
+    if periodic is None:
+        periodic = (False,) * dim
+    x = tuple(P.linrange(0.0, L, Ne + 1 + k) for k in range(dim))  # unequal counts per dim
+    phys = P._lib.default_phys(gravity=gravity)
+    if eqn == 1:
+        x = tuple(P.linrange(0.0, 1.0, Ne + 1 + k) for k in range(dim))
+        phys = P._lib.default_phys(swe_gravity=10.0, delta_nl=1.0)
+    return P.build(eqn, dim, N, x, periodic, phys=phys, label="small")
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b.ravel())
+    return np.linalg.norm((a - b).ravel()) / (nb if nb > 0 else 1.0)
+
+
+def max_ulp(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64).ravel().view(np.int64)
+    b = np.ascontiguousarray(b, dtype=np.float64).ravel().view(np.int64)
+    a = np.where(a < 0, np.int64(-0x8000000000000000) - a, a)
+    b = np.where(b < 0, np.int64(-0x8000000000000000) - b, b)
+    return int(np.max(np.abs(a - b))) if a.size else 0

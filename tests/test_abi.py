@@ -1,0 +1,57 @@
+"""The C-ABI library loads and exports every function include/*.h declares
+(no compute calls: these run without a GPU)."""
+import ctypes
+import glob
+import os
+import re
+
+import canary_amd._lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(canary_\w+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.lib()
+    names = declared_functions()
+    assert len(names) > 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_bindings_cover_header():
+    assert set(declared_functions()) <= set(L.exported_symbols())
+
+
+def test_device_count_without_gpu_is_safe():
+    n = L.lib().canary_device_count()
+    assert n >= 0
+
+
+def test_device_entry_points_reject_bad_arguments_before_launch():
+    lib = L.lib()
+    p = L.default_phys()
+    # unsupported (eqn, dim), N and precision are rejected on the host
+    assert lib.canary_volumerhs(1, 3, 4, 0, None, None, None, None, 1, ctypes.byref(p), None) == L.ERR_UNSUPPORTED
+    assert lib.canary_volumerhs(0, 3, 9, 0, None, None, None, None, 1, ctypes.byref(p), None) == L.ERR_UNSUPPORTED
+    assert lib.canary_volumerhs(0, 3, 4, 7, None, None, None, None, 1, ctypes.byref(p), None) == L.ERR_UNSUPPORTED
+    # null pointers with work to do
+    assert lib.canary_volumerhs(0, 3, 4, 0, None, None, None, None, 1, ctypes.byref(p), None) == L.ERR_ARG
+    assert lib.canary_facerhs(0, 3, 4, 0, None, None, None, 4, 2, None, None, None, ctypes.byref(p), None) == L.ERR_ARG
+    # zero elements is a no-op
+    assert lib.canary_updatesolution(0, 2, 3, 0, None, None, None, 0, 0.0, 0.0, 0.0, 0, None) == 0
+
+
+def test_phys_defaults_are_the_reference_rationals():
+    p = L.default_phys()
+    assert (p.gamma, p.p0, p.R_gas, p.c_p, p.c_v, p.gravity) == (14 / 10, 100000.0, 28717 / 100,
+                                                                  100467 / 100, 7175 / 10, 10.0)

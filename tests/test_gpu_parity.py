@@ -1,0 +1,260 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle.
+
+Bar: the exact-order FP64 kernels differ from the oracle only through pow()
+(OCML on the GPU, libm on the host).  So
+  * on states whose pressure needs no pow (R rho theta / p0 == 1 exactly),
+    and for the shallow-water equations (no pow), results are bit-identical;
+  * on generic states the relative L2 difference per RHS is <= 1e-12 and,
+    after 100 LSRK54 steps, <= 1e-10 (the tolerances the task states);
+  * the fused single-pass stage is bit-identical to volume + face + update.
+"""
+import numpy as np
+import pytest
+import torch
+
+import canary_amd.kernels as K
+import canary_amd.problems as P
+from canary_amd._lib import CanaryError, ERR_BC, ERR_VMAP
+from oracle import oracle as O
+
+from gpu_util import DeviceSetup, dev, host
+from states import (max_ulp, pow_free_euler_state, rel_l2, small_setup, smooth_euler_state,
+                    smooth_swe_state)
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(0, 3, 4), (0, 3, 3), (0, 3, 1), (0, 3, 7), (0, 2, 4), (0, 2, 3), (0, 2, 7),
+         (1, 2, 3), (1, 2, 4), (1, 2, 5), (1, 2, 7)]
+
+
+def case_setup(eqn, dim, N, periodic=None):
+    Ne = 3 if dim == 3 else 4
+    return small_setup(eqn, dim, N, Ne=Ne, periodic=periodic)
+
+
+def state(s, seed, powfree=False):
+    if s.eqn == 1:
+        return smooth_swe_state(s, seed=seed)
+    return pow_free_euler_state(s, seed=seed) if powfree else smooth_euler_state(s, seed=seed)
+
+
+def gpu_rhs(s, ds, Q, rhs0):
+    rhs = dev(rhs0)
+    q = dev(Q)
+    K.volumerhs(s.dim, s.N, rhs, q, ds.vgeo, ds.D, s.nelem, eqn=s.eqn, phys=s.phys)
+    K.facerhs(s.dim, s.N, rhs, q, ds.sgeo, s.nelem, ds.vmapM, ds.vmapP, ds.bndy, eqn=s.eqn,
+              phys=s.phys)
+    K.check_errors()
+    return host(rhs, Q.shape)
+
+
+@pytest.mark.parametrize("eqn,dim,N", CASES)
+def test_rhs_parity(cuda, eqn, dim, N):
+    print("case eqn=%d dim=%d N=%d" % (eqn, dim, N), flush=True)
+    periodic = (True,) * dim if N == 3 else None
+    s = case_setup(eqn, dim, N, periodic)
+    ds = DeviceSetup(s)
+    rng = np.random.default_rng(N)
+    for powfree in ((False, True) if eqn == 0 else (False,)):
+        Q = np.asfortranarray(state(s, seed=7 + N, powfree=powfree))
+        rhs0 = np.asfortranarray(rng.standard_normal(Q.shape) * 1e3)
+        if eqn == 1:
+            rhs0[:, 3, :] = 0.0
+        want = O.rhs_eval(s, Q, rhs0)
+        got = gpu_rhs(s, ds, Q, rhs0)
+        if powfree or eqn == 1:
+            assert max_ulp(got, want) == 0, "expected bit-identical results"
+        else:
+            assert rel_l2(got, want) <= 1e-12
+
+
+@pytest.mark.parametrize("eqn,dim,N", [(0, 3, 4), (0, 2, 4), (1, 2, 4)])
+def test_volume_and_face_separately(cuda, eqn, dim, N):
+    s = case_setup(eqn, dim, N)
+    ds = DeviceSetup(s)
+    Q = np.asfortranarray(state(s, seed=3, powfree=True))
+    # volume alone
+    rv = s.zeros_state()
+    O.volumerhs(s, rv, Q)
+    g = dev(s.zeros_state())
+    K.volumerhs(dim, N, g, dev(Q), ds.vgeo, ds.D, range(1, s.nelem + 1), eqn=eqn, phys=s.phys)
+    assert max_ulp(host(g, Q.shape), rv) == 0
+    # face alone
+    rf = s.zeros_state()
+    O.facerhs(s, rf, Q)
+    g = dev(s.zeros_state())
+    K.facerhs(dim, N, g, dev(Q), ds.sgeo, s.nelem, ds.vmapM, ds.vmapP, ds.bndy, eqn=eqn,
+              phys=s.phys)
+    K.check_errors()
+    assert max_ulp(host(g, Q.shape), rf) == 0
+
+
+@pytest.mark.parametrize("eqn,dim,N", [(0, 3, 4), (0, 2, 5), (1, 2, 3)])
+def test_updatesolution_bitwise(cuda, eqn, dim, N):
+    s = case_setup(eqn, dim, N)
+    ds = DeviceSetup(s)
+    rng = np.random.default_rng(11)
+    Q = np.asfortranarray(state(s, seed=1))
+    rhs = np.asfortranarray(rng.standard_normal(Q.shape))
+    if eqn == 1:
+        rhs[:, 3, :] = 0.0
+    A, B, _ = P.rk_coefs()
+    Qo, ro = Q.copy(order="F"), rhs.copy(order="F")
+    O.updatesolution(s, ro, Qo, A[2], B[1], 0.37)
+    q, r = dev(Q), dev(rhs)
+    K.updatesolution(dim, N, r, q, ds.vgeo, s.nelem, A[2], B[1], 0.37, eqn=eqn)
+    assert max_ulp(host(q, Q.shape), Qo) == 0
+    assert max_ulp(host(r, Q.shape), ro) == 0
+
+
+def test_swe_advection_update(cuda):
+    s = case_setup(1, 2, 3)
+    ds = DeviceSetup(s)
+    Q = np.asfortranarray(smooth_swe_state(s, seed=4))
+    rhs = np.asfortranarray(np.random.default_rng(2).standard_normal(Q.shape))
+    rhs[:, 3, :] = 0.0
+    Qo, ro = Q.copy(order="F"), rhs.copy(order="F")
+    O.updatesolution(s, ro, Qo, 0.5, 0.25, 0.1, advection=True)
+    q, r = dev(Q), dev(rhs)
+    K.updatesolution(2, 3, r, q, ds.vgeo, s.nelem, 0.5, 0.25, 0.1, eqn=1, advection=True)
+    assert max_ulp(host(q, Q.shape), Qo) == 0
+
+
+@pytest.mark.parametrize("eqn,dim,N", [(0, 3, 4), (0, 3, 2), (0, 3, 6), (0, 2, 4), (0, 2, 6),
+                                       (1, 2, 4), (1, 2, 7)])
+def test_fused_stage_equals_three_kernels(cuda, eqn, dim, N):
+    s = case_setup(eqn, dim, N, periodic=(True,) + (False,) * (dim - 1))
+    ds = DeviceSetup(s)
+    Q = np.asfortranarray(state(s, seed=5))
+    rhs0 = np.asfortranarray(np.random.default_rng(3).standard_normal(Q.shape) * 10)
+    if eqn == 1:
+        rhs0[:, 3, :] = 0.0
+    A, B, _ = P.rk_coefs()
+    q1, r1 = dev(Q), dev(rhs0)
+    K.volumerhs(dim, N, r1, q1, ds.vgeo, ds.D, s.nelem, eqn=eqn, phys=s.phys)
+    K.facerhs(dim, N, r1, q1, ds.sgeo, s.nelem, ds.vmapM, ds.vmapP, ds.bndy, eqn=eqn, phys=s.phys)
+    K.updatesolution(dim, N, r1, q1, ds.vgeo, s.nelem, A[3], B[2], 0.01, eqn=eqn)
+    qin, qout, r2 = dev(Q), dev(np.zeros_like(Q)), dev(rhs0)
+    K.stage_fused(dim, N, r2, qin, qout, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM, ds.vmapP,
+                  ds.bndy, A[3], B[2], 0.01, eqn=eqn, phys=s.phys)
+    K.check_errors()
+    assert torch.equal(q1, qout) and torch.equal(r1, r2)
+
+
+def run_gpu_lsrk(s, ds, Q, dt, nsteps, fused):
+    A, B, _ = P.rk_coefs()
+    q = dev(Q)
+    rhs = dev(s.zeros_state())
+    q2 = torch.empty_like(q) if fused else None
+    for _ in range(nsteps):
+        for st in range(5):
+            if fused:
+                K.stage_fused(s.dim, s.N, rhs, q, q2, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM,
+                              ds.vmapP, ds.bndy, A[(st + 1) % 5], B[st], dt, eqn=s.eqn,
+                              phys=s.phys)
+                q, q2 = q2, q
+            else:
+                K.volumerhs(s.dim, s.N, rhs, q, ds.vgeo, ds.D, s.nelem, eqn=s.eqn, phys=s.phys)
+                K.facerhs(s.dim, s.N, rhs, q, ds.sgeo, s.nelem, ds.vmapM, ds.vmapP, ds.bndy,
+                          eqn=s.eqn, phys=s.phys)
+                K.updatesolution(s.dim, s.N, rhs, q, ds.vgeo, s.nelem, A[(st + 1) % 5], B[st], dt,
+                                 eqn=s.eqn)
+    K.check_errors()
+    return host(q, Q.shape), host(rhs, Q.shape)
+
+
+@pytest.mark.parametrize("which", ["bubble3d", "bubble2d", "swe"])
+@pytest.mark.parametrize("fused", [False, True])
+def test_lsrk_100_steps(cuda, which, fused):
+    print("lsrk100 %s fused=%s" % (which, fused), flush=True)
+    if which == "bubble3d":
+        s = P.bubble3d(Ne=4, N=4, dtheta=2.0, jitter=1e-3, seed=3)
+        dt = 0.02
+    elif which == "bubble2d":
+        s = P.bubble2d(Ne=6, N=4)
+        dt = 0.05
+    else:
+        s = P.swe2d(Ne=6, N=4, bumps=((0.4, 0.5, 0.05, 0.1), (0.7, 0.3, 0.02, 0.15)))
+        dt = 1e-4
+    ds = DeviceSetup(s)
+    Q0 = np.asfortranarray(s.Q)
+    Qo, ro = Q0.copy(order="F"), s.zeros_state()
+    assert O.lsrk(s, Qo, ro, dt, 100, nthreads=8) == 0
+    Qg, rg = run_gpu_lsrk(s, ds, Q0, dt, 100, fused)
+    assert np.all(np.isfinite(Qg))
+    err = rel_l2(Qg, Qo)
+    print("rel L2 after 100 steps: %.3e" % err, flush=True)
+    assert err <= 1e-10
+    if which == "swe":
+        assert max_ulp(Qg, Qo) == 0  # no pow in the shallow-water path
+
+
+def test_invalid_bc_reported(cuda):
+    s = case_setup(0, 3, 2)
+    ds = DeviceSetup(s)
+    bad = s.elemtobndy.copy(order="F")
+    bad[3, 1] = 5
+    Q = dev(smooth_euler_state(s))
+    rhs = dev(s.zeros_state())
+    K.facerhs(3, 2, rhs, Q, ds.sgeo, s.nelem, ds.vmapM, ds.vmapP, dev(bad), phys=s.phys)
+    with pytest.raises(CanaryError) as ei:
+        K.check_errors()
+    assert ei.value.code == ERR_BC and ei.value.info[1:] == (4, 2, 5)
+    K.check_errors()  # cleared
+
+
+def test_vmap_out_of_range_reported(cuda):
+    s = case_setup(0, 2, 3)
+    ds = DeviceSetup(s)
+    vP = s.vmapP.copy(order="F")
+    vP[1, 0, 2] = s.nelem * s.Np + 7  # beyond Q
+    Q = dev(smooth_euler_state(s))
+    rhs = dev(s.zeros_state())
+    K.facerhs(2, 3, rhs, Q, ds.sgeo, s.nelem, ds.vmapM, dev(vP), ds.bndy, phys=s.phys)
+    with pytest.raises(CanaryError) as ei:
+        K.check_errors()
+    assert ei.value.code == ERR_VMAP
+
+
+def test_ghost_elements(cuda):
+    """Elements after nreal are ghosts: read through vmapP, never written."""
+    s = case_setup(0, 3, 3, periodic=(True, True, True))
+    ds = DeviceSetup(s)
+    Q = np.asfortranarray(smooth_euler_state(s, seed=9))
+    nreal = s.nelem - 5
+    want = s.zeros_state()
+    O.volumerhs(s, want, Q, nelem=nreal)
+    O.facerhs(s, want, Q, nelem=nreal)
+    rhs = dev(s.zeros_state())
+    q = dev(Q)
+    K.volumerhs(3, 3, rhs, q, ds.vgeo, ds.D, nreal, phys=s.phys)
+    K.facerhs(3, 3, rhs, q, ds.sgeo, nreal, ds.vmapM, ds.vmapP, ds.bndy, phys=s.phys,
+              nelem_total=s.nelem)
+    K.check_errors()
+    got = host(rhs, Q.shape)
+    assert rel_l2(got[:, :, :nreal], want[:, :, :nreal]) <= 1e-12
+    assert np.all(got[:, :, nreal:] == 0)
+    # fused: ghosts of Qout are untouched too
+    qout = dev(np.full(Q.shape, -1.0, order="F"))
+    r2 = dev(s.zeros_state())
+    K.stage_fused(3, 3, r2, q, qout, ds.vgeo, ds.sgeo, ds.D, nreal, ds.vmapM, ds.vmapP, ds.bndy,
+                  0.0, 0.1, 0.01, phys=s.phys, nelem_total=s.nelem)
+    K.check_errors()
+    assert np.all(host(qout, Q.shape)[:, :, nreal:] == -1.0)
+
+
+def test_empty_and_bad_args(cuda):
+    s = case_setup(0, 2, 2)
+    ds = DeviceSetup(s)
+    q = dev(smooth_euler_state(s))
+    rhs = dev(s.zeros_state())
+    K.volumerhs(2, 2, rhs, q, ds.vgeo, ds.D, 0, phys=s.phys)  # no-op
+    K.facerhs(2, 2, rhs, q, ds.sgeo, 0, ds.vmapM, ds.vmapP, ds.bndy, phys=s.phys)
+    assert torch.count_nonzero(rhs) == 0
+    with pytest.raises(CanaryError):
+        K.volumerhs(2, 9, rhs, q, ds.vgeo, dev(np.eye(10)), 1, phys=s.phys)  # N > 7
+    with pytest.raises(ValueError):
+        K.volumerhs(2, 2, rhs[:5], q, ds.vgeo, ds.D, s.nelem, phys=s.phys)  # too small
+    with pytest.raises(ValueError):
+        K.stage_fused(2, 2, rhs, q, q, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM, ds.vmapP,
+                      ds.bndy, 0.0, 0.1, 0.01, phys=s.phys)
