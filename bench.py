@@ -86,7 +86,8 @@ def main():
     vM, vP, bnd = d(s.vmapM), d(s.vmapP), d(s.elemtobndy)
     Q0 = d(s.Q)
     A, B, _ = P.rk_coefs()
-    dt = 0.01
+    # the reference's time step: cfl(...) / N^sqrt(2) (euler3d_set2c.jl:1104)
+    dt = P.cfl(s, s.Q) / s.N ** np.sqrt(2)
     stream = torch.cuda.current_stream()
 
     def run_fused(q, q2, rhs, nsteps):

@@ -125,9 +125,102 @@ __device__ __forceinline__ void volume_sweeps(double* r, const double* sD, const
 }
 
 // ------------------------------------------------------------ face terms
-// Contribution sMJ * Rusanov flux of face node `fi` (f*Nfp + fn) of element
-// e; M-side state read through `qm_at(s, vid)`, P side gathered from Q.
-// Returns vidM (0-based node of e), or -1 after recording an error.
+// A face item is one face node `fi` (= f*Nfp + fn) of element e.  Its work
+// is split in three steps so that a kernel can issue the loads early and
+// consume them later: face_meta (normal, sMJ, maps, bc), face_gather (the
+// P-side state from Q) and face_finish (wall state, pressure, Rusanov flux,
+// times sMJ -- the amount the reference subtracts from rhs).
+template <class Phys>
+struct FaceItem {
+  double nrm[Phys::DIM];
+  double sMJ;
+  double qP[Phys::NS];
+  int64_t eP;
+  int vidM;  // -1: no contribution (an error was recorded)
+  int vidP;
+  int bc;
+};
+
+// Barrier for threads that exchange data through LDS only: waits for LDS
+// traffic (lgkmcnt) but leaves global loads in flight across the barrier.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <class Phys, int NQ>
+__device__ __forceinline__ void face_meta(FaceItem<Phys>& it, const double* __restrict__ sgeo,
+                                          const int64_t* __restrict__ vmapM,
+                                          const int64_t* __restrict__ vmapP,
+                                          const int64_t* __restrict__ bndy, int64_t e, int fi,
+                                          int64_t ntotal) {
+  using S = Shape<Phys, NQ>;
+  constexpr int Np = S::Np, DIM = Phys::DIM, NSG = Phys::NSGEO;
+  const int f = fi / S::Nfp;
+  const size_t fo = (size_t)fi + (size_t)S::NFN * (size_t)e;
+  const double* sg = sgeo + (size_t)NSG * fo;
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) it.nrm[c] = sg[c];
+  it.sMJ = sg[DIM];
+  const int64_t idM = vmapM[fo];
+  const int64_t idP = vmapP[fo];
+  const int64_t bc = bndy[f + S::NFACE * e];
+  const int64_t nodes = ntotal * (int64_t)Np;
+  it.vidM = -1;
+  it.bc = (int)bc;
+  it.eP = 0;
+  it.vidP = 0;
+  if (idM < 1 || idM > nodes) {
+    record_error(5 /*CANARY_ERR_VMAP*/, f + 1, e + 1, idM);
+    return;
+  }
+  if (bc == 0) {
+    if (idP < 1 || idP > nodes) {
+      record_error(5 /*CANARY_ERR_VMAP*/, f + 1, e + 1, idP);
+      return;
+    }
+    split_id<Np>(idP, it.eP, it.vidP);
+  } else if (bc != 1) {
+    record_error(4 /*CANARY_ERR_BC*/, f + 1, e + 1, bc);
+    return;
+  }
+  int64_t eMdummy;
+  split_id<Np>(idM, eMdummy, it.vidM);
+}
+
+template <class Phys, int NQ>
+__device__ __forceinline__ void face_gather(FaceItem<Phys>& it, const double* __restrict__ Q) {
+  constexpr int Np = Shape<Phys, NQ>::Np, NS = Phys::NS;
+  if (it.vidM >= 0 && it.bc == 0) {
+    const double* qe = Q + (size_t)NS * Np * (size_t)it.eP;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) it.qP[s] = qe[it.vidP + Np * s];
+  }
+}
+
+// M-side state through qm_at(s, node) and pm_at(node, qM); out[NSR].
+template <class Phys, class QM, class PMF>
+__device__ __forceinline__ int face_finish(FaceItem<Phys>& it, const PhysParams& p, QM qm_at,
+                                           PMF pm_at, double* out) {
+  constexpr int NS = Phys::NS;
+  if (it.vidM < 0) return -1;
+  double qM[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) qM[s] = qm_at(s, it.vidM);
+  const double PM = pm_at(it.vidM, qM);
+  double PP = 0.0;
+  if (it.bc == 0) {
+    if (Phys::B_FROM_M) it.qP[NS - 1] = qm_at(NS - 1, it.vidP);
+    PP = Phys::HAS_P ? Phys::pressure(it.qP, p) : 0.0;
+  } else {
+    Phys::reflect(qM, PM, it.nrm, it.qP, PP);
+  }
+  Phys::face_flux(qM, PM, it.qP, PP, it.nrm, it.sMJ, p, out);
+  return it.vidM;
+}
+
+// All three steps at once.
 template <class Phys, int NQ, class QM, class PMF>
 __device__ __forceinline__ int face_contribution(const double* __restrict__ Q,
                                                  const double* __restrict__ sgeo,
@@ -136,52 +229,10 @@ __device__ __forceinline__ int face_contribution(const double* __restrict__ Q,
                                                  const int64_t* __restrict__ bndy, int64_t e,
                                                  int fi, int64_t ntotal, const PhysParams& p,
                                                  QM qm_at, PMF pm_at, double* out) {
-  using S = Shape<Phys, NQ>;
-  constexpr int Np = S::Np, NS = Phys::NS, DIM = Phys::DIM, NSG = Phys::NSGEO;
-  const int f = fi / S::Nfp;
-  const size_t fo = (size_t)fi + (size_t)S::NFN * (size_t)e;
-  const double* sg = sgeo + (size_t)NSG * fo;
-  double nrm[DIM];
-#pragma unroll
-  for (int c = 0; c < DIM; ++c) nrm[c] = sg[c];
-  const double sMJ = sg[DIM];
-  const int64_t idM = vmapM[fo];
-  const int64_t idP = vmapP[fo];
-  const int64_t bc = bndy[f + S::NFACE * e];
-  const int64_t nodes = ntotal * (int64_t)Np;
-  if (idM < 1 || idM > nodes) {
-    record_error(5 /*CANARY_ERR_VMAP*/, f + 1, e + 1, idM);
-    return -1;
-  }
-  int64_t eMdummy;
-  int vidM;
-  split_id<Np>(idM, eMdummy, vidM);
-  double qM[NS], qP[NS];
-#pragma unroll
-  for (int s = 0; s < NS; ++s) qM[s] = qm_at(s, vidM);
-  const double PM = pm_at(vidM, qM);
-  double PP = 0.0;
-  if (bc == 0) {
-    if (idP < 1 || idP > nodes) {
-      record_error(5 /*CANARY_ERR_VMAP*/, f + 1, e + 1, idP);
-      return -1;
-    }
-    int64_t eP;
-    int vidP;
-    split_id<Np>(idP, eP, vidP);
-    const double* qe = Q + (size_t)NS * Np * (size_t)eP;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) qP[s] = qe[vidP + Np * s];
-    if (Phys::B_FROM_M) qP[NS - 1] = qm_at(NS - 1, vidP);
-    PP = Phys::HAS_P ? Phys::pressure(qP, p) : 0.0;
-  } else if (bc == 1) {
-    Phys::reflect(qM, PM, nrm, qP, PP);
-  } else {
-    record_error(4 /*CANARY_ERR_BC*/, f + 1, e + 1, bc);
-    return -1;
-  }
-  Phys::face_flux(qM, PM, qP, PP, nrm, sMJ, p, out);
-  return vidM;
+  FaceItem<Phys> it;
+  face_meta<Phys, NQ>(it, sgeo, vmapM, vmapP, bndy, e, fi, ntotal);
+  face_gather<Phys, NQ>(it, Q);
+  return face_finish<Phys>(it, p, qm_at, pm_at, out);
 }
 
 // ================================================================ kernels
@@ -227,7 +278,7 @@ __global__ __launch_bounds__((VolumeCfg<Phys, NQ, EPW>::NT)) void volume_kernel(
     for (int s = 0; s < NSR; ++s) r[s] = re[n + Np * s];
     Phys::source(q, MJ, p, r);
   }
-  __syncthreads();
+  lds_barrier();
   if (valid) {
     volume_sweeps<DIM, NQ, NSR>(r, sD, sF[0][slot], sF[DIM > 1 ? 1 : 0][slot],
                                 sF[DIM > 2 ? 2 : 0][slot], n);
@@ -275,7 +326,7 @@ __global__ __launch_bounds__((FaceCfg<Phys, NQ, EPW>::NT)) void face_kernel(
 #pragma unroll
     for (int s = 0; s < NSR; ++s) sC[slot][fi * NSR + s] = vid >= 0 ? c[s] : __builtin_nan("");
   }
-  __syncthreads();
+  lds_barrier();
 #pragma unroll
   for (int d = 0; d < S::DIM; ++d) {
     for (int it = t; it < EPW * 2 * Nfp; it += NT) {
@@ -287,7 +338,7 @@ __global__ __launch_bounds__((FaceCfg<Phys, NQ, EPW>::NT)) void face_kernel(
         for (int s = 0; s < NSR; ++s) sR[slot][s * Np + v] = sR[slot][s * Np + v] - sC[slot][fi * NSR + s];
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
   for (int idx = t; idx < EPW * NSR * Np; idx += NT) {
     const int slot = idx / (NSR * Np), rem = idx - slot * (NSR * Np);
@@ -337,6 +388,16 @@ __global__ __launch_bounds__(256) void update_kernel(double* __restrict__ rhs,
 // One LSRK stage of EPW elements per workgroup: volume term, face terms and
 // update in one pass over HBM.  Reads Qin (own + neighbour traces), writes
 // Qout; rhs in place.
+//
+// Schedule (per workgroup):
+//   1. each thread issues the loads of its first face item (maps, normals,
+//      bc) together with its volume node's Q, metric terms and rhs;
+//   2. volume fluxes -> LDS; the P-side gather of the face item is issued
+//      as soon as its map has arrived;
+//   3. the xi/eta/zeta sweeps run from LDS while the gathers are in flight
+//      (barriers fence LDS only, so loads stay outstanding across them);
+//   4. face fluxes -> LDS, scatter onto the staged rhs (DIM rounds);
+//   5. update: Qout = Q + ((rkb*dt)*rhs)*MJI, rhs *= rka.
 template <class Phys, int NQ, int EPW>
 struct FusedCfg {
   using S = Shape<Phys, NQ>;
@@ -371,22 +432,31 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW>::NT)) void stage_kernel(
   const int slot = t / Np, n = t - slot * Np;
   const int64_t e = e0 + slot;
   const bool valid = slot < EPW && e < nreal;
+
+  // 1. first face item of this thread: metadata now, gather after step 2
+  const int fsl = t / NFN, ffi = t - fsl * NFN;
+  const bool fvalid = t < EPW * NFN && e0 + fsl < nreal;
+  FaceItem<Phys> f0;
+  if (fvalid) face_meta<Phys, NQ>(f0, sgeo, vmapM, vmapP, bndy, e0 + fsl, ffi, ntotal);
   if (t < NQ * NQ) sD[t] = D[t];
 
-  double q[NS], r[NSR], MJI = 0.0;
+  double r[NSR], MJI = 0.0;
   if (valid) {
+    double q[NS];
     const double* qe = Qin + (size_t)NS * Np * (size_t)e;
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      q[s] = qe[n + Np * s];
-      sQ[slot][s * Np + n] = q[s];
-    }
+    for (int s = 0; s < NS; ++s) q[s] = qe[n + Np * s];
     const double* vg = vgeo + (size_t)NV * Np * (size_t)e;
     double met[DIM * DIM];
 #pragma unroll
     for (int c = 0; c < DIM * DIM; ++c) met[c] = vg[n + Np * c];
     const double MJ = vg[n + Np * Phys::iMJ];
     MJI = vg[n + Np * Phys::iMJI];
+    const double* re = rhs + (size_t)NS * Np * (size_t)e;
+#pragma unroll
+    for (int s = 0; s < NSR; ++s) r[s] = re[n + Np * s];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) sQ[slot][s * Np + n] = q[s];
     const double P = Phys::HAS_P ? Phys::pressure(q, p) : 0.0;
     if (Phys::HAS_P) sP[slot][n] = P;
     double F[DIM][NSR];
@@ -395,24 +465,33 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW>::NT)) void stage_kernel(
     for (int d = 0; d < DIM; ++d)
 #pragma unroll
       for (int s = 0; s < NSR; ++s) sB[slot][(d * NSR + s) * Np + n] = F[d][s];
-    const double* re = rhs + (size_t)NS * Np * (size_t)e;
-#pragma unroll
-    for (int s = 0; s < NSR; ++s) r[s] = re[n + Np * s];
     Phys::source(q, MJ, p, r);
   }
-  __syncthreads();
+  // 2. P-side gather of the first face item (stays in flight over step 3)
+  if (fvalid) face_gather<Phys, NQ>(f0, Qin);
+  lds_barrier();
+  // 3. sweeps
   if (valid) {
     const double* b = sB[slot];
     volume_sweeps<DIM, NQ, NSR>(r, sD, b, b + (DIM > 1 ? NSR * Np : 0),
                                 b + (DIM > 2 ? 2 * NSR * Np : 0), n);
   }
-  __syncthreads();  // fluxes consumed; sB is reused below
-  double* const sC = &sB[0][0];  // per slot: [NFN * NSR] contributions, then [NSR * Np] rhs
+  lds_barrier();  // fluxes consumed; sB is reused below
   if (valid) {
 #pragma unroll
     for (int s = 0; s < NSR; ++s) sB[slot][NFN * NSR + s * Np + n] = r[s];
   }
-  for (int item = t; item < EPW * NFN; item += NT) {
+  // 4. face fluxes: M side from LDS, P side gathered
+  if (fvalid) {
+    auto qm_at = [&](int s, int v) { return sQ[fsl][s * Np + v]; };
+    auto pm_at = [&](int v, const double*) { return Phys::HAS_P ? sP[fsl][Phys::HAS_P ? v : 0] : 0.0; };
+    double c[NSR];
+    const int vid = face_finish<Phys>(f0, p, qm_at, pm_at, c);
+    sV[fsl][ffi] = vid;
+#pragma unroll
+    for (int s = 0; s < NSR; ++s) sB[fsl][ffi * NSR + s] = c[s];
+  }
+  for (int item = t + NT; item < EPW * NFN; item += NT) {
     const int sl = item / NFN, fi = item - sl * NFN;
     const int64_t ee = e0 + sl;
     if (ee >= nreal) continue;
@@ -423,10 +502,9 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW>::NT)) void stage_kernel(
                                                 p, qm_at, pm_at, c);
     sV[sl][fi] = vid;
 #pragma unroll
-    for (int s = 0; s < NSR; ++s) sB[sl][fi * NSR + s] = vid >= 0 ? c[s] : __builtin_nan("");
+    for (int s = 0; s < NSR; ++s) sB[sl][fi * NSR + s] = c[s];
   }
-  (void)sC;
-  __syncthreads();
+  lds_barrier();
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
     for (int it = t; it < EPW * 2 * Nfp; it += NT) {
@@ -439,8 +517,9 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW>::NT)) void stage_kernel(
         for (int s = 0; s < NSR; ++s) rr[s * Np + v] = rr[s * Np + v] - sB[sl][fi * NSR + s];
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
+  // 5. update
   if (valid) {
     const double bdt = rkb * dt;
     double* qo = Qout + (size_t)NS * Np * (size_t)e;
@@ -448,11 +527,11 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW>::NT)) void stage_kernel(
 #pragma unroll
     for (int s = 0; s < NSR; ++s) {
       const double rs = sB[slot][NFN * NSR + s * Np + n];
-      qo[n + Np * s] = q[s] + bdt * rs * MJI;
+      qo[n + Np * s] = sQ[slot][s * Np + n] + bdt * rs * MJI;
       re[n + Np * s] = rs * rka;
     }
 #pragma unroll
-    for (int s = NSR; s < NS; ++s) qo[n + Np * s] = q[s];  // read-only states (SWE b)
+    for (int s = NSR; s < NS; ++s) qo[n + Np * s] = sQ[slot][s * Np + n];  // read-only states
   }
 }
 
