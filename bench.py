@@ -57,6 +57,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-elems", type=int, default=32768)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-gib", type=float, default=0.0,
+                    help="also run one device-to-device copy of this many GiB (untimed), the "
+                         "known-size reference for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU kernel; "
@@ -114,6 +117,14 @@ def main():
         e1.record(stream)
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / 1e3, time.perf_counter() - t_wall
+
+    if args.copy_gib > 0:
+        nwords = int(args.copy_gib * (1 << 30)) // 8
+        src = torch.ones(nwords, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # fused path (the measured value)
     q, q2 = Q0.clone(), torch.empty_like(Q0)

@@ -7,7 +7,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "libcanary_dg.so")
+# CANARY_DG_LIB points at another build of the same library (kernel tuning runs)
+LIBPATH = os.environ.get("CANARY_DG_LIB") or os.path.join(HERE, "libcanary_dg.so")
 
 # error codes (canary_dg.h)
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_LAUNCH, ERR_BC, ERR_VMAP = 0, 1, 2, 3, 4, 5

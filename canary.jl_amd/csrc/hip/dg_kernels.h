@@ -27,6 +27,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "phys.h"
 
 namespace canary {
@@ -398,17 +400,26 @@ __global__ __launch_bounds__(256) void update_kernel(double* __restrict__ rhs,
 //      (barriers fence LDS only, so loads stay outstanding across them);
 //   4. face fluxes -> LDS, scatter onto the staged rhs (DIM rounds);
 //   5. update: Qout = Q + ((rkb*dt)*rhs)*MJI, rhs *= rka.
-template <class Phys, int NQ, int EPW>
+template <class Phys, int NQ, int EPW, bool WIDE>
 struct FusedCfg {
+  // vidM of each face node fits a byte up to Np = 255, else a short
+  using VidT = typename std::conditional<(Shape<Phys, NQ>::Np < 256), uint8_t, uint16_t>::type;
   using S = Shape<Phys, NQ>;
-  static constexpr int NT = round64(EPW * S::Np);
-  static constexpr int FLUX = Phys::DIM * S::Np;               // per state
-  static constexpr int POST = S::NFN + S::Np;                   // sC + sR per state
+  // WIDE: enough threads for every face item of the workgroup in one round
+  static constexpr int NVOL = round64(EPW * S::Np);
+  static constexpr int NFACEIT = round64(EPW * S::NFN);
+  static constexpr int NT = WIDE && NFACEIT > NVOL ? NFACEIT : NVOL;
+  static constexpr int KF = (EPW * S::NFN + NT - 1) / NT;  // face items per thread
+  static constexpr int FLUX = Phys::DIM * S::Np;           // per state
+  static constexpr int POST = S::NFN + S::Np;               // sC + sR per state
   static constexpr int BUF = (FLUX > POST ? FLUX : POST) * Phys::NSR;
 };
 
-template <class Phys, int NQ, int EPW>
-__global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW>::NT)) void stage_kernel(
+// PF: face items per thread whose loads are issued before the volume work
+// (the rest are handled afterwards); PM_LDS: keep the nodal pressure in LDS
+// for the M side of the faces (else recompute it there).
+template <class Phys, int NQ, int EPW, bool WIDE, int PF, bool PM_LDS>
+__global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, WIDE>::NT)) void stage_kernel(
     double* __restrict__ rhs, const double* __restrict__ Qin, double* __restrict__ Qout,
     const double* __restrict__ vgeo, const double* __restrict__ sgeo,
     const double* __restrict__ D, int64_t nreal, int64_t ntotal,
@@ -416,14 +427,18 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW>::NT)) void stage_kernel(
     const int64_t* __restrict__ bndy, double rka, double rkb, double dt, int64_t chunk,
     PhysParams p) {
   using S = Shape<Phys, NQ>;
-  using C = FusedCfg<Phys, NQ, EPW>;
+  using C = FusedCfg<Phys, NQ, EPW, WIDE>;
   constexpr int Np = S::Np, DIM = S::DIM, NS = Phys::NS, NSR = Phys::NSR, NV = Phys::NVGEO;
   constexpr int NFN = S::NFN, Nfp = S::Nfp, NT = C::NT;
+  constexpr int KP = PF < C::KF ? PF : C::KF;
+  constexpr bool SP = Phys::HAS_P && PM_LDS;
+  using VidT = typename C::VidT;
+  constexpr VidT NOVID = (VidT)~(VidT)0;
   __shared__ double sD[NQ * NQ];
   __shared__ double sQ[EPW][NS * Np];
-  __shared__ double sP[EPW][Phys::HAS_P ? Np : 1];
+  __shared__ double sP[EPW][SP ? Np : 1];
   __shared__ double sB[EPW][C::BUF];  // fluxes, then (contributions | staged rhs)
-  __shared__ int sV[EPW][NFN];
+  __shared__ VidT sV[EPW][NFN];
 
   const int64_t blk = xcd_block(blockIdx.x, chunk);
   const int64_t e0 = blk * EPW;
@@ -433,11 +448,16 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW>::NT)) void stage_kernel(
   const int64_t e = e0 + slot;
   const bool valid = slot < EPW && e < nreal;
 
-  // 1. first face item of this thread: metadata now, gather after step 2
-  const int fsl = t / NFN, ffi = t - fsl * NFN;
-  const bool fvalid = t < EPW * NFN && e0 + fsl < nreal;
-  FaceItem<Phys> f0;
-  if (fvalid) face_meta<Phys, NQ>(f0, sgeo, vmapM, vmapP, bndy, e0 + fsl, ffi, ntotal);
+  // 1. this thread's face items (t, t + NT, ...): metadata now
+  FaceItem<Phys> fit[KP];
+  bool fval[KP];
+#pragma unroll
+  for (int k = 0; k < KP; ++k) {
+    const int item = t + k * NT;
+    const int sl = item / NFN;
+    fval[k] = item < EPW * NFN && e0 + sl < nreal;
+    if (fval[k]) face_meta<Phys, NQ>(fit[k], sgeo, vmapM, vmapP, bndy, e0 + sl, item - sl * NFN, ntotal);
+  }
   if (t < NQ * NQ) sD[t] = D[t];
 
   double r[NSR], MJI = 0.0;
@@ -458,7 +478,7 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW>::NT)) void stage_kernel(
 #pragma unroll
     for (int s = 0; s < NS; ++s) sQ[slot][s * Np + n] = q[s];
     const double P = Phys::HAS_P ? Phys::pressure(q, p) : 0.0;
-    if (Phys::HAS_P) sP[slot][n] = P;
+    if (SP) sP[slot][n] = P;
     double F[DIM][NSR];
     Phys::volume_flux(q, P, met, MJ, p, F);
 #pragma unroll
@@ -467,8 +487,10 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW>::NT)) void stage_kernel(
       for (int s = 0; s < NSR; ++s) sB[slot][(d * NSR + s) * Np + n] = F[d][s];
     Phys::source(q, MJ, p, r);
   }
-  // 2. P-side gather of the first face item (stays in flight over step 3)
-  if (fvalid) face_gather<Phys, NQ>(f0, Qin);
+  // 2. P-side gathers (stay in flight over the sweeps)
+#pragma unroll
+  for (int k = 0; k < KP; ++k)
+    if (fval[k]) face_gather<Phys, NQ>(fit[k], Qin);
   lds_barrier();
   // 3. sweeps
   if (valid) {
@@ -482,25 +504,32 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW>::NT)) void stage_kernel(
     for (int s = 0; s < NSR; ++s) sB[slot][NFN * NSR + s * Np + n] = r[s];
   }
   // 4. face fluxes: M side from LDS, P side gathered
-  if (fvalid) {
-    auto qm_at = [&](int s, int v) { return sQ[fsl][s * Np + v]; };
-    auto pm_at = [&](int v, const double*) { return Phys::HAS_P ? sP[fsl][Phys::HAS_P ? v : 0] : 0.0; };
-    double c[NSR];
-    const int vid = face_finish<Phys>(f0, p, qm_at, pm_at, c);
-    sV[fsl][ffi] = vid;
+  auto pm_of = [&](int sl, int v, const double* qm) {
+    if (!Phys::HAS_P) return 0.0;
+    return SP ? sP[sl][SP ? v : 0] : Phys::pressure(qm, p);
+  };
 #pragma unroll
-    for (int s = 0; s < NSR; ++s) sB[fsl][ffi * NSR + s] = c[s];
-  }
-  for (int item = t + NT; item < EPW * NFN; item += NT) {
+  for (int k = 0; k < KP; ++k) {
+    if (!fval[k]) continue;
+    const int item = t + k * NT;
     const int sl = item / NFN, fi = item - sl * NFN;
-    const int64_t ee = e0 + sl;
-    if (ee >= nreal) continue;
     auto qm_at = [&](int s, int v) { return sQ[sl][s * Np + v]; };
-    auto pm_at = [&](int v, const double*) { return Phys::HAS_P ? sP[sl][Phys::HAS_P ? v : 0] : 0.0; };
+    auto pm_at = [&](int v, const double* qm) { return pm_of(sl, v, qm); };
     double c[NSR];
-    const int vid = face_contribution<Phys, NQ>(Qin, sgeo, vmapM, vmapP, bndy, ee, fi, ntotal,
-                                                p, qm_at, pm_at, c);
-    sV[sl][fi] = vid;
+    const int vid = face_finish<Phys>(fit[k], p, qm_at, pm_at, c);
+    sV[sl][fi] = vid < 0 ? NOVID : (VidT)vid;
+#pragma unroll
+    for (int s = 0; s < NSR; ++s) sB[sl][fi * NSR + s] = c[s];
+  }
+  for (int item = t + KP * NT; item < EPW * NFN; item += NT) {
+    const int sl = item / NFN, fi = item - sl * NFN;
+    if (e0 + sl >= nreal) continue;
+    auto qm_at = [&](int s, int v) { return sQ[sl][s * Np + v]; };
+    auto pm_at = [&](int v, const double* qm) { return pm_of(sl, v, qm); };
+    double c[NSR];
+    const int vid = face_contribution<Phys, NQ>(Qin, sgeo, vmapM, vmapP, bndy, e0 + sl, fi,
+                                                ntotal, p, qm_at, pm_at, c);
+    sV[sl][fi] = vid < 0 ? NOVID : (VidT)vid;
 #pragma unroll
     for (int s = 0; s < NSR; ++s) sB[sl][fi * NSR + s] = c[s];
   }
@@ -510,8 +539,8 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW>::NT)) void stage_kernel(
     for (int it = t; it < EPW * 2 * Nfp; it += NT) {
       const int sl = it / (2 * Nfp), fi = 2 * d * Nfp + (it - sl * 2 * Nfp);
       const int64_t ee = e0 + sl;
-      const int v = sV[sl][fi];
-      if (ee < nreal && v >= 0) {
+      const VidT v = sV[sl][fi];
+      if (ee < nreal && v != NOVID) {
         double* rr = &sB[sl][NFN * NSR];
 #pragma unroll
         for (int s = 0; s < NSR; ++s) rr[s * Np + v] = rr[s * Np + v] - sB[sl][fi * NSR + s];

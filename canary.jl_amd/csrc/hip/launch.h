@@ -64,9 +64,14 @@ int launch(int op, const KArgs& a) {
       break;
     }
     case OP_STAGE: {
-      constexpr int NT = FusedCfg<Phys, NQ, EPW>::NT;
+      // measured on 3D N=4 (32^3): one prefetched face item per thread, the
+      // nodal pressure cached in LDS, workgroup sized by the volume nodes
+      constexpr bool WIDE = false;
+      constexpr int PF = 1;
+      constexpr bool PM_LDS = true;
+      constexpr int NT = FusedCfg<Phys, NQ, EPW, WIDE>::NT;
       const int64_t chunk = (nb + 7) / 8;
-      stage_kernel<Phys, NQ, EPW><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
+      stage_kernel<Phys, NQ, EPW, WIDE, PF, PM_LDS><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
           rhs, Q, (double*)a.Qout, (const double*)a.vgeo, (const double*)a.sgeo, a.D, a.nreal,
           a.ntotal, a.vmapM, a.vmapP, a.bndy, a.rka, a.rkb, a.dt, chunk, a.phys);
       break;
