@@ -57,6 +57,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-elems", type=int, default=32768)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--precision-ne", type=int, default=48,
+                    help="elements per direction of the FP32-vs-FP64 precision-form run (0: skip)")
+    ap.add_argument("--precision-steps", type=str, default="1,100,1000",
+                    help="LSRK steps after which the FP32 form's error is reported")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also run one device-to-device copy of this many GiB (untimed), the "
                          "known-size reference for HBM byte counters")
@@ -161,6 +165,10 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(s, args.cpu_sample_elems)
+    if args.precision_ne > 0:
+        del q, q2, rhs, vgeo, sgeo, vM, vP, bnd, Q0, holder, qf
+        torch.cuda.empty_cache()
+        extra["precision_form"] = precision_form(args.precision_ne, s.N, args.precision_steps)
 
     traffic = measured_traffic("euler3d_N%d_%d" % (s.N, args.ne))
     line = {
@@ -192,6 +200,88 @@ def main():
         **extra,
     }
     print(json.dumps(line), flush=True)
+
+
+def precision_form(ne, N, steps_csv, timed_steps=10):
+    """FP32 perturbation form against the FP64 fused path on the same bubble:
+    time per stage of both, and the relative L2 error of rho, rho*u, rho*w and
+    rho*theta of the FP32 form after the given numbers of LSRK54 steps."""
+    import numpy as np
+    import torch
+    import canary_amd.kernels as K
+    import canary_amd.problems as P
+    from canary_amd.perturb import Perturb32
+
+    t0 = time.time()
+    s = P.bubble3d(Ne=ne, N=N, gravity=9.81, dtheta=2.0, jitter=1e-3, seed=1)
+    Q0 = P.bubble_state(s, dtheta=0.0, jitter=0.0)
+    dt = P.cfl(s, s.Q) / N ** np.sqrt(2)
+    print("setup precision form %s: %.1f s" % (s.label, time.time() - t0), file=sys.stderr, flush=True)
+    dev = torch.device("cuda:0")
+
+    def d(a):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to(dev)
+
+    vgeo, sgeo, D = d(s.vgeo), d(s.sgeo), d(s.D)
+    vM, vP, bnd = d(s.vmapM), d(s.vmapP), d(s.elemtobndy)
+    Qinit = d(s.Q)
+    A, B, _ = P.rk_coefs()
+    stream = torch.cuda.current_stream()
+    st = {"q": Qinit.clone(), "q2": torch.empty_like(Qinit), "rhs": torch.zeros_like(Qinit)}
+
+    def run64(nsteps):
+        for _ in range(nsteps):
+            for k in range(5):
+                K.stage_fused(3, N, st["rhs"], st["q"], st["q2"], vgeo, sgeo, D, s.nelem, vM, vP,
+                              bnd, A[(k + 1) % 5], B[k], dt, phys=s.phys)
+                st["q"], st["q2"] = st["q2"], st["q"]
+
+    pf = Perturb32(s, Q0, s.Q)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3
+
+    marks = sorted(int(v) for v in steps_csv.split(","))
+    errors = {}
+    done = 0
+    t64 = t32 = None
+    for m in marks:
+        n = m - done
+        if t64 is None and n >= timed_steps:
+            t64 = timed(lambda: run64(timed_steps)) / (5 * timed_steps)
+            t32 = timed(lambda: pf.run(dt, timed_steps)) / (5 * timed_steps)
+            n -= timed_steps
+        run64(n)
+        pf.run(dt, n)
+        done = m
+        K.check_errors()
+        pf.check()
+        a = pf.full_state().cpu().numpy().reshape(s.Q.shape, order="F")
+        b = st["q"].cpu().numpy().reshape(s.Q.shape, order="F")
+        errors[str(m)] = {nm: float("%.3e" % (np.linalg.norm(a[:, k] - b[:, k]) / np.linalg.norm(b[:, k])))
+                          for k, nm in ((3, "rho"), (0, "rhou"), (2, "rhow"), (4, "rhotheta"))}
+    if t64 is None:
+        run64(1)
+        pf.run(dt, 1)
+        t64 = timed(lambda: run64(timed_steps)) / (5 * timed_steps)
+        t32 = timed(lambda: pf.run(dt, timed_steps)) / (5 * timed_steps)
+    dof = s.nelem * s.Np * 5
+    b32 = Perturb32.bytes_per_element_stage(N)
+    return {"workload": "3D bubble %d^3 hexes N=%d, FP32 perturbation about an FP64 hydrostatic "
+                        "background vs FP64 fused stage" % (ne, N),
+            "fp64_ms_per_stage": round(t64 * 1e3, 4), "fp32_ms_per_stage": round(t32 * 1e3, 4),
+            "speedup_fp32_vs_fp64": round(t64 / t32, 3),
+            "fp32_gdof_stage_s": round(dof / t32 / 1e9, 3),
+            "fp64_gdof_stage_s": round(dof / t64 / 1e9, 3),
+            "fp32_bytes_per_element_stage": b32,
+            "fp32_hbm_frac": round(b32 * s.nelem / t32 / 1e9 / HBM_PEAK_GBS, 4),
+            "rel_l2_vs_fp64": errors, "dt": dt}
 
 
 def measured_traffic(tag):

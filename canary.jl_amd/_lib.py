@@ -65,6 +65,11 @@ _SIGS = {
     "canary_stage_fused": (INT, [INT, INT, INT, INT, P, P, P, P, P, P, I64, I64, P, P, P,
                                  D, D, D, C.POINTER(Phys), P]),
     "canary_check_errors": (INT, [P, P]),
+    "canary_perturb32_prepare": (INT, [INT, I64, I64, P, P, P, P, P, P, P, C.POINTER(Phys), P, P,
+                                       P, P, P, P, P]),
+    "canary_stage_perturb32": (INT, [INT, P, P, P, P, P, P, P, P, P, I64, D, D, D,
+                                     C.POINTER(Phys), P]),
+    "canary_perturb32_reconstruct": (INT, [INT, I64, P, P, P, P]),
     "canary_device_count": (INT, []),
     "canary_gauss_rule": (INT, [INT, INT, INT, P, P]),
     "canary_lglpoints": (INT, [INT, P, P]),

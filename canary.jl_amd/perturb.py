@@ -1,0 +1,101 @@
+"""Precision form of the 3D Euler stage: FP32 perturbation about an FP64
+hydrostatic background (no reference counterpart; the reference evolves Q in
+one precision, euler3d_set2c.jl:754-891).
+
+    pf = Perturb32(setup, Q_background, Q_initial)
+    pf.run(dt, nsteps)          # LSRK54 steps, one fused FP32 launch per stage
+    Q = pf.full_state()         # FP64 (Np, 5, nelem) tensor: Q0 + q'
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib
+from .problems import rk_coefs
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _dev(a, device):
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a).reshape(-1, order="F"))).to(device)
+
+
+class Perturb32:
+    def __init__(self, s, Q0, Q, device="cuda", stream=None):
+        if s.dim != 3 or s.eqn != _lib.EULER:
+            raise ValueError("the precision form is implemented for 3D Euler")
+        if not 2 <= s.N <= 6:
+            raise ValueError("precision form compiled for N = 2..6")
+        self.s = s
+        self.device = torch.device(device)
+        self.stream = stream
+        Np, Nfp, ne = s.Np, (s.N + 1) ** 2, s.nelem
+        f32, f64 = torch.float32, torch.float64
+        dv = self.device
+        self.vg = torch.empty(Np * 11 * ne, dtype=f32, device=dv)
+        self.sg = torch.empty(4 * Nfp * 6 * ne, dtype=f32, device=dv)
+        self.nbr = torch.empty(Nfp * 6 * ne, dtype=torch.int32, device=dv)
+        self.bnd = torch.empty(6 * ne, dtype=torch.int8, device=dv)
+        self.bg = torch.empty(Np * 3 * ne, dtype=f64, device=dv)
+        self.q = torch.empty(Np * 5 * ne, dtype=f32, device=dv)
+        self.q2 = torch.empty_like(self.q)
+        self.rhs = torch.zeros_like(self.q)
+        self.D = _dev(s.D, dv)
+        vgeo, sgeo = _dev(s.vgeo, dv), _dev(s.sgeo, dv)
+        vM, vP, bd = _dev(s.vmapM, dv), _dev(s.vmapP, dv), _dev(s.elemtobndy, dv)
+        q0, qf = _dev(Q0, dv), _dev(Q, dv)
+        rc = lib().canary_perturb32_prepare(s.N, ne, ne, _ptr(vgeo), _ptr(sgeo), _ptr(vM), _ptr(vP),
+                                           _ptr(bd), _ptr(q0), _ptr(qf), C.byref(s.phys),
+                                           _ptr(self.vg), _ptr(self.sg), _ptr(self.nbr),
+                                           _ptr(self.bnd), _ptr(self.bg), _ptr(self.q),
+                                           self._st())
+        check(rc, "perturb32_prepare")
+        self.check()
+        del vgeo, sgeo, vM, vP, bd, q0, qf
+
+    def _st(self):
+        s = torch.cuda.current_stream(self.device) if self.stream is None else self.stream
+        return C.c_void_p(s.cuda_stream)
+
+    def check(self):
+        info = (C.c_int64 * 4)()
+        rc = lib().canary_check_errors(self._st(), info)
+        if rc:
+            raise _lib.CanaryError(rc, "perturb32 (face %d, element %d, value %d)"
+                                   % (info[1], info[2], info[3]))
+
+    def stage(self, rka, rkb, dt):
+        s = self.s
+        rc = lib().canary_stage_perturb32(s.N, _ptr(self.rhs), _ptr(self.q), _ptr(self.q2),
+                                         _ptr(self.bg), _ptr(self.vg), _ptr(self.sg),
+                                         _ptr(self.nbr), _ptr(self.bnd), _ptr(self.D), s.nelem,
+                                         float(rka), float(rkb), float(dt), C.byref(s.phys),
+                                         self._st())
+        check(rc, "stage_perturb32")
+        self.q, self.q2 = self.q2, self.q
+
+    def run(self, dt, nsteps):
+        A, B, _ = rk_coefs()
+        for _ in range(nsteps):
+            for st in range(5):
+                self.stage(A[(st + 1) % 5], B[st], dt)
+
+    def full_state(self):
+        s = self.s
+        out = torch.empty(s.Np * 5 * s.nelem, dtype=torch.float64, device=self.device)
+        check(lib().canary_perturb32_reconstruct(s.N, s.nelem, _ptr(self.q), _ptr(self.bg),
+                                                 _ptr(out), self._st()), "perturb32_reconstruct")
+        return out
+
+    @staticmethod
+    def bytes_per_element_stage(N):
+        """Bytes one element moves per stage, every array counted once: q' r+w
+        and dq r+w (FP32), the FP64 background (3 fields), 11 FP32 metric
+        fields, 4 FP32 face fields, the int32 P-side map and the int8 bc."""
+        Nq = N + 1
+        Np, Nfp = Nq ** 3, Nq * Nq
+        return 4 * (4 * 5 * Np + 11 * Np + 6 * Nfp * 4) + 8 * 3 * Np + 4 * 6 * Nfp + 6

@@ -118,6 +118,29 @@ int canary_stage_fused(int eqn, int dim, int N, int prec, void* rhs, const void*
                        const int64_t* elemtobndy, double rka, double rkb, double dt,
                        const canary_phys* phys, void* stream);
 
+/* ---- precision form (3D Euler set2c, N = 2..6) --------------------------
+ * FP32 perturbation q' = Q - Q0 about an FP64 hydrostatic background Q0, no
+ * reference counterpart: the reference evolves Q in one precision
+ * (euler3d_set2c.jl:754-891).  Arrays (column-major, per element):
+ *   vg32 float (Np,11,nelem) xi_x..zeta_z, MJ, MJI      sg32 float (4,Nfp,6,nelem) nx ny nz sMJ
+ *   nbr  int32 (Nfp,6,nelem) vmapP-1                    bnd8 int8 (6,nelem)
+ *   bg   double (Np,3,nelem) rho0, rhotheta0, P0        qp, rhs float (Np,5,nelem)
+ * prepare: builds them from the FP64 drop-in arrays, the background Q0 and a
+ * full state Q (qp = Q - Q0); requires the canonical vmapM of `mappings`. */
+int canary_perturb32_prepare(int N, int64_t nelem, int64_t nelem_total, const void* vgeo,
+                             const void* sgeo, const int64_t* vmapM, const int64_t* vmapP,
+                             const int64_t* elemtobndy, const void* Q0, const void* Q,
+                             const canary_phys* phys, void* vg32, void* sg32, int32_t* nbr,
+                             int8_t* bnd8, void* bg, void* qp, void* stream);
+/* one fused LSRK stage of the perturbation (qin -> qout, rhs in place) */
+int canary_stage_perturb32(int N, void* rhs, const void* qin, void* qout, const void* bg,
+                           const void* vg32, const void* sg32, const int32_t* nbr,
+                           const int8_t* bnd8, const double* D, int64_t nreal, double rka,
+                           double rkb, double dt, const canary_phys* phys, void* stream);
+/* Q (FP64, (Np,5,nelem)) = Q0 + qp */
+int canary_perturb32_reconstruct(int N, int64_t nelem, const void* qp, const void* bg, void* Q,
+                                 void* stream);
+
 /* Synchronises `stream`, returns the first recorded device error
  * (CANARY_ERR_BC / CANARY_ERR_VMAP or 0), writes (code, face, element,
  * value) of it to info[0..3] if info != NULL, and clears the record. */

@@ -1,0 +1,64 @@
+"""Precision form (FP32 perturbation about an FP64 hydrostatic background)
+against the FP64 GPU path, itself bit-compared with the CPU oracle in
+test_gpu_parity.py.  Bar: relative L2 error of rho, rho*u and rho*theta
+<= 1e-5 after 1, 10 and 100 LSRK54 steps."""
+import numpy as np
+import pytest
+import torch
+
+import canary_amd.kernels as K
+import canary_amd.problems as P
+from canary_amd._lib import CanaryError
+from canary_amd.perturb import Perturb32
+
+from gpu_util import DeviceSetup, dev, host
+
+pytestmark = pytest.mark.gpu
+
+
+def fp64_run(s, ds, Q, dt, nsteps):
+    A, B, _ = P.rk_coefs()
+    q, q2, rhs = dev(Q), torch.empty(Q.size, dtype=torch.float64, device="cuda"), dev(np.zeros_like(Q))
+    for _ in range(nsteps):
+        for st in range(5):
+            K.stage_fused(3, s.N, rhs, q, q2, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM, ds.vmapP,
+                          ds.bndy, A[(st + 1) % 5], B[st], dt, phys=s.phys)
+            q, q2 = q2, q
+    K.check_errors()
+    return host(q, Q.shape)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_perturb32_tracks_fp64(cuda, N):
+    s = P.bubble3d(Ne=5, N=N, dtheta=2.0, jitter=1e-3, seed=4)
+    Q0 = P.bubble_state(s, dtheta=0.0, jitter=0.0)
+    dt = P.cfl(s, s.Q) / s.N ** np.sqrt(2)
+    ds = DeviceSetup(s)
+    pf = Perturb32(s, Q0, s.Q)
+    # round trip: reconstruct(prepare(Q)) == Q up to FP32 rounding of q'
+    Qr = host(pf.full_state(), s.Q.shape)
+    assert np.max(np.abs(Qr - s.Q)) <= 1e-6 * np.max(np.abs(s.Q - Q0)) + 1e-300
+    done = 0
+    for n in (1, 10, 100):
+        pf.run(dt, n - done)
+        done = n
+        pf.check()
+        got = host(pf.full_state(), s.Q.shape)
+        want = fp64_run(s, ds, s.Q, dt, n)
+        e = {nm: rel(got[:, k, :], want[:, k, :]) for k, nm in ((3, "rho"), (0, "rhou"), (2, "rhow"),
+                                                               (4, "rhotheta"))}
+        print("N=%d steps=%d" % (N, n), {k: "%.2e" % v for k, v in e.items()}, flush=True)
+        assert all(v <= 1e-5 for v in e.values()), e
+
+
+def test_perturb32_rejects_noncanonical_vmapM(cuda):
+    s = P.bubble3d(Ne=2, N=3, dtheta=2.0, jitter=0.0)
+    Q0 = P.bubble_state(s, dtheta=0.0, jitter=0.0)
+    s.vmapM = s.vmapM.copy(order="F")
+    s.vmapM[0, 0, 0] += 1
+    with pytest.raises(CanaryError):
+        Perturb32(s, Q0, s.Q)
