@@ -50,6 +50,25 @@ class Vortex(C.Structure):
                 ("beta", "x0", "y0", "u0", "v0", "rho_inf", "p_inf", "gamma", "Lx", "Ly")]
 
 
+class BatchSolve(C.Structure):
+    _fields_ = [("Q", C.c_void_p * 2), ("rhs", C.c_void_p), ("vgeo", C.c_void_p),
+                ("sgeo", C.c_void_p), ("D", C.c_void_p), ("vmapM", C.c_void_p),
+                ("vmapP", C.c_void_p), ("elemtobndy", C.c_void_p), ("nreal", C.c_int64),
+                ("nelem_total", C.c_int64), ("dt", C.c_double), ("phys", Phys),
+                ("N", C.c_int32), ("reserved", C.c_int32)]
+
+
+class P32Solve(C.Structure):
+    _fields_ = [("q", C.c_void_p * 2), ("rhs", C.c_void_p), ("bg", C.c_void_p),
+                ("vg", C.c_void_p), ("sg", C.c_void_p), ("nbr", C.c_void_p), ("bnd", C.c_void_p),
+                ("D", C.c_void_p), ("nreal", C.c_int64), ("dt", C.c_double), ("phys", Phys),
+                ("N", C.c_int32), ("dim", C.c_int32)]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("solve", C.c_int32), ("N", C.c_int16), ("epw", C.c_int16), ("e0", C.c_int64)]
+
+
 _lib = None
 
 P = C.c_void_p
@@ -65,11 +84,15 @@ _SIGS = {
     "canary_stage_fused": (INT, [INT, INT, INT, INT, P, P, P, P, P, P, I64, I64, P, P, P,
                                  D, D, D, C.POINTER(Phys), P]),
     "canary_check_errors": (INT, [P, P]),
-    "canary_perturb32_prepare": (INT, [INT, I64, I64, P, P, P, P, P, P, P, C.POINTER(Phys), P, P,
-                                       P, P, P, P, P]),
-    "canary_stage_perturb32": (INT, [INT, P, P, P, P, P, P, P, P, P, I64, D, D, D,
+    "canary_perturb32_prepare": (INT, [INT, INT, I64, I64, P, P, P, P, P, P, P, C.POINTER(Phys),
+                                       P, P, P, P, P, P, P]),
+    "canary_stage_perturb32": (INT, [INT, INT, P, P, P, P, P, P, P, P, P, I64, D, D, D,
                                      C.POINTER(Phys), P]),
-    "canary_perturb32_reconstruct": (INT, [INT, I64, P, P, P, P]),
+    "canary_perturb32_reconstruct": (INT, [INT, INT, I64, P, P, P, P]),
+    "canary_p32_batch_variant": (INT, [INT, INT, INT, C.POINTER(INT), C.POINTER(INT)]),
+    "canary_p32_batch_stage": (INT, [INT, INT, INT, P, P, I64, INT, D, D, P]),
+    "canary_batch_variant": (INT, [INT, INT, INT, INT, INT, C.POINTER(INT), C.POINTER(INT)]),
+    "canary_batch_stage": (INT, [INT, INT, INT, INT, INT, P, P, I64, INT, D, D, P]),
     "canary_device_count": (INT, []),
     "canary_gauss_rule": (INT, [INT, INT, INT, P, P]),
     "canary_lglpoints": (INT, [INT, P, P]),

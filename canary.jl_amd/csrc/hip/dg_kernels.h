@@ -101,9 +101,9 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t chunk) {
 // r[s] += sum_m D[m, i] F[m, j, k, s] (xi), then eta, then zeta, with the
 // reference CPU order.  sF[d] points at the flux of direction d,
 // laid out [s][node].
-template <int DIM, int NQ, int NSR>
-__device__ __forceinline__ void volume_sweeps(double* r, const double* sD, const double* sF0,
-                                              const double* sF1, const double* sF2, int n) {
+template <int DIM, int NQ, int NSR, class Real>
+__device__ __forceinline__ void volume_sweeps(Real* r, const Real* sD, const Real* sF0,
+                                              const Real* sF1, const Real* sF2, int n) {
   constexpr int Np = ipow(NQ, DIM);
   const int i = n % NQ;
   const int j = (n / NQ) % NQ;
@@ -113,7 +113,7 @@ __device__ __forceinline__ void volume_sweeps(double* r, const double* sD, const
   const int base_zeta = n - k * NQ * NQ;          // (i, j, 0)
 #pragma unroll
   for (int s = 0; s < NSR; ++s) {
-    double a = r[s];
+    Real a = r[s];
 #pragma unroll
     for (int m = 0; m < NQ; ++m) a = a + sD[m + NQ * i] * sF0[s * Np + base_xi + m];
 #pragma unroll
@@ -134,9 +134,10 @@ __device__ __forceinline__ void volume_sweeps(double* r, const double* sD, const
 // times sMJ -- the amount the reference subtracts from rhs).
 template <class Phys>
 struct FaceItem {
-  double nrm[Phys::DIM];
-  double sMJ;
-  double qP[Phys::NS];
+  using Real = typename Phys::Real;
+  Real nrm[Phys::DIM];
+  Real sMJ;
+  Real qP[Phys::NS];
   int64_t eP;
   int vidM;  // -1: no contribution (an error was recorded)
   int vidP;
@@ -152,7 +153,8 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 template <class Phys, int NQ>
-__device__ __forceinline__ void face_meta(FaceItem<Phys>& it, const double* __restrict__ sgeo,
+__device__ __forceinline__ void face_meta(FaceItem<Phys>& it,
+                                          const typename Phys::Real* __restrict__ sgeo,
                                           const int64_t* __restrict__ vmapM,
                                           const int64_t* __restrict__ vmapP,
                                           const int64_t* __restrict__ bndy, int64_t e, int fi,
@@ -161,7 +163,7 @@ __device__ __forceinline__ void face_meta(FaceItem<Phys>& it, const double* __re
   constexpr int Np = S::Np, DIM = Phys::DIM, NSG = Phys::NSGEO;
   const int f = fi / S::Nfp;
   const size_t fo = (size_t)fi + (size_t)S::NFN * (size_t)e;
-  const double* sg = sgeo + (size_t)NSG * fo;
+  const typename Phys::Real* sg = sgeo + (size_t)NSG * fo;
 #pragma unroll
   for (int c = 0; c < DIM; ++c) it.nrm[c] = sg[c];
   it.sMJ = sg[DIM];
@@ -192,10 +194,11 @@ __device__ __forceinline__ void face_meta(FaceItem<Phys>& it, const double* __re
 }
 
 template <class Phys, int NQ>
-__device__ __forceinline__ void face_gather(FaceItem<Phys>& it, const double* __restrict__ Q) {
+__device__ __forceinline__ void face_gather(FaceItem<Phys>& it,
+                                            const typename Phys::Real* __restrict__ Q) {
   constexpr int Np = Shape<Phys, NQ>::Np, NS = Phys::NS;
   if (it.vidM >= 0 && it.bc == 0) {
-    const double* qe = Q + (size_t)NS * Np * (size_t)it.eP;
+    const typename Phys::Real* qe = Q + (size_t)NS * Np * (size_t)it.eP;
 #pragma unroll
     for (int s = 0; s < NS; ++s) it.qP[s] = qe[it.vidP + Np * s];
   }
@@ -204,17 +207,18 @@ __device__ __forceinline__ void face_gather(FaceItem<Phys>& it, const double* __
 // M-side state through qm_at(s, node) and pm_at(node, qM); out[NSR].
 template <class Phys, class QM, class PMF>
 __device__ __forceinline__ int face_finish(FaceItem<Phys>& it, const PhysParams& p, QM qm_at,
-                                           PMF pm_at, double* out) {
+                                           PMF pm_at, typename Phys::Real* out) {
+  using Real = typename Phys::Real;
   constexpr int NS = Phys::NS;
   if (it.vidM < 0) return -1;
-  double qM[NS];
+  Real qM[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) qM[s] = qm_at(s, it.vidM);
-  const double PM = pm_at(it.vidM, qM);
-  double PP = 0.0;
+  const Real PM = pm_at(it.vidM, qM);
+  Real PP = 0;
   if (it.bc == 0) {
     if (Phys::B_FROM_M) it.qP[NS - 1] = qm_at(NS - 1, it.vidP);
-    PP = Phys::HAS_P ? Phys::pressure(it.qP, p) : 0.0;
+    PP = Phys::HAS_P ? Phys::pressure(it.qP, p) : Real(0);
   } else {
     Phys::reflect(qM, PM, it.nrm, it.qP, PP);
   }
@@ -224,13 +228,13 @@ __device__ __forceinline__ int face_finish(FaceItem<Phys>& it, const PhysParams&
 
 // All three steps at once.
 template <class Phys, int NQ, class QM, class PMF>
-__device__ __forceinline__ int face_contribution(const double* __restrict__ Q,
-                                                 const double* __restrict__ sgeo,
+__device__ __forceinline__ int face_contribution(const typename Phys::Real* __restrict__ Q,
+                                                 const typename Phys::Real* __restrict__ sgeo,
                                                  const int64_t* __restrict__ vmapM,
                                                  const int64_t* __restrict__ vmapP,
                                                  const int64_t* __restrict__ bndy, int64_t e,
                                                  int fi, int64_t ntotal, const PhysParams& p,
-                                                 QM qm_at, PMF pm_at, double* out) {
+                                                 QM qm_at, PMF pm_at, typename Phys::Real* out) {
   FaceItem<Phys> it;
   face_meta<Phys, NQ>(it, sgeo, vmapM, vmapP, bndy, e, fi, ntotal);
   face_gather<Phys, NQ>(it, Q);
@@ -405,44 +409,55 @@ struct FusedCfg {
   // vidM of each face node fits a byte up to Np = 255, else a short
   using VidT = typename std::conditional<(Shape<Phys, NQ>::Np < 256), uint8_t, uint16_t>::type;
   using S = Shape<Phys, NQ>;
+  using Real = typename Phys::Real;
   // WIDE: enough threads for every face item of the workgroup in one round
   static constexpr int NVOL = round64(EPW * S::Np);
   static constexpr int NFACEIT = round64(EPW * S::NFN);
   static constexpr int NT = WIDE && NFACEIT > NVOL ? NFACEIT : NVOL;
-  static constexpr int KF = (EPW * S::NFN + NT - 1) / NT;  // face items per thread
   static constexpr int FLUX = Phys::DIM * S::Np;           // per state
   static constexpr int POST = S::NFN + S::Np;               // sC + sR per state
   static constexpr int BUF = (FLUX > POST ? FLUX : POST) * Phys::NSR;
+  // LDS carve-up (bytes, each piece 16-byte aligned)
+  static constexpr int a16(int b) { return (b + 15) / 16 * 16; }
+  static constexpr int OFF_D = 0;
+  static constexpr int OFF_Q = OFF_D + a16(sizeof(Real) * NQ * NQ);
+  static constexpr int OFF_P = OFF_Q + a16(sizeof(Real) * EPW * Phys::NS * S::Np);
+  static constexpr int OFF_B = OFF_P + a16(sizeof(Real) * EPW * (Phys::HAS_P ? S::Np : 1));
+  static constexpr int OFF_V = OFF_B + a16(sizeof(Real) * EPW * BUF);
+  static constexpr int LDS = OFF_V + a16(sizeof(VidT) * EPW * S::NFN);
 };
 
-// PF: face items per thread whose loads are issued before the volume work
-// (the rest are handled afterwards); PM_LDS: keep the nodal pressure in LDS
-// for the M side of the faces (else recompute it there).
-template <class Phys, int NQ, int EPW, bool WIDE, int PF, bool PM_LDS>
-__global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, WIDE>::NT)) void stage_kernel(
-    double* __restrict__ rhs, const double* __restrict__ Qin, double* __restrict__ Qout,
-    const double* __restrict__ vgeo, const double* __restrict__ sgeo,
-    const double* __restrict__ D, int64_t nreal, int64_t ntotal,
+// One fused stage of the EPW-element block starting at e0, computed by NT
+// threads (NT >= the volume nodes of the block).  `lds` holds
+// FusedCfg::LDS bytes.  PF: face items per thread whose loads are issued
+// before the volume work (the rest are handled afterwards); PM_LDS: keep
+// the nodal pressure in LDS for the M side of the faces (else recompute it).
+template <class Phys, int NQ, int EPW, int NT, int PF, bool PM_LDS>
+__device__ __forceinline__ void stage_body(
+    char* __restrict__ lds, typename Phys::Real* __restrict__ rhs,
+    const typename Phys::Real* __restrict__ Qin, typename Phys::Real* __restrict__ Qout,
+    const typename Phys::Real* __restrict__ vgeo, const typename Phys::Real* __restrict__ sgeo,
+    const double* __restrict__ D, int64_t e0, int64_t nreal, int64_t ntotal,
     const int64_t* __restrict__ vmapM, const int64_t* __restrict__ vmapP,
-    const int64_t* __restrict__ bndy, double rka, double rkb, double dt, int64_t chunk,
-    PhysParams p) {
+    const int64_t* __restrict__ bndy, double rka, double rkb, double dt, const PhysParams& p) {
   using S = Shape<Phys, NQ>;
-  using C = FusedCfg<Phys, NQ, EPW, WIDE>;
+  using C = FusedCfg<Phys, NQ, EPW, false>;
+  using Real = typename Phys::Real;
   constexpr int Np = S::Np, DIM = S::DIM, NS = Phys::NS, NSR = Phys::NSR, NV = Phys::NVGEO;
-  constexpr int NFN = S::NFN, Nfp = S::Nfp, NT = C::NT;
-  constexpr int KP = PF < C::KF ? PF : C::KF;
+  constexpr int NFN = S::NFN, Nfp = S::Nfp;
+  constexpr int KF = (EPW * NFN + NT - 1) / NT;
+  constexpr int KP = PF < KF ? PF : KF;
   constexpr bool SP = Phys::HAS_P && PM_LDS;
+  constexpr int BUF = C::BUF;
   using VidT = typename C::VidT;
   constexpr VidT NOVID = (VidT)~(VidT)0;
-  __shared__ double sD[NQ * NQ];
-  __shared__ double sQ[EPW][NS * Np];
-  __shared__ double sP[EPW][SP ? Np : 1];
-  __shared__ double sB[EPW][C::BUF];  // fluxes, then (contributions | staged rhs)
-  __shared__ VidT sV[EPW][NFN];
+  static_assert(NT >= EPW * Np, "one volume node per thread");
+  Real* sD = reinterpret_cast<Real*>(lds + C::OFF_D);
+  Real* sQ = reinterpret_cast<Real*>(lds + C::OFF_Q);  // [EPW][NS*Np]
+  Real* sP = reinterpret_cast<Real*>(lds + C::OFF_P);  // [EPW][Np]
+  Real* sB = reinterpret_cast<Real*>(lds + C::OFF_B);  // [EPW][BUF]
+  VidT* sV = reinterpret_cast<VidT*>(lds + C::OFF_V);  // [EPW][NFN]
 
-  const int64_t blk = xcd_block(blockIdx.x, chunk);
-  const int64_t e0 = blk * EPW;
-  if (e0 >= nreal) return;  // whole workgroup leaves together
   const int t = threadIdx.x;
   const int slot = t / Np, n = t - slot * Np;
   const int64_t e = e0 + slot;
@@ -458,33 +473,33 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, WIDE>::NT)) void stage_ker
     fval[k] = item < EPW * NFN && e0 + sl < nreal;
     if (fval[k]) face_meta<Phys, NQ>(fit[k], sgeo, vmapM, vmapP, bndy, e0 + sl, item - sl * NFN, ntotal);
   }
-  if (t < NQ * NQ) sD[t] = D[t];
+  if (t < NQ * NQ) sD[t] = (Real)D[t];
 
-  double r[NSR], MJI = 0.0;
+  Real r[NSR], MJI = 0;
   if (valid) {
-    double q[NS];
-    const double* qe = Qin + (size_t)NS * Np * (size_t)e;
+    Real q[NS];
+    const Real* qe = Qin + (size_t)NS * Np * (size_t)e;
 #pragma unroll
     for (int s = 0; s < NS; ++s) q[s] = qe[n + Np * s];
-    const double* vg = vgeo + (size_t)NV * Np * (size_t)e;
-    double met[DIM * DIM];
+    const Real* vg = vgeo + (size_t)NV * Np * (size_t)e;
+    Real met[DIM * DIM];
 #pragma unroll
     for (int c = 0; c < DIM * DIM; ++c) met[c] = vg[n + Np * c];
-    const double MJ = vg[n + Np * Phys::iMJ];
+    const Real MJ = vg[n + Np * Phys::iMJ];
     MJI = vg[n + Np * Phys::iMJI];
-    const double* re = rhs + (size_t)NS * Np * (size_t)e;
+    const Real* re = rhs + (size_t)NS * Np * (size_t)e;
 #pragma unroll
     for (int s = 0; s < NSR; ++s) r[s] = re[n + Np * s];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) sQ[slot][s * Np + n] = q[s];
-    const double P = Phys::HAS_P ? Phys::pressure(q, p) : 0.0;
-    if (SP) sP[slot][n] = P;
-    double F[DIM][NSR];
+    for (int s = 0; s < NS; ++s) sQ[slot * NS * Np + s * Np + n] = q[s];
+    const Real P = Phys::HAS_P ? Phys::pressure(q, p) : Real(0);
+    if (SP) sP[slot * Np + n] = P;
+    Real F[DIM][NSR];
     Phys::volume_flux(q, P, met, MJ, p, F);
 #pragma unroll
     for (int d = 0; d < DIM; ++d)
 #pragma unroll
-      for (int s = 0; s < NSR; ++s) sB[slot][(d * NSR + s) * Np + n] = F[d][s];
+      for (int s = 0; s < NSR; ++s) sB[slot * BUF + (d * NSR + s) * Np + n] = F[d][s];
     Phys::source(q, MJ, p, r);
   }
   // 2. P-side gathers (stay in flight over the sweeps)
@@ -494,74 +509,91 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, WIDE>::NT)) void stage_ker
   lds_barrier();
   // 3. sweeps
   if (valid) {
-    const double* b = sB[slot];
+    const Real* b = sB + slot * BUF;
     volume_sweeps<DIM, NQ, NSR>(r, sD, b, b + (DIM > 1 ? NSR * Np : 0),
                                 b + (DIM > 2 ? 2 * NSR * Np : 0), n);
   }
   lds_barrier();  // fluxes consumed; sB is reused below
   if (valid) {
 #pragma unroll
-    for (int s = 0; s < NSR; ++s) sB[slot][NFN * NSR + s * Np + n] = r[s];
+    for (int s = 0; s < NSR; ++s) sB[slot * BUF + NFN * NSR + s * Np + n] = r[s];
   }
   // 4. face fluxes: M side from LDS, P side gathered
-  auto pm_of = [&](int sl, int v, const double* qm) {
-    if (!Phys::HAS_P) return 0.0;
-    return SP ? sP[sl][SP ? v : 0] : Phys::pressure(qm, p);
+  auto pm_of = [&](int sl, int v, const Real* qm) -> Real {
+    if (!Phys::HAS_P) return Real(0);
+    return SP ? sP[sl * Np + v] : Phys::pressure(qm, p);
   };
 #pragma unroll
   for (int k = 0; k < KP; ++k) {
     if (!fval[k]) continue;
     const int item = t + k * NT;
     const int sl = item / NFN, fi = item - sl * NFN;
-    auto qm_at = [&](int s, int v) { return sQ[sl][s * Np + v]; };
-    auto pm_at = [&](int v, const double* qm) { return pm_of(sl, v, qm); };
-    double c[NSR];
+    auto qm_at = [&](int s, int v) { return sQ[sl * NS * Np + s * Np + v]; };
+    auto pm_at = [&](int v, const Real* qm) { return pm_of(sl, v, qm); };
+    Real c[NSR];
     const int vid = face_finish<Phys>(fit[k], p, qm_at, pm_at, c);
-    sV[sl][fi] = vid < 0 ? NOVID : (VidT)vid;
+    sV[sl * NFN + fi] = vid < 0 ? NOVID : (VidT)vid;
 #pragma unroll
-    for (int s = 0; s < NSR; ++s) sB[sl][fi * NSR + s] = c[s];
+    for (int s = 0; s < NSR; ++s) sB[sl * BUF + fi * NSR + s] = c[s];
   }
   for (int item = t + KP * NT; item < EPW * NFN; item += NT) {
     const int sl = item / NFN, fi = item - sl * NFN;
     if (e0 + sl >= nreal) continue;
-    auto qm_at = [&](int s, int v) { return sQ[sl][s * Np + v]; };
-    auto pm_at = [&](int v, const double* qm) { return pm_of(sl, v, qm); };
-    double c[NSR];
+    auto qm_at = [&](int s, int v) { return sQ[sl * NS * Np + s * Np + v]; };
+    auto pm_at = [&](int v, const Real* qm) { return pm_of(sl, v, qm); };
+    Real c[NSR];
     const int vid = face_contribution<Phys, NQ>(Qin, sgeo, vmapM, vmapP, bndy, e0 + sl, fi,
                                                 ntotal, p, qm_at, pm_at, c);
-    sV[sl][fi] = vid < 0 ? NOVID : (VidT)vid;
+    sV[sl * NFN + fi] = vid < 0 ? NOVID : (VidT)vid;
 #pragma unroll
-    for (int s = 0; s < NSR; ++s) sB[sl][fi * NSR + s] = c[s];
+    for (int s = 0; s < NSR; ++s) sB[sl * BUF + fi * NSR + s] = c[s];
   }
   lds_barrier();
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
     for (int it = t; it < EPW * 2 * Nfp; it += NT) {
       const int sl = it / (2 * Nfp), fi = 2 * d * Nfp + (it - sl * 2 * Nfp);
-      const int64_t ee = e0 + sl;
-      const VidT v = sV[sl][fi];
-      if (ee < nreal && v != NOVID) {
-        double* rr = &sB[sl][NFN * NSR];
+      const VidT v = sV[sl * NFN + fi];
+      if (e0 + sl < nreal && v != NOVID) {
+        Real* rr = sB + sl * BUF + NFN * NSR;
 #pragma unroll
-        for (int s = 0; s < NSR; ++s) rr[s * Np + v] = rr[s * Np + v] - sB[sl][fi * NSR + s];
+        for (int s = 0; s < NSR; ++s) rr[s * Np + v] = rr[s * Np + v] - sB[sl * BUF + fi * NSR + s];
       }
     }
     lds_barrier();
   }
-  // 5. update
+  // 5. update: Qout = Q + ((rkb*dt)*rhs)*MJI, rhs *= rka
   if (valid) {
-    const double bdt = rkb * dt;
-    double* qo = Qout + (size_t)NS * Np * (size_t)e;
-    double* re = rhs + (size_t)NS * Np * (size_t)e;
+    const Real bdt = (Real)(rkb * dt);
+    const Real ka = (Real)rka;
+    Real* qo = Qout + (size_t)NS * Np * (size_t)e;
+    Real* re = rhs + (size_t)NS * Np * (size_t)e;
 #pragma unroll
     for (int s = 0; s < NSR; ++s) {
-      const double rs = sB[slot][NFN * NSR + s * Np + n];
-      qo[n + Np * s] = sQ[slot][s * Np + n] + bdt * rs * MJI;
-      re[n + Np * s] = rs * rka;
+      const Real rs = sB[slot * BUF + NFN * NSR + s * Np + n];
+      qo[n + Np * s] = sQ[slot * NS * Np + s * Np + n] + bdt * rs * MJI;
+      re[n + Np * s] = rs * ka;
     }
 #pragma unroll
-    for (int s = NSR; s < NS; ++s) qo[n + Np * s] = sQ[slot][s * Np + n];  // read-only states
+    for (int s = NSR; s < NS; ++s) qo[n + Np * s] = sQ[slot * NS * Np + s * Np + n];  // read-only states
   }
+}
+
+template <class Phys, int NQ, int EPW, bool WIDE, int PF, bool PM_LDS>
+__global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, WIDE>::NT)) void stage_kernel(
+    double* __restrict__ rhs, const double* __restrict__ Qin, double* __restrict__ Qout,
+    const double* __restrict__ vgeo, const double* __restrict__ sgeo,
+    const double* __restrict__ D, int64_t nreal, int64_t ntotal,
+    const int64_t* __restrict__ vmapM, const int64_t* __restrict__ vmapP,
+    const int64_t* __restrict__ bndy, double rka, double rkb, double dt, int64_t chunk,
+    PhysParams p) {
+  using C = FusedCfg<Phys, NQ, EPW, WIDE>;
+  __shared__ __attribute__((aligned(16))) char lds[C::LDS];
+  const int64_t blk = xcd_block(blockIdx.x, chunk);
+  const int64_t e0 = blk * EPW;
+  if (e0 >= nreal) return;  // whole workgroup leaves together
+  stage_body<Phys, NQ, EPW, C::NT, PF, PM_LDS>(lds, rhs, Qin, Qout, vgeo, sgeo, D, e0, nreal,
+                                               ntotal, vmapM, vmapP, bndy, rka, rkb, dt, p);
 }
 
 }  // namespace canary

@@ -1,28 +1,32 @@
-// Precision form of the 3D set2c Euler stage: the state is split into an FP64
-// hydrostatic background Q0 = (0, 0, 0, rho0, rhotheta0) kept on the GPU and
-// an FP32 perturbation q' = Q - Q0, and only q' is evolved.  The right-hand
-// side is the FP32 difference L(Q0 + q') - L(Q0) of the reference operator
-// (euler3d_set2c.jl:144-348):
+// Precision form of the set2c Euler stage (2D and 3D): the state is split
+// into an FP64 background Q0 = (0, .., 0, rho0, rhotheta0) at rest, kept on
+// the GPU, and an FP32 perturbation q' = Q - Q0, and only q' is evolved.
+// The right-hand side is the FP32 difference L(Q0 + q') - L(Q0) of the
+// reference operator (euler3d_set2c.jl:144-348, euler2d_set2c.jl:140-300):
 //   * momentum fluxes rho^-1 U_a U_b + delta_ab P', with the pressure
 //     perturbation P' = P0 expm1(kappa log1p(rhotheta' / rhotheta0)) (no
 //     cancellation against the large background pressure);
 //   * rho and rho*theta fluxes are U and rhotheta U / rho (Q0 is at rest);
-//   * buoyancy -MJ rho' g (the background part balances the background
-//     pressure gradient);
+//   * buoyancy -MJ rho' g on the vertical momentum (the background part
+//     balances the background pressure gradient);
 //   * Rusanov flux of the perturbation with the full-state wave speed.
-// L(Q0) itself is roundoff for the hydrostatic state (measured below 2e-12 of
-// the buoyancy scale), so it is not carried.  Metric terms and normals are
+// For a hydrostatic (or uniform) background L(Q0) is roundoff (measured
+// below 2e-12 of the buoyancy scale in 3D, exactly 0 for a uniform state
+// without gravity), so it is not carried.  Metric terms and normals are
 // computed in FP64 and rounded to FP32; the background is read in FP64; the
 // LSRK update q' + ((rkb*dt) dq) MJI is evaluated in FP64 and rounded once.
 //
-// Data (per element contiguous, reference-style column-major):
-//   qp, rhs  float (Np, 5, nelem)          perturbation state / LSRK register
-//   bg       double (Np, 3, nelem)         rho0, rhotheta0, P0
-//   vg       float (Np, 11, nelem)         xi_x .. zeta_z, MJ, MJI
-//   sg       float (4, Nfp, 6, nelem)      nx, ny, nz, sMJ
-//   nbr      int32 (Nfp, 6, nelem)         0-based P-side global node (vmapP - 1)
-//   bnd      int8  (6, nelem)              elemtobndy
+// Data (per element contiguous, reference-style column-major; NS = dim + 2,
+// NF = 2 dim faces):
+//   qp, rhs  float (Np, NS, nelem)          perturbation state / LSRK register
+//   bg       double (Np, 3, nelem)          rho0, rhotheta0, P0
+//   vg       float (Np, dim*dim + 2, nelem) metric terms, MJ, MJI
+//   sg       float4 (Nfp, NF, nelem)        (nx, ny, nz, sMJ), nz = 0 in 2D
+//   nbr      int32 (Nfp, NF, nelem)         0-based P-side global node (vmapP - 1)
+//   bnd      int8  (NF, nelem)              elemtobndy
 // vmapM must be the canonical face mask of `mappings` (checked by prepare).
+// Beside the single-solve stage there is a batched stage over a work table
+// of (solve, first element) entries, as in batched.hip.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -47,10 +51,26 @@ struct P32Params {
   float gamma, kappa, g;
 };
 
-// canonical volume node of face node (f, fn) in 3D (src/mesh.jl:973-1001)
-template <int NQ>
-__device__ __forceinline__ int fmask3(int f, int fn) {
-  const int a = fn % NQ, b = fn / NQ, N = NQ - 1;
+__device__ __forceinline__ P32Params params_of(const canary_phys& c) {
+  return P32Params{(float)c.gamma, (float)(c.c_p / c.c_v), (float)c.gravity};
+}
+
+template <int DIM>
+struct Dims {
+  static constexpr int NS = DIM + 2;            // U, V, (W), rho, rhotheta
+  static constexpr int RHO = DIM, E = DIM + 1;
+  static constexpr int NV = DIM * DIM + 2;      // FP32 vg fields
+  static constexpr int NVG = DIM == 3 ? 14 : 8; // FP64 vgeo fields
+  static constexpr int NSG = DIM == 3 ? 5 : 4;  // FP64 sgeo fields
+  static constexpr int NF = 2 * DIM;
+};
+
+// canonical volume node of face node (f, fn) (src/mesh.jl:973-1001)
+template <int DIM, int NQ>
+__device__ __forceinline__ int fmask(int f, int fn) {
+  constexpr int N = NQ - 1;
+  if (DIM == 2) return (f >> 1) == 0 ? (f & 1) * N + NQ * fn : fn + NQ * ((f & 1) * N);
+  const int a = fn % NQ, b = fn / NQ;
   switch (f >> 1) {
     case 0: return (f & 1) * N + NQ * a + NQ * NQ * b;
     case 1: return a + NQ * ((f & 1) * N) + NQ * NQ * b;
@@ -63,7 +83,7 @@ __device__ __forceinline__ float pprime(float P0, float rt0, float rtp, float ka
 }
 
 // ------------------------------------------------------------ prepare
-template <int NQ>
+template <int DIM, int NQ>
 __global__ __launch_bounds__(256) void prepare_kernel(
     int64_t nelem, int64_t ntotal, const double* __restrict__ vgeo, const double* __restrict__ sgeo,
     const int64_t* __restrict__ vmapM, const int64_t* __restrict__ vmapP,
@@ -71,37 +91,40 @@ __global__ __launch_bounds__(256) void prepare_kernel(
     double kappa_d, double p0, double R, float* __restrict__ vg, float* __restrict__ sg,
     int32_t* __restrict__ nbr, int8_t* __restrict__ bnd, double* __restrict__ bg,
     float* __restrict__ qp) {
-  constexpr int Np = NQ * NQ * NQ, Nfp = NQ * NQ, NFN = 6 * Nfp;
+  using G = Dims<DIM>;
+  constexpr int Np = ipow(NQ, DIM), Nfp = ipow(NQ, DIM - 1), NFN = G::NF * Nfp, NS = G::NS;
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t g = tid; g < nelem * Np; g += stride) {
     const int64_t e = g / Np;
     const int n = (int)(g - e * Np);
-    const double* v = vgeo + (size_t)14 * Np * e;
-    float* o = vg + (size_t)11 * Np * e;
+    const double* v = vgeo + (size_t)G::NVG * Np * e;
+    float* o = vg + (size_t)G::NV * Np * e;
 #pragma unroll
-    for (int c = 0; c < 11; ++c) o[n + Np * c] = (float)v[n + Np * c];
-    const double* q0 = Q0 + (size_t)5 * Np * e;
-    const double* q = Q + (size_t)5 * Np * e;
-    const double rho0 = q0[n + 3 * Np], rt0 = q0[n + 4 * Np];
+    for (int c = 0; c < G::NV; ++c) o[n + Np * c] = (float)v[n + Np * c];
+    const double* q0 = Q0 + (size_t)NS * Np * e;
+    const double* q = Q + (size_t)NS * Np * e;
+    const double rho0 = q0[n + G::RHO * Np], rt0 = q0[n + G::E * Np];
     double* b = bg + (size_t)3 * Np * e;
     b[n] = rho0;
     b[n + Np] = rt0;
     b[n + 2 * Np] = p0 * pow(R * rt0 / p0, kappa_d);
-    float* qo = qp + (size_t)5 * Np * e;
+    float* qo = qp + (size_t)NS * Np * e;
 #pragma unroll
-    for (int s = 0; s < 5; ++s) qo[n + Np * s] = (float)(q[n + Np * s] - q0[n + Np * s]);
+    for (int s = 0; s < NS; ++s) qo[n + Np * s] = (float)(q[n + Np * s] - q0[n + Np * s]);
   }
   for (int64_t g = tid; g < nelem * NFN; g += stride) {
     const int64_t e = g / NFN;
     const int fi = (int)(g - e * NFN);
     const int f = fi / Nfp, fn = fi - f * Nfp;
-    const double* s = sgeo + (size_t)5 * g;
+    const double* s = sgeo + (size_t)G::NSG * g;
     float* so = sg + (size_t)4 * g;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) so[c] = (float)s[c];
+    so[0] = (float)s[0];
+    so[1] = (float)s[1];
+    so[2] = DIM == 3 ? (float)s[2] : 0.f;
+    so[3] = (float)s[DIM];
     const int64_t idM = vmapM[g], idP = vmapP[g];
-    if (idM != (int64_t)Np * e + fmask3<NQ>(f, fn) + 1) p32_error(5, f + 1, e + 1, idM);
+    if (idM != (int64_t)Np * e + fmask<DIM, NQ>(f, fn) + 1) p32_error(5, f + 1, e + 1, idM);
     int64_t pn = idP - 1;
     if (idP < 1 || idP > ntotal * Np || pn > 0x7fffffffll) {
       p32_error(5, f + 1, e + 1, idP);
@@ -109,60 +132,66 @@ __global__ __launch_bounds__(256) void prepare_kernel(
     }
     nbr[g] = (int32_t)pn;
     if (fn == 0) {
-      const int64_t bc = bndy[f + 6 * e];
+      const int64_t bc = bndy[f + G::NF * e];
       if (bc != 0 && bc != 1) p32_error(4, f + 1, e + 1, bc);
-      bnd[f + 6 * e] = (int8_t)(bc == 0 ? 0 : 1);
+      bnd[f + G::NF * e] = (int8_t)(bc == 0 ? 0 : 1);
     }
   }
 }
 
-template <int NQ>
+template <int DIM, int NQ>
 __global__ __launch_bounds__(256) void reconstruct_kernel(int64_t nelem, const float* __restrict__ qp,
                                                           const double* __restrict__ bg,
                                                           double* __restrict__ Q) {
-  constexpr int Np = NQ * NQ * NQ;
+  using G = Dims<DIM>;
+  constexpr int Np = ipow(NQ, DIM), NS = G::NS;
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < nelem * Np;
        g += (int64_t)gridDim.x * blockDim.x) {
     const int64_t e = g / Np;
     const int n = (int)(g - e * Np);
-    const float* q = qp + (size_t)5 * Np * e;
+    const float* q = qp + (size_t)NS * Np * e;
     const double* b = bg + (size_t)3 * Np * e;
-    double* o = Q + (size_t)5 * Np * e;
-    o[n] = (double)q[n];
-    o[n + Np] = (double)q[n + Np];
-    o[n + 2 * Np] = (double)q[n + 2 * Np];
-    o[n + 3 * Np] = b[n] + (double)q[n + 3 * Np];
-    o[n + 4 * Np] = b[n + Np] + (double)q[n + 4 * Np];
+    double* o = Q + (size_t)NS * Np * e;
+#pragma unroll
+    for (int s = 0; s < DIM; ++s) o[n + s * Np] = (double)q[n + s * Np];
+    o[n + G::RHO * Np] = b[n] + (double)q[n + G::RHO * Np];
+    o[n + G::E * Np] = b[n + Np] + (double)q[n + G::E * Np];
   }
 }
 
 // ------------------------------------------------------------- stage
-template <int NQ>
+template <int DIM, int NQ, int EPW>
 struct Cfg32 {
-  static constexpr int Np = NQ * NQ * NQ, Nfp = NQ * NQ, NFN = 6 * Nfp;
-  static constexpr int EPW = default_epw<Np>();
+  static constexpr int NS = Dims<DIM>::NS;
+  static constexpr int Np = ipow(NQ, DIM), Nfp = ipow(NQ, DIM - 1), NFN = 2 * DIM * Nfp;
   static constexpr int NT = round64(EPW * Np);
-  static constexpr int FLUX = 3 * Np * 5;
-  static constexpr int POST = NFN * 5 + Np * 5;
+  static constexpr int FLUX = DIM * Np * NS;
+  static constexpr int POST = NFN * NS + Np * NS;
   static constexpr int BUF = FLUX > POST ? FLUX : POST;
 };
 
-template <int NQ>
-__global__ __launch_bounds__((Cfg32<NQ>::NT)) void stage32_kernel(
-    float* __restrict__ rhs, const float* __restrict__ qin, float* __restrict__ qout,
-    const double* __restrict__ bg, const float* __restrict__ vg, const float* __restrict__ sg,
-    const int32_t* __restrict__ nbr, const int8_t* __restrict__ bnd, const double* __restrict__ D,
-    int64_t nreal, float rka, double bdt, int64_t chunk, P32Params p) {
-  using C = Cfg32<NQ>;
-  constexpr int Np = C::Np, Nfp = C::Nfp, NFN = C::NFN, EPW = C::EPW, NT = C::NT;
-  __shared__ float sD[NQ * NQ];
-  __shared__ float sQ[EPW][5 * Np];   // perturbation
-  __shared__ float sBg[EPW][4 * Np];  // rho0, rhotheta0, P0, P'
-  __shared__ float sB[EPW][C::BUF];
+template <int DIM, int NQ, int EPW>
+struct Lds32 {
+  using C = Cfg32<DIM, NQ, EPW>;
+  float D[NQ * NQ];
+  float Q[EPW][C::NS * C::Np];  // perturbation
+  float Bg[EPW][4 * C::Np];     // rho0, rhotheta0, P0, P'
+  float B[EPW][C::BUF];         // fluxes, then face terms + rhs
+};
 
-  const int64_t blk = xcd_block(blockIdx.x, chunk);
-  const int64_t e0 = blk * EPW;
-  if (e0 >= nreal) return;
+// One stage of the EPW-element block at e0 (NT threads, NT >= EPW * Np).
+template <int DIM, int NQ, int EPW>
+__device__ __forceinline__ void body(Lds32<DIM, NQ, EPW>& L, float* __restrict__ rhs,
+                                     const float* __restrict__ qin, float* __restrict__ qout,
+                                     const double* __restrict__ bg, const float* __restrict__ vg,
+                                     const float* __restrict__ sg, const int32_t* __restrict__ nbr,
+                                     const int8_t* __restrict__ bnd, const double* __restrict__ D,
+                                     int64_t e0, int64_t nreal, float rka, double bdt,
+                                     const P32Params& p) {
+  using C = Cfg32<DIM, NQ, EPW>;
+  using G = Dims<DIM>;
+  constexpr int Np = C::Np, Nfp = C::Nfp, NFN = C::NFN, NT = C::NT, NS = G::NS;
+  constexpr int RHO = G::RHO, E = G::E, NF = G::NF;
   const int t = threadIdx.x;
   const int slot = t / Np, n = t - slot * Np;
   const int64_t e = e0 + slot;
@@ -174,120 +203,129 @@ __global__ __launch_bounds__((Cfg32<NQ>::NT)) void stage32_kernel(
   float fn4[4] = {0.f, 0.f, 0.f, 0.f};
   int32_t pnode = 0;
   int fbc = 1;
-  float qP[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+  float qP[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) qP[s] = 0.f;
   if (fvalid) {
     const size_t fo = (size_t)ffi + (size_t)NFN * (size_t)(e0 + fsl);
     const float4 s4 = *reinterpret_cast<const float4*>(sg + 4 * fo);
     fn4[0] = s4.x; fn4[1] = s4.y; fn4[2] = s4.z; fn4[3] = s4.w;
     pnode = nbr[fo];
-    fbc = bnd[ffi / Nfp + 6 * (e0 + fsl)];
+    fbc = bnd[ffi / Nfp + NF * (e0 + fsl)];
   }
-  if (t < NQ * NQ) sD[t] = (float)D[t];
+  if (t < NQ * NQ) L.D[t] = (float)D[t];
 
-  float r[5], q[5];
+  float r[NS], q[NS];
   double MJI = 0.0;
   if (valid) {
-    const float* qe = qin + (size_t)5 * Np * e;
+    const float* qe = qin + (size_t)NS * Np * e;
 #pragma unroll
-    for (int s = 0; s < 5; ++s) q[s] = qe[n + Np * s];
-    const float* v = vg + (size_t)11 * Np * e;
-    float met[9];
+    for (int s = 0; s < NS; ++s) q[s] = qe[n + Np * s];
+    const float* v = vg + (size_t)G::NV * Np * e;
+    float met[DIM * DIM];
 #pragma unroll
-    for (int c = 0; c < 9; ++c) met[c] = v[n + Np * c];
-    const float MJ = v[n + Np * 9];
-    MJI = (double)v[n + Np * 10];
+    for (int c = 0; c < DIM * DIM; ++c) met[c] = v[n + Np * c];
+    const float MJ = v[n + Np * DIM * DIM];
+    MJI = (double)v[n + Np * (DIM * DIM + 1)];
     const double* b = bg + (size_t)3 * Np * e;
     const float rho0 = (float)b[n], rt0 = (float)b[n + Np], P0 = (float)b[n + 2 * Np];
-    const float* re = rhs + (size_t)5 * Np * e;
+    const float* re = rhs + (size_t)NS * Np * e;
 #pragma unroll
-    for (int s = 0; s < 5; ++s) r[s] = re[n + Np * s];
-    const float Pp = pprime(P0, rt0, q[4], p.kappa);
+    for (int s = 0; s < NS; ++s) r[s] = re[n + Np * s];
+    const float Pp = pprime(P0, rt0, q[E], p.kappa);
 #pragma unroll
-    for (int s = 0; s < 5; ++s) sQ[slot][s * Np + n] = q[s];
-    sBg[slot][n] = rho0;
-    sBg[slot][Np + n] = rt0;
-    sBg[slot][2 * Np + n] = P0;
-    sBg[slot][3 * Np + n] = Pp;
-    const float rho = rho0 + q[3], rt = rt0 + q[4];
+    for (int s = 0; s < NS; ++s) L.Q[slot][s * Np + n] = q[s];
+    L.Bg[slot][n] = rho0;
+    L.Bg[slot][Np + n] = rt0;
+    L.Bg[slot][2 * Np + n] = P0;
+    L.Bg[slot][3 * Np + n] = Pp;
+    const float rho = rho0 + q[RHO], rt = rt0 + q[E];
     const float ri = 1.0f / rho;
-    const float U = q[0], V = q[1], W = q[2];
-    const float u = U * ri, vv = V * ri, w = W * ri;
-    // physical perturbation fluxes f[c][s], c = x, y, z
-    const float f[3][5] = {{U * u + Pp, V * u, W * u, U, rt * u},
-                           {U * vv, V * vv + Pp, W * vv, V, rt * vv},
-                           {U * w, V * w, W * w + Pp, W, rt * w}};
+    // physical perturbation fluxes f[c][s] along coordinate c
+    float f[DIM][NS];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const float mx = met[d], my = met[d + 3], mz = met[d + 6];
+    for (int c = 0; c < DIM; ++c) {
+      const float uc = q[c] * ri;
 #pragma unroll
-      for (int s = 0; s < 5; ++s)
-        sB[slot][(d * 5 + s) * Np + n] = MJ * (mx * f[0][s] + my * f[1][s] + mz * f[2][s]);
+      for (int s = 0; s < DIM; ++s) f[c][s] = q[s] * uc + (s == c ? Pp : 0.f);
+      f[c][RHO] = q[c];
+      f[c][E] = rt * uc;
     }
-    r[2] -= MJ * q[3] * p.g;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        float a = 0.f;
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) a += met[d + DIM * c] * f[c][s];
+        L.B[slot][(d * NS + s) * Np + n] = MJ * a;
+      }
+    }
+    r[DIM - 1] -= MJ * q[RHO] * p.g;
   }
   if (fvalid && fbc == 0) {
-    const float* qe = qin + (size_t)5 * Np * (size_t)(pnode / Np);
+    const float* qe = qin + (size_t)NS * Np * (size_t)(pnode / Np);
     const int pv = pnode % Np;
 #pragma unroll
-    for (int s = 0; s < 5; ++s) qP[s] = qe[pv + Np * s];
+    for (int s = 0; s < NS; ++s) qP[s] = qe[pv + Np * s];
   }
   lds_barrier();
   if (valid) {
-    const int i = n % NQ, j = (n / NQ) % NQ, k = n / (NQ * NQ);
-    const float* F0 = sB[slot];
-    const float* F1 = F0 + 5 * Np;
-    const float* F2 = F1 + 5 * Np;
-#pragma unroll
-    for (int s = 0; s < 5; ++s) {
-      float a = r[s];
-#pragma unroll
-      for (int m = 0; m < NQ; ++m) a += sD[m + NQ * i] * F0[s * Np + (n - i) + m];
-#pragma unroll
-      for (int m = 0; m < NQ; ++m) a += sD[m + NQ * j] * F1[s * Np + (n - j * NQ) + m * NQ];
-#pragma unroll
-      for (int m = 0; m < NQ; ++m) a += sD[m + NQ * k] * F2[s * Np + (n - k * NQ * NQ) + m * NQ * NQ];
-      r[s] = a;
-    }
+    const float* F0 = L.B[slot];
+    volume_sweeps<DIM, NQ, NS, float>(r, L.D, F0, F0 + NS * Np, F0 + (DIM > 2 ? 2 : 0) * NS * Np,
+                                      n);
   }
   lds_barrier();
   if (valid) {
 #pragma unroll
-    for (int s = 0; s < 5; ++s) sB[slot][NFN * 5 + s * Np + n] = r[s];
+    for (int s = 0; s < NS; ++s) L.B[slot][NFN * NS + s * Np + n] = r[s];
   }
   auto face = [&](int sl, int fi, const float* nn, float* qPl, int bc) {
-    const int vid = fmask3<NQ>(fi / Nfp, fi % Nfp);
-    float qM[5];
+    const int vid = fmask<DIM, NQ>(fi / Nfp, fi % Nfp);
+    float qM[NS];
 #pragma unroll
-    for (int s = 0; s < 5; ++s) qM[s] = sQ[sl][s * Np + vid];
-    const float rho0 = sBg[sl][vid], rt0 = sBg[sl][Np + vid], P0 = sBg[sl][2 * Np + vid];
-    const float PpM = sBg[sl][3 * Np + vid];
-    const float nx = nn[0], ny = nn[1], nz = nn[2], sMJ = nn[3];
+    for (int s = 0; s < NS; ++s) qM[s] = L.Q[sl][s * Np + vid];
+    const float rho0 = L.Bg[sl][vid], rt0 = L.Bg[sl][Np + vid], P0 = L.Bg[sl][2 * Np + vid];
+    const float PpM = L.Bg[sl][3 * Np + vid];
+    const float sMJ = nn[3];
     float PpP;
     if (bc == 0) {
-      PpP = pprime(P0, rt0, qPl[4], p.kappa);
+      PpP = pprime(P0, rt0, qPl[E], p.kappa);
     } else {
-      const float un = nx * qM[0] + ny * qM[1] + nz * qM[2];
-      qPl[0] = qM[0] - 2.f * un * nx;
-      qPl[1] = qM[1] - 2.f * un * ny;
-      qPl[2] = qM[2] - 2.f * un * nz;
-      qPl[3] = qM[3];
-      qPl[4] = qM[4];
+      float un = 0.f;
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) un += nn[c] * qM[c];
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) qPl[c] = qM[c] - 2.f * un * nn[c];
+      qPl[RHO] = qM[RHO];
+      qPl[E] = qM[E];
       PpP = PpM;
     }
-    const float rM = rho0 + qM[3], rP = rho0 + qPl[3];
-    const float rtM = rt0 + qM[4], rtP = rt0 + qPl[4];
-    const float unM = (nx * qM[0] + ny * qM[1] + nz * qM[2]) / rM;
-    const float unP = (nx * qPl[0] + ny * qPl[1] + nz * qPl[2]) / rP;
+    const float rM = rho0 + qM[RHO], rP = rho0 + qPl[RHO];
+    const float rtM = rt0 + qM[E], rtP = rt0 + qPl[E];
+    float UnM = 0.f, UnP = 0.f;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      UnM += nn[c] * qM[c];
+      UnP += nn[c] * qPl[c];
+    }
+    const float unM = UnM / rM, unP = UnP / rP;
     const float lM = fabsf(unM) + sqrtf(p.gamma * (P0 + PpM) / rM);
     const float lP = fabsf(unP) + sqrtf(p.gamma * (P0 + PpP) / rP);
     const float l = fmaxf(lM, lP);
-    const float nfM[5] = {qM[0] * unM + nx * PpM, qM[1] * unM + ny * PpM, qM[2] * unM + nz * PpM,
-                          rM * unM, rtM * unM};
-    const float nfP[5] = {qPl[0] * unP + nx * PpP, qPl[1] * unP + ny * PpP,
-                          qPl[2] * unP + nz * PpP, rP * unP, rtP * unP};
+    float nfM[NS], nfP[NS];
 #pragma unroll
-    for (int s = 0; s < 5; ++s)
-      sB[sl][fi * 5 + s] = sMJ * (0.5f * (nfM[s] + nfP[s] - l * (qPl[s] - qM[s])));
+    for (int c = 0; c < DIM; ++c) {
+      nfM[c] = qM[c] * unM + nn[c] * PpM;
+      nfP[c] = qPl[c] * unP + nn[c] * PpP;
+    }
+    nfM[RHO] = UnM;
+    nfP[RHO] = UnP;
+    nfM[E] = rtM * unM;
+    nfP[E] = rtP * unP;
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      L.B[sl][fi * NS + s] = sMJ * (0.5f * (nfM[s] + nfP[s] - l * (qPl[s] - qM[s])));
   };
   if (fvalid) face(fsl, ffi, fn4, qP, fbc);
   for (int item = t + NT; item < EPW * NFN; item += NT) {
@@ -296,97 +334,217 @@ __global__ __launch_bounds__((Cfg32<NQ>::NT)) void stage32_kernel(
     const size_t fo = (size_t)fi + (size_t)NFN * (size_t)(e0 + sl);
     const float4 s4 = *reinterpret_cast<const float4*>(sg + 4 * fo);
     const float nn[4] = {s4.x, s4.y, s4.z, s4.w};
-    const int bc = bnd[fi / Nfp + 6 * (e0 + sl)];
-    float ql[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    const int bc = bnd[fi / Nfp + NF * (e0 + sl)];
+    float ql[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) ql[s] = 0.f;
     if (bc == 0) {
       const int32_t pn = nbr[fo];
-      const float* qe = qin + (size_t)5 * Np * (size_t)(pn / Np);
+      const float* qe = qin + (size_t)NS * Np * (size_t)(pn / Np);
 #pragma unroll
-      for (int s = 0; s < 5; ++s) ql[s] = qe[pn % Np + Np * s];
+      for (int s = 0; s < NS; ++s) ql[s] = qe[pn % Np + Np * s];
     }
     face(sl, fi, nn, ql, bc);
   }
   lds_barrier();
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
+  for (int d = 0; d < DIM; ++d) {
     for (int it = t; it < EPW * 2 * Nfp; it += NT) {
       const int sl = it / (2 * Nfp), fi = 2 * d * Nfp + (it - sl * 2 * Nfp);
       if (e0 + sl < nreal) {
-        const int v = fmask3<NQ>(fi / Nfp, fi % Nfp);
-        float* rr = &sB[sl][NFN * 5];
+        const int v = fmask<DIM, NQ>(fi / Nfp, fi % Nfp);
+        float* rr = &L.B[sl][NFN * NS];
 #pragma unroll
-        for (int s = 0; s < 5; ++s) rr[s * Np + v] -= sB[sl][fi * 5 + s];
+        for (int s = 0; s < NS; ++s) rr[s * Np + v] -= L.B[sl][fi * NS + s];
       }
     }
     lds_barrier();
   }
   if (valid) {
-    float* qo = qout + (size_t)5 * Np * e;
-    float* re = rhs + (size_t)5 * Np * e;
+    float* qo = qout + (size_t)NS * Np * e;
+    float* re = rhs + (size_t)NS * Np * e;
 #pragma unroll
-    for (int s = 0; s < 5; ++s) {
-      const float rs = sB[slot][NFN * 5 + s * Np + n];
+    for (int s = 0; s < NS; ++s) {
+      const float rs = L.B[slot][NFN * NS + s * Np + n];
       qo[n + Np * s] = (float)((double)q[s] + (bdt * (double)rs) * MJI);
       re[n + Np * s] = rka * rs;
     }
   }
 }
 
+template <int DIM, int NQ>
+constexpr int single_epw() {
+  return default_epw<ipow(NQ, DIM)>();
+}
+
+template <int DIM, int NQ>
+__global__ __launch_bounds__((Cfg32<DIM, NQ, single_epw<DIM, NQ>()>::NT)) void stage32_kernel(
+    float* __restrict__ rhs, const float* __restrict__ qin, float* __restrict__ qout,
+    const double* __restrict__ bg, const float* __restrict__ vg, const float* __restrict__ sg,
+    const int32_t* __restrict__ nbr, const int8_t* __restrict__ bnd, const double* __restrict__ D,
+    int64_t nreal, float rka, double bdt, int64_t chunk, P32Params p) {
+  constexpr int EPW = single_epw<DIM, NQ>();
+  __shared__ Lds32<DIM, NQ, EPW> L;
+  const int64_t e0 = xcd_block(blockIdx.x, chunk) * EPW;
+  if (e0 >= nreal) return;
+  body<DIM, NQ, EPW>(L, rhs, qin, qout, bg, vg, sg, nbr, bnd, D, e0, nreal, rka, bdt, p);
+}
+
+// batched: one workgroup per work-table entry, entries of one XCD contiguous
+template <int DIM, int NQ, int EPW>
+__global__ __launch_bounds__((Cfg32<DIM, NQ, EPW>::NT)) void batch32_kernel(
+    const canary_p32_solve* __restrict__ solves, const canary_batch_item* __restrict__ items,
+    int64_t nitems, int64_t chunk, int parity, double rka, double rkb) {
+  __shared__ Lds32<DIM, NQ, EPW> L;
+  const int64_t w = xcd_block(blockIdx.x, chunk);
+  if (w >= nitems) return;
+  const canary_batch_item it = items[w];
+  const canary_p32_solve& sv = solves[it.solve];
+  if (it.N != NQ - 1 || it.epw != EPW || sv.N != NQ - 1 || sv.dim != DIM) {
+    if (threadIdx.x == 0) p32_error(1 /*CANARY_ERR_ARG*/, it.solve, it.e0 + 1, it.N);
+    return;
+  }
+  if (it.e0 < 0 || it.e0 >= sv.nreal) {
+    if (threadIdx.x == 0) p32_error(1 /*CANARY_ERR_ARG*/, it.solve, it.e0 + 1, sv.nreal);
+    return;
+  }
+  const P32Params p = params_of(sv.phys);
+  body<DIM, NQ, EPW>(L, (float*)sv.rhs, (const float*)sv.q[parity], (float*)sv.q[1 - parity],
+                     (const double*)sv.bg, (const float*)sv.vg, (const float*)sv.sg, sv.nbr,
+                     sv.bnd, sv.D, it.e0, sv.nreal, (float)rka, rkb * sv.dt, p);
+}
+
 int launch_status() { return hipGetLastError() == hipSuccess ? 0 : CANARY_ERR_LAUNCH; }
 
-template <int NQ>
+template <int DIM, int NQ>
 int prepare(int64_t nelem, int64_t ntotal, const void* vgeo, const void* sgeo, const int64_t* vmapM,
             const int64_t* vmapP, const int64_t* bndy, const void* Q0, const void* Q,
             const canary_phys* ph, void* vg, void* sg, int32_t* nbr, int8_t* bnd, void* bg, void* qp,
             hipStream_t st) {
-  const int64_t work = nelem * NQ * NQ * NQ;
+  const int64_t work = nelem * ipow(NQ, DIM);
   int64_t blocks = (work + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  prepare_kernel<NQ><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
+  prepare_kernel<DIM, NQ><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
       nelem, ntotal, (const double*)vgeo, (const double*)sgeo, vmapM, vmapP, bndy,
       (const double*)Q0, (const double*)Q, ph->c_p / ph->c_v, ph->p0, ph->R_gas, (float*)vg,
       (float*)sg, nbr, bnd, (double*)bg, (float*)qp);
   return launch_status();
 }
 
-template <int NQ>
+template <int DIM, int NQ>
 int stage(void* rhs, const void* qin, void* qout, const void* bg, const void* vg, const void* sg,
           const int32_t* nbr, const int8_t* bnd, const double* D, int64_t nreal, double rka,
           double rkb, double dt, const canary_phys* ph, hipStream_t st) {
-  using C = Cfg32<NQ>;
-  const int64_t nb = (nreal + C::EPW - 1) / C::EPW;
+  constexpr int EPW = single_epw<DIM, NQ>();
+  using C = Cfg32<DIM, NQ, EPW>;
+  const int64_t nb = (nreal + EPW - 1) / EPW;
   const int64_t chunk = (nb + 7) / 8;
   P32Params p{(float)ph->gamma, (float)(ph->c_p / ph->c_v), (float)ph->gravity};
-  stage32_kernel<NQ><<<dim3((unsigned)(8 * chunk)), dim3(C::NT), 0, st>>>(
+  stage32_kernel<DIM, NQ><<<dim3((unsigned)(8 * chunk)), dim3(C::NT), 0, st>>>(
       (float*)rhs, (const float*)qin, (float*)qout, (const double*)bg, (const float*)vg,
       (const float*)sg, nbr, bnd, D, nreal, (float)rka, rkb * dt, chunk, p);
   return launch_status();
 }
 
-template <int NQ>
+template <int DIM, int NQ>
 int reconstruct(int64_t nelem, const void* qp, const void* bg, void* Q, hipStream_t st) {
-  const int64_t work = nelem * NQ * NQ * NQ;
+  const int64_t work = nelem * ipow(NQ, DIM);
   int64_t blocks = (work + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  reconstruct_kernel<NQ><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
+  reconstruct_kernel<DIM, NQ><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
       nelem, (const float*)qp, (const double*)bg, (double*)Q);
   return launch_status();
+}
+
+// ------------------------------------------------------ batched variants
+constexpr int MAXT = 256;
+
+// candidate WHICH of elements per workgroup: 0 = most that fit MAXT
+// threads, 1 = about half of that, 2 = the single-solve choice; 0 = none
+template <int DIM, int NQ, int WHICH>
+constexpr int epw_candidate() {
+  constexpr int Np = ipow(NQ, DIM);
+  constexpr int emax = MAXT / Np;
+  if (WHICH == 0) return emax;
+  if (WHICH == 1) return emax >= 2 ? (emax + 1) / 2 : 0;
+  if (WHICH == 2) return single_epw<DIM, NQ>() * Np <= MAXT ? single_epw<DIM, NQ>() : 0;
+  return 0;
+}
+
+template <int DIM, int NQ, int WHICH>
+int batch_which(int* epw, int* threads, const canary_p32_solve* solves,
+                const canary_batch_item* items, int64_t nitems, int parity, double rka,
+                double rkb, hipStream_t st) {
+  constexpr int E = epw_candidate<DIM, NQ, WHICH>();
+  if constexpr (E < 1) {
+    return CANARY_ERR_UNSUPPORTED;
+  } else {
+    constexpr int NT = Cfg32<DIM, NQ, E>::NT;
+    if (epw) {
+      *epw = E;
+      *threads = NT;
+      return 0;
+    }
+    const int64_t chunk = (nitems + 7) / 8;
+    if (chunk > ((int64_t)1 << 27)) return CANARY_ERR_ARG;
+    batch32_kernel<DIM, NQ, E><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, st>>>(
+        solves, items, nitems, chunk, parity, rka, rkb);
+    return launch_status();
+  }
+}
+
+template <int DIM, int NQ>
+int batch_nq(int which, int want, int* epw, int* threads, const canary_p32_solve* solves,
+             const canary_batch_item* items, int64_t nitems, int parity, double rka, double rkb,
+             hipStream_t st) {
+  if (which >= 0) {
+    if (which == 0) return batch_which<DIM, NQ, 0>(epw, threads, 0, 0, 0, 0, 0, 0, 0);
+    if (which == 1) return batch_which<DIM, NQ, 1>(epw, threads, 0, 0, 0, 0, 0, 0, 0);
+    if (which == 2) return batch_which<DIM, NQ, 2>(epw, threads, 0, 0, 0, 0, 0, 0, 0);
+    return CANARY_ERR_UNSUPPORTED;
+  }
+  constexpr int E0 = epw_candidate<DIM, NQ, 0>();
+  constexpr int E1 = epw_candidate<DIM, NQ, 1>();
+  constexpr int E2 = epw_candidate<DIM, NQ, 2>();
+  if (want == E0 && E0 > 0)
+    return batch_which<DIM, NQ, 0>(nullptr, nullptr, solves, items, nitems, parity, rka, rkb, st);
+  if (want == E1 && E1 > 0)
+    return batch_which<DIM, NQ, 1>(nullptr, nullptr, solves, items, nitems, parity, rka, rkb, st);
+  if (want == E2 && E2 > 0)
+    return batch_which<DIM, NQ, 2>(nullptr, nullptr, solves, items, nitems, parity, rka, rkb, st);
+  return CANARY_ERR_UNSUPPORTED;
 }
 
 }  // namespace p32
 }  // namespace canary
 
-#define P32_DISPATCH(N, FN, ARGS)          \
-  switch (N) {                             \
-    case 2: return canary::p32::FN<3> ARGS; \
-    case 3: return canary::p32::FN<4> ARGS; \
-    case 4: return canary::p32::FN<5> ARGS; \
-    case 5: return canary::p32::FN<6> ARGS; \
-    case 6: return canary::p32::FN<7> ARGS; \
-  }                                        \
+// N -> NQ dispatch, N = 1..7 (the batched variants of 3D N >= 6 do not
+// exist: one element has more than 256 nodes)
+#define P32_DISPATCH(DIM_, N, FN, ARGS)                    \
+  if ((DIM_) == 3) {                                       \
+    switch (N) {                                           \
+      case 1: return canary::p32::FN<3, 2> ARGS;           \
+      case 2: return canary::p32::FN<3, 3> ARGS;           \
+      case 3: return canary::p32::FN<3, 4> ARGS;           \
+      case 4: return canary::p32::FN<3, 5> ARGS;           \
+      case 5: return canary::p32::FN<3, 6> ARGS;           \
+      case 6: return canary::p32::FN<3, 7> ARGS;           \
+      case 7: return canary::p32::FN<3, 8> ARGS;           \
+    }                                                      \
+  } else if ((DIM_) == 2) {                                \
+    switch (N) {                                           \
+      case 1: return canary::p32::FN<2, 2> ARGS;           \
+      case 2: return canary::p32::FN<2, 3> ARGS;           \
+      case 3: return canary::p32::FN<2, 4> ARGS;           \
+      case 4: return canary::p32::FN<2, 5> ARGS;           \
+      case 5: return canary::p32::FN<2, 6> ARGS;           \
+      case 6: return canary::p32::FN<2, 7> ARGS;           \
+      case 7: return canary::p32::FN<2, 8> ARGS;           \
+    }                                                      \
+  }                                                        \
   return CANARY_ERR_UNSUPPORTED;
 
-extern "C" int canary_perturb32_prepare(int N, int64_t nelem, int64_t nelem_total,
+extern "C" int canary_perturb32_prepare(int dim, int N, int64_t nelem, int64_t nelem_total,
                                         const void* vgeo, const void* sgeo, const int64_t* vmapM,
                                         const int64_t* vmapP, const int64_t* elemtobndy,
                                         const void* Q0, const void* Q, const canary_phys* phys,
@@ -403,14 +561,15 @@ extern "C" int canary_perturb32_prepare(int N, int64_t nelem, int64_t nelem_tota
     phys = &d;
   }
   hipStream_t st = (hipStream_t)stream;
-  P32_DISPATCH(N, prepare, (nelem, nelem_total, vgeo, sgeo, vmapM, vmapP, elemtobndy, Q0, Q,
-                            phys, vg32, sg32, nbr, bnd8, bg, qp, st))
+  P32_DISPATCH(dim, N, prepare, (nelem, nelem_total, vgeo, sgeo, vmapM, vmapP, elemtobndy, Q0, Q,
+                                 phys, vg32, sg32, nbr, bnd8, bg, qp, st))
 }
 
-extern "C" int canary_stage_perturb32(int N, void* rhs, const void* qin, void* qout, const void* bg,
-                                      const void* vg32, const void* sg32, const int32_t* nbr,
-                                      const int8_t* bnd8, const double* D, int64_t nreal, double rka,
-                                      double rkb, double dt, const canary_phys* phys, void* stream) {
+extern "C" int canary_stage_perturb32(int dim, int N, void* rhs, const void* qin, void* qout,
+                                      const void* bg, const void* vg32, const void* sg32,
+                                      const int32_t* nbr, const int8_t* bnd8, const double* D,
+                                      int64_t nreal, double rka, double rkb, double dt,
+                                      const canary_phys* phys, void* stream) {
   if (nreal < 0) return CANARY_ERR_ARG;
   if (nreal == 0) return 0;
   if (!rhs || !qin || !qout || !bg || !vg32 || !sg32 || !nbr || !bnd8 || !D || qin == qout)
@@ -422,17 +581,34 @@ extern "C" int canary_stage_perturb32(int N, void* rhs, const void* qin, void* q
     phys = &d;
   }
   hipStream_t st = (hipStream_t)stream;
-  P32_DISPATCH(N, stage,
+  P32_DISPATCH(dim, N, stage,
                (rhs, qin, qout, bg, vg32, sg32, nbr, bnd8, D, nreal, rka, rkb, dt, phys, st))
 }
 
-extern "C" int canary_perturb32_reconstruct(int N, int64_t nelem, const void* qp, const void* bg,
-                                            void* Q, void* stream) {
+extern "C" int canary_perturb32_reconstruct(int dim, int N, int64_t nelem, const void* qp,
+                                            const void* bg, void* Q, void* stream) {
   if (nelem < 0) return CANARY_ERR_ARG;
   if (nelem == 0) return 0;
   if (!qp || !bg || !Q) return CANARY_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
-  P32_DISPATCH(N, reconstruct, (nelem, qp, bg, Q, st))
+  P32_DISPATCH(dim, N, reconstruct, (nelem, qp, bg, Q, st))
+}
+
+extern "C" int canary_p32_batch_variant(int dim, int N, int which, int* epw, int* threads) {
+  if (!epw || !threads || which < 0) return CANARY_ERR_ARG;
+  P32_DISPATCH(dim, N, batch_nq,
+               (which, 0, epw, threads, nullptr, nullptr, 0, 0, 0.0, 0.0, nullptr))
+}
+
+extern "C" int canary_p32_batch_stage(int dim, int N, int epw, const canary_p32_solve* solves,
+                                      const canary_batch_item* items, int64_t nitems, int parity,
+                                      double rka, double rkb, void* stream) {
+  if (nitems < 0 || (parity != 0 && parity != 1) || epw < 1) return CANARY_ERR_ARG;
+  if (nitems == 0) return 0;
+  if (!solves || !items) return CANARY_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  P32_DISPATCH(dim, N, batch_nq,
+               (-1, epw, nullptr, nullptr, solves, items, nitems, parity, rka, rkb, st))
 }
 
 extern "C" int canary_k_perturb32_status(int64_t* info, int clear) {

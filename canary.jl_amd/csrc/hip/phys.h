@@ -30,6 +30,7 @@ __device__ __forceinline__ double jmax(double a, double b) {
 
 // ------------------------------------------------------------ 3D Euler set2c
 struct Euler3D {
+  using Real = double;
   static constexpr int DIM = 3, NS = 5, NSR = 5, NVGEO = 14, NSGEO = 5;
   static constexpr int iMJ = 9, iMJI = 10;
   static constexpr bool HAS_P = true, B_FROM_M = false, HAS_ADVECTION = false;
@@ -96,6 +97,7 @@ struct Euler3D {
 
 // ------------------------------------------------------------ 2D Euler set2c
 struct Euler2D {
+  using Real = double;
   static constexpr int DIM = 2, NS = 4, NSR = 4, NVGEO = 8, NSGEO = 4;
   static constexpr int iMJ = 4, iMJI = 5;
   static constexpr bool HAS_P = true, B_FROM_M = false, HAS_ADVECTION = false;
@@ -154,6 +156,7 @@ struct Euler2D {
 // ------------------------------------------------------------ shallow water
 // Q = (H u, H v, h, b); b (bathymetry / still depth) has no tendency.
 struct SWE2D {
+  using Real = double;
   static constexpr int DIM = 2, NS = 4, NSR = 3, NVGEO = 8, NSGEO = 4;
   static constexpr int iMJ = 4, iMJI = 5;
   // B_FROM_M: the reference face kernel reads the P-side b from element eM
@@ -214,6 +217,70 @@ struct SWE2D {
     out[H] = sMJ * ((nx * (fhMx + fhPx) + ny * (fhMy + fhPy) + -(l * (hP - hM))) / 2);
     out[U] = sMJ * ((nx * (fUMx + fUPx) + ny * (fUMy + fUPy) + -(l * (UP - UM))) / 2);
     out[V] = sMJ * ((nx * (fVMx + fVPx) + ny * (fVMy + fVPy) + -(l * (VP - VM))) / 2);
+  }
+};
+
+// --------------------------------------------- shallow water, FP32 variant
+// The same fluxes as SWE2D evaluated in FP32 (batched FP32 ensembles).
+// b = still depth is read only and h is already the deviation from it, so
+// FP32 keeps h's relative precision.
+struct SWE2DF32 {
+  using Real = float;
+  static constexpr int DIM = 2, NS = 4, NSR = 3, NVGEO = 8, NSGEO = 4;
+  static constexpr int iMJ = 4, iMJI = 5;
+  static constexpr bool HAS_P = false, B_FROM_M = true, HAS_ADVECTION = false;
+  enum { U = 0, V = 1, H = 2, B = 3 };
+
+  __device__ static float pressure(const float*, const PhysParams&) { return 0.f; }
+  __device__ static void volume_flux(const float* q, float, const float* met, float MJ,
+                                     const PhysParams& p, float F[DIM][NSR]) {
+    const float g = (float)p.swe_g, dnl = (float)p.dnl;
+    const float Uq = q[U], Vq = q[V], h = q[H], b = q[B];
+    const float Ht = h + b;
+    const float u = Uq / Ht, v = Vq / Ht;
+    const float fUx = (Ht * u * u + 0.5f * g * (h * h)) * dnl + g * h * b;
+    const float fUy = (Ht * u * v) * dnl;
+    const float fVx = (Ht * v * u) * dnl;
+    const float fVy = (Ht * v * v + 0.5f * g * (h * h)) * dnl + g * h * b;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const float mx = met[d], my = met[d + 2];
+      F[d][H] = MJ * (mx * Uq + my * Vq);
+      F[d][U] = MJ * (mx * fUx + my * fUy);
+      F[d][V] = MJ * (mx * fVx + my * fVy);
+    }
+  }
+  __device__ static void source(const float*, float, const PhysParams&, float*) {}
+  __device__ static void reflect(const float* qM, float, const float* n, float* qP, float& PP) {
+    const float UnM = n[0] * qM[U] + n[1] * qM[V];
+    qP[U] = qM[U] - 2.f * UnM * n[0];
+    qP[V] = qM[V] - 2.f * UnM * n[1];
+    qP[H] = qM[H];
+    qP[B] = qM[B];
+    PP = 0.f;
+  }
+  __device__ static void face_flux(const float* qM, float, const float* qP, float,
+                                   const float* n, float sMJ, const PhysParams& p, float* out) {
+    const float g = (float)p.swe_g, dnl = (float)p.dnl;
+    const float nx = n[0], ny = n[1];
+    const float hM = qM[H], bM = qM[B], UM = qM[U], VM = qM[V];
+    const float hP = qP[H], bP = qP[B], UP = qP[U], VP = qP[V];
+    const float HM = hM + bM, uM = UM / HM, vM = VM / HM;
+    const float HP = hP + bP, uP = UP / HP, vP = VP / HP;
+    const float fUMx = (HM * uM * uM + 0.5f * g * (hM * hM)) * dnl + g * hM * bM;
+    const float fUMy = HM * uM * vM * dnl;
+    const float fVMx = HM * vM * uM * dnl;
+    const float fVMy = (HM * vM * vM + 0.5f * g * (hM * hM)) * dnl + g * hM * bM;
+    const float fUPx = (HP * uP * uP + 0.5f * g * (hP * hP)) * dnl + g * hP * bP;
+    const float fUPy = HP * uP * vP * dnl;
+    const float fVPx = HP * vP * uP * dnl;
+    const float fVPy = (HP * vP * vP + 0.5f * g * (hP * hP)) * dnl + g * hP * bP;
+    const float lM = (fabsf(nx * uM + ny * vM) + sqrtf(g * HM)) * dnl + sqrtf(g * bM) * (1.f - dnl);
+    const float lP = (fabsf(nx * uP + ny * vP) + sqrtf(g * HP)) * dnl + sqrtf(g * bP) * (1.f - dnl);
+    const float l = fmaxf(lM, lP);
+    out[H] = sMJ * ((nx * (UM + UP) + ny * (VM + VP) - l * (hP - hM)) * 0.5f);
+    out[U] = sMJ * ((nx * (fUMx + fUPx) + ny * (fUMy + fUPy) - l * (UP - UM)) * 0.5f);
+    out[V] = sMJ * ((nx * (fVMx + fVPx) + ny * (fVMy + fVPy) - l * (VP - VM)) * 0.5f);
   }
 };
 

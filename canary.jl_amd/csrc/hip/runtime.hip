@@ -14,6 +14,7 @@ int canary_k_euler3d_status(int64_t* info, int clear);
 int canary_k_euler2d_status(int64_t* info, int clear);
 int canary_k_swe2d_status(int64_t* info, int clear);
 int canary_k_perturb32_status(int64_t* info, int clear);
+int canary_k_batch_status(int64_t* info, int clear);
 }
 
 namespace {
@@ -163,8 +164,9 @@ extern "C" int canary_check_errors(void* stream, int64_t* info) {
   if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return CANARY_ERR_LAUNCH;
   int64_t tmp[4];
   int first = 0;
-  int (*readers[4])(int64_t*, int) = {canary_k_euler3d_status, canary_k_euler2d_status,
-                                      canary_k_swe2d_status, canary_k_perturb32_status};
+  int (*readers[5])(int64_t*, int) = {canary_k_euler3d_status, canary_k_euler2d_status,
+                                      canary_k_swe2d_status, canary_k_perturb32_status,
+                                      canary_k_batch_status};
   for (auto rd : readers) {
     int rc = rd(tmp, 1);
     if (rc && !first) {

@@ -1,10 +1,11 @@
-"""Precision form of the 3D Euler stage: FP32 perturbation about an FP64
-hydrostatic background (no reference counterpart; the reference evolves Q in
-one precision, euler3d_set2c.jl:754-891).
+"""Precision form of the 2D / 3D Euler stage: FP32 perturbation about an
+FP64 background at rest (hydrostatic, or uniform without gravity); no
+reference counterpart: the reference evolves Q in one precision
+(euler3d_set2c.jl:754-891, euler2d_set2c.jl:639-775).
 
     pf = Perturb32(setup, Q_background, Q_initial)
     pf.run(dt, nsteps)          # LSRK54 steps, one fused FP32 launch per stage
-    Q = pf.full_state()         # FP64 (Np, 5, nelem) tensor: Q0 + q'
+    Q = pf.full_state()         # FP64 (Np, dim+2, nelem) tensor: Q0 + q'
 """
 import ctypes as C
 
@@ -26,29 +27,30 @@ def _dev(a, device):
 
 class Perturb32:
     def __init__(self, s, Q0, Q, device="cuda", stream=None):
-        if s.dim != 3 or s.eqn != _lib.EULER:
-            raise ValueError("the precision form is implemented for 3D Euler")
-        if not 2 <= s.N <= 6:
-            raise ValueError("precision form compiled for N = 2..6")
+        if s.eqn != _lib.EULER or s.dim not in (2, 3):
+            raise ValueError("the precision form is implemented for 2D / 3D Euler")
+        if not 1 <= s.N <= 7:
+            raise ValueError("precision form compiled for N = 1..7")
         self.s = s
         self.device = torch.device(device)
         self.stream = stream
-        Np, Nfp, ne = s.Np, (s.N + 1) ** 2, s.nelem
+        d = s.dim
+        Np, Nfp, ne, nf, ns = s.Np, (s.N + 1) ** (d - 1), s.nelem, 2 * d, d + 2
         f32, f64 = torch.float32, torch.float64
         dv = self.device
-        self.vg = torch.empty(Np * 11 * ne, dtype=f32, device=dv)
-        self.sg = torch.empty(4 * Nfp * 6 * ne, dtype=f32, device=dv)
-        self.nbr = torch.empty(Nfp * 6 * ne, dtype=torch.int32, device=dv)
-        self.bnd = torch.empty(6 * ne, dtype=torch.int8, device=dv)
+        self.vg = torch.empty(Np * (d * d + 2) * ne, dtype=f32, device=dv)
+        self.sg = torch.empty(4 * Nfp * nf * ne, dtype=f32, device=dv)
+        self.nbr = torch.empty(Nfp * nf * ne, dtype=torch.int32, device=dv)
+        self.bnd = torch.empty(nf * ne, dtype=torch.int8, device=dv)
         self.bg = torch.empty(Np * 3 * ne, dtype=f64, device=dv)
-        self.q = torch.empty(Np * 5 * ne, dtype=f32, device=dv)
+        self.q = torch.empty(Np * ns * ne, dtype=f32, device=dv)
         self.q2 = torch.empty_like(self.q)
         self.rhs = torch.zeros_like(self.q)
         self.D = _dev(s.D, dv)
         vgeo, sgeo = _dev(s.vgeo, dv), _dev(s.sgeo, dv)
         vM, vP, bd = _dev(s.vmapM, dv), _dev(s.vmapP, dv), _dev(s.elemtobndy, dv)
         q0, qf = _dev(Q0, dv), _dev(Q, dv)
-        rc = lib().canary_perturb32_prepare(s.N, ne, ne, _ptr(vgeo), _ptr(sgeo), _ptr(vM), _ptr(vP),
+        rc = lib().canary_perturb32_prepare(d, s.N, ne, ne, _ptr(vgeo), _ptr(sgeo), _ptr(vM), _ptr(vP),
                                            _ptr(bd), _ptr(q0), _ptr(qf), C.byref(s.phys),
                                            _ptr(self.vg), _ptr(self.sg), _ptr(self.nbr),
                                            _ptr(self.bnd), _ptr(self.bg), _ptr(self.q),
@@ -70,7 +72,7 @@ class Perturb32:
 
     def stage(self, rka, rkb, dt):
         s = self.s
-        rc = lib().canary_stage_perturb32(s.N, _ptr(self.rhs), _ptr(self.q), _ptr(self.q2),
+        rc = lib().canary_stage_perturb32(s.dim, s.N, _ptr(self.rhs), _ptr(self.q), _ptr(self.q2),
                                          _ptr(self.bg), _ptr(self.vg), _ptr(self.sg),
                                          _ptr(self.nbr), _ptr(self.bnd), _ptr(self.D), s.nelem,
                                          float(rka), float(rkb), float(dt), C.byref(s.phys),
@@ -86,16 +88,18 @@ class Perturb32:
 
     def full_state(self):
         s = self.s
-        out = torch.empty(s.Np * 5 * s.nelem, dtype=torch.float64, device=self.device)
-        check(lib().canary_perturb32_reconstruct(s.N, s.nelem, _ptr(self.q), _ptr(self.bg),
+        out = torch.empty(s.Np * s.nstate * s.nelem, dtype=torch.float64, device=self.device)
+        check(lib().canary_perturb32_reconstruct(s.dim, s.N, s.nelem, _ptr(self.q), _ptr(self.bg),
                                                  _ptr(out), self._st()), "perturb32_reconstruct")
         return out
 
     @staticmethod
-    def bytes_per_element_stage(N):
+    def bytes_per_element_stage(N, dim=3):
         """Bytes one element moves per stage, every array counted once: q' r+w
-        and dq r+w (FP32), the FP64 background (3 fields), 11 FP32 metric
-        fields, 4 FP32 face fields, the int32 P-side map and the int8 bc."""
+        and dq r+w (FP32), the FP64 background (3 fields), dim^2 + 2 FP32
+        metric fields, 4 FP32 face fields, the int32 P-side map and the int8
+        bc."""
         Nq = N + 1
-        Np, Nfp = Nq ** 3, Nq * Nq
-        return 4 * (4 * 5 * Np + 11 * Np + 6 * Nfp * 4) + 8 * 3 * Np + 4 * 6 * Nfp + 6
+        Np, Nfp, nf, ns = Nq ** dim, Nq ** (dim - 1), 2 * dim, dim + 2
+        return (4 * (4 * ns * Np + (dim * dim + 2) * Np + nf * Nfp * 4) + 8 * 3 * Np
+                + 4 * nf * Nfp + nf)

@@ -49,6 +49,7 @@ class Setup:
     phys: object = None
     Q: np.ndarray = None
     label: str = ""
+    Q0: np.ndarray = None  # background at rest for the precision form (None: not defined)
 
     @property
     def Nq(self):
@@ -119,6 +120,17 @@ def vortex_state(s, t=0.0, params=None):
     return Q
 
 
+def uniform_rest_state(s, rho, p):
+    """Euler state at rest with density rho and pressure p everywhere:
+    rho*theta from P = p0 (R rho theta / p0)^(c_p / c_v)."""
+    ph = s.phys
+    rt = (ph.p0 / ph.R_gas) * (p / ph.p0) ** (ph.c_v / ph.c_p)
+    Q = s.zeros_state()
+    Q[:, s.dim, :] = rho
+    Q[:, s.dim + 1, :] = rt
+    return Q
+
+
 def swe_state(s, bumps, depth=10.0, L=(1.0, 1.0)):
     bumps = np.ascontiguousarray(np.asarray(bumps, dtype=np.float64).reshape(-1, 4))
     Q = s.zeros_state()
@@ -131,6 +143,43 @@ def cfl(s, Q):
     """Reference cfl() value (before the `/ N^sqrt(2)`)."""
     Qf = np.asfortranarray(Q)
     return lib().canary_cfl(s.eqn, s.dim, s.N, _p(s.vgeo), _p(Qf), s.nelem, C.byref(s.phys))
+
+
+def swe_courant(s, Q, advection=False):
+    """Time step of the shallow-water example (swe2d.jl:100-124):
+    min over nodes of 0.5 dx / wave_speed / N with
+    dx = sqrt((1/(2 xi_x))^2 + (1/(2 eta_y))^2) and
+    wave_speed = |u| + delta_wave sqrt(g H) delta_nl + g b (1 - delta_nl)."""
+    Q = np.asarray(Q)
+    g, dnl = s.phys.swe_gravity, s.phys.delta_nl
+    h, b, U, V = Q[:, 2, :], Q[:, 3, :], Q[:, 0, :], Q[:, 1, :]
+    xix, etay = s.vgeo[:, 0, :], s.vgeo[:, 3, :]
+    H = h + b
+    u, v = U / H, V / H
+    dx = np.sqrt((1.0 / (2 * xix)) ** 2 + (1.0 / (2 * etay)) ** 2)
+    vel = np.sqrt(u ** 2 + v ** 2)
+    dwave = 0.0 if advection else 1.0
+    wave = vel + dwave * np.sqrt(g * H) * dnl + g * b * (1 - dnl)
+    return float(np.min(0.5 * dx / wave / s.N))
+
+
+def euler_courant(s, Q, courant=0.5):
+    """Courant-number time step for the Euler sets, in the form of the
+    shallow-water example's courantnumber (swe2d.jl:100-124):
+    min over nodes of courant * dx / wave_speed / N with
+    dx = sqrt(sum_d (1/(2 xi^d_d))^2) and wave_speed = |u| + c,
+    P = p0 (R rho theta / p0)^(c_p/c_v), c = sqrt(gamma P / rho).  (The
+    Euler drivers' cfl() leaves the sound speed unscaled by the metric, which
+    is only stable on kilometre-scale boxes.)"""
+    Q = np.asarray(Q)
+    d, ph = s.dim, s.phys
+    rho, rt = Q[:, d, :], Q[:, d + 1, :]
+    vel = np.sqrt(sum((Q[:, c, :] / rho) ** 2 for c in range(d)))
+    P = ph.p0 * (ph.R_gas * rt / ph.p0) ** (ph.c_p / ph.c_v)
+    c = np.sqrt(ph.gamma * P / rho)
+    diag = (0, 4, 8) if d == 3 else (0, 3)   # xi_x, eta_y(, zeta_z) in vgeo
+    dx = np.sqrt(sum((1.0 / (2 * s.vgeo[:, k, :])) ** 2 for k in diag))
+    return float(np.min(courant * dx / (vel + c) / s.N))
 
 
 def l2energysquared(s, Q):
@@ -151,6 +200,7 @@ def bubble3d(Ne=32, N=4, L=1000.0, gravity=9.81, dtheta=2.0, jitter=1e-3, seed=1
     s = build(_lib.EULER, 3, N, (x, x, x), per, phys=phys,
               label="euler3d bubble %d^3 N=%d" % (Ne, N))
     s.Q = bubble_state(s, dtheta=dtheta, spherical=spherical, jitter=jitter, seed=seed)
+    s.Q0 = bubble_state(s, dtheta=0.0, spherical=spherical)  # hydrostatic background
     return s
 
 
@@ -161,6 +211,7 @@ def bubble2d(Ne=10, N=4, L=1000.0, gravity=10.0, dtheta=0.5, jitter=0.0, seed=1)
     s = build(_lib.EULER, 2, N, (x, x), (True, False), phys=phys,
               label="euler2d bubble %d^2 N=%d" % (Ne, N))
     s.Q = bubble_state(s, dtheta=dtheta, center=(500.0, 350.0), jitter=jitter, seed=seed)
+    s.Q0 = bubble_state(s, dtheta=0.0, center=(500.0, 350.0))
     return s
 
 
@@ -177,6 +228,7 @@ def vortex2d(Ne=10, N=4, half=5.0, params=None):
               label="euler2d vortex %d^2 N=%d" % (Ne, N))
     s.Q = vortex_state(s, 0.0, params)
     s.vortex = params
+    s.Q0 = uniform_rest_state(s, params.rho_inf, params.p_inf)
     return s
 
 

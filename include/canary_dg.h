@@ -118,28 +118,99 @@ int canary_stage_fused(int eqn, int dim, int N, int prec, void* rhs, const void*
                        const int64_t* elemtobndy, double rka, double rkb, double dt,
                        const canary_phys* phys, void* stream);
 
-/* ---- precision form (3D Euler set2c, N = 2..6) --------------------------
- * FP32 perturbation q' = Q - Q0 about an FP64 hydrostatic background Q0, no
- * reference counterpart: the reference evolves Q in one precision
- * (euler3d_set2c.jl:754-891).  Arrays (column-major, per element):
- *   vg32 float (Np,11,nelem) xi_x..zeta_z, MJ, MJI      sg32 float (4,Nfp,6,nelem) nx ny nz sMJ
- *   nbr  int32 (Nfp,6,nelem) vmapP-1                    bnd8 int8 (6,nelem)
- *   bg   double (Np,3,nelem) rho0, rhotheta0, P0        qp, rhs float (Np,5,nelem)
+/* ---- precision form (Euler set2c, dim 2 or 3, N = 1..7) -------------------
+ * FP32 perturbation q' = Q - Q0 about an FP64 background Q0 at rest
+ * (hydrostatic, or uniform without gravity); no reference counterpart: the
+ * reference evolves Q in one precision (euler3d_set2c.jl:754-891,
+ * euler2d_set2c.jl:639-775).  Arrays (column-major, per element; NS = dim+2,
+ * NF = 2 dim):
+ *   vg32 float (Np,dim*dim+2,nelem) metric, MJ, MJI   sg32 float (4,Nfp,NF,nelem) nx ny nz sMJ
+ *   nbr  int32 (Nfp,NF,nelem) vmapP-1                  bnd8 int8 (NF,nelem)
+ *   bg   double (Np,3,nelem) rho0, rhotheta0, P0       qp, rhs float (Np,NS,nelem)
  * prepare: builds them from the FP64 drop-in arrays, the background Q0 and a
  * full state Q (qp = Q - Q0); requires the canonical vmapM of `mappings`. */
-int canary_perturb32_prepare(int N, int64_t nelem, int64_t nelem_total, const void* vgeo,
-                             const void* sgeo, const int64_t* vmapM, const int64_t* vmapP,
-                             const int64_t* elemtobndy, const void* Q0, const void* Q,
-                             const canary_phys* phys, void* vg32, void* sg32, int32_t* nbr,
-                             int8_t* bnd8, void* bg, void* qp, void* stream);
+int canary_perturb32_prepare(int dim, int N, int64_t nelem, int64_t nelem_total,
+                             const void* vgeo, const void* sgeo, const int64_t* vmapM,
+                             const int64_t* vmapP, const int64_t* elemtobndy, const void* Q0,
+                             const void* Q, const canary_phys* phys, void* vg32, void* sg32,
+                             int32_t* nbr, int8_t* bnd8, void* bg, void* qp, void* stream);
 /* one fused LSRK stage of the perturbation (qin -> qout, rhs in place) */
-int canary_stage_perturb32(int N, void* rhs, const void* qin, void* qout, const void* bg,
-                           const void* vg32, const void* sg32, const int32_t* nbr,
-                           const int8_t* bnd8, const double* D, int64_t nreal, double rka,
-                           double rkb, double dt, const canary_phys* phys, void* stream);
-/* Q (FP64, (Np,5,nelem)) = Q0 + qp */
-int canary_perturb32_reconstruct(int N, int64_t nelem, const void* qp, const void* bg, void* Q,
-                                 void* stream);
+int canary_stage_perturb32(int dim, int N, void* rhs, const void* qin, void* qout,
+                           const void* bg, const void* vg32, const void* sg32,
+                           const int32_t* nbr, const int8_t* bnd8, const double* D,
+                           int64_t nreal, double rka, double rkb, double dt,
+                           const canary_phys* phys, void* stream);
+/* Q (FP64, (Np,NS,nelem)) = Q0 + qp */
+int canary_perturb32_reconstruct(int dim, int N, int64_t nelem, const void* qp, const void* bg,
+                                 void* Q, void* stream);
+
+/* ---- batched form ---------------------------------------------------------
+ * Many independent solves of one equation set / dimension, each with its own
+ * N, element count, mesh, dt and constants, advanced through a device work
+ * table of element blocks.  No reference counterpart (the reference runs one
+ * solve per process, euler3d_set2c.jl:1049).  Arrays of a solve are the
+ * drop-in ones (FP64), or FP32 copies of Q, rhs, vgeo and sgeo for
+ * prec = CANARY_FP32 (shallow water only).  Both tables live in device
+ * memory. */
+typedef struct canary_batch_solve {
+  void* Q[2];                  /* ping-pong states; stage reads Q[parity], writes Q[1-parity] */
+  void* rhs;                   /* LSRK register                                  */
+  const void* vgeo;
+  const void* sgeo;
+  const double* D;
+  const int64_t* vmapM;
+  const int64_t* vmapP;
+  const int64_t* elemtobndy;
+  int64_t nreal, nelem_total;
+  double dt;
+  canary_phys phys;
+  int32_t N;
+  int32_t reserved;
+} canary_batch_solve;
+
+typedef struct canary_batch_item {
+  int32_t solve;  /* index into the solve table          */
+  int16_t N;      /* polynomial order of that solve      */
+  int16_t epw;    /* elements per workgroup (variant)    */
+  int64_t e0;     /* first element of the block (0-based) */
+} canary_batch_item;
+
+/* Candidate `which` (0, 1, 2) of elements-per-workgroup for (eqn, dim, prec,
+ * N): writes epw and the threads per workgroup of that variant; returns
+ * CANARY_ERR_UNSUPPORTED for a candidate that does not exist. */
+int canary_batch_variant(int eqn, int dim, int prec, int N, int which, int* epw, int* threads);
+/* One LSRK stage of the `nitems` work-table entries at `items`, all of
+ * order N and variant epw: Q[parity] -> Q[1-parity] of their solves.  An
+ * entry of another N / epw, or an e0 outside its solve, is skipped and
+ * recorded as CANARY_ERR_ARG (canary_check_errors).  One workgroup per
+ * entry; consecutive entries land on the same XCD (shared L2). */
+int canary_batch_stage(int eqn, int dim, int prec, int N, int epw,
+                       const canary_batch_solve* solves, const canary_batch_item* items,
+                       int64_t nitems, int parity, double rka, double rkb, void* stream);
+
+/* Batched precision form: one solve of the table is the set of arrays of
+ * the precision form above (q[parity] -> q[1-parity]); items as for
+ * canary_batch_stage.  3D needs N <= 5 (Np <= 256). */
+typedef struct canary_p32_solve {
+  void* q[2];
+  void* rhs;
+  const void* bg;
+  const void* vg;
+  const void* sg;
+  const int32_t* nbr;
+  const int8_t* bnd;
+  const double* D;
+  int64_t nreal;
+  double dt;
+  canary_phys phys;
+  int32_t N;
+  int32_t dim;
+} canary_p32_solve;
+
+int canary_p32_batch_variant(int dim, int N, int which, int* epw, int* threads);
+int canary_p32_batch_stage(int dim, int N, int epw, const canary_p32_solve* solves,
+                           const canary_batch_item* items, int64_t nitems, int parity,
+                           double rka, double rkb, void* stream);
 
 /* Synchronises `stream`, returns the first recorded device error
  * (CANARY_ERR_BC / CANARY_ERR_VMAP or 0), writes (code, face, element,

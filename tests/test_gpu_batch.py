@@ -1,0 +1,212 @@
+"""Batched form on the GPU.
+
+Every element of a batched stage runs the same exact-order FP64 code as the
+single-solve fused stage, so the bar is bit-identity with per-solve
+stage_fused runs (themselves bit-compared with the CPU oracle in
+test_gpu_parity.py), for any mix of N, meshes, dt and constants, for every
+elements-per-workgroup variant, eagerly and through captured graphs, and
+when solves join and leave a running batch.  The FP32 shallow-water variant
+is held to a relative L2 error against FP64 instead.
+"""
+import numpy as np
+import pytest
+import torch
+
+import canary_amd.kernels as K
+import canary_amd.problems as P
+from canary_amd._lib import CanaryError, EULER, SWE, FP32, FP64, ERR_ARG
+from canary_amd.batch import Batch, ITEM_DTYPE, variants
+from oracle import oracle as O
+
+from gpu_util import DeviceSetup, dev, host
+from states import max_ulp, rel_l2, smooth_euler_state
+
+pytestmark = pytest.mark.gpu
+
+
+def swe_mix():
+    """Shallow-water solves of mixed N, size, boundary and gravity."""
+    out = []
+    spec = [(3, 4, True, 10.0, 1), (4, 3, False, 9.81, 2), (5, 5, True, 10.0, 3),
+            (7, 2, True, 12.0, 4), (3, 6, False, 10.0, 5), (2, 3, True, 10.0, 6)]
+    for N, Ne, per, g, seed in spec:
+        rng = np.random.default_rng(seed)
+        bumps = [(rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8), 0.05, rng.uniform(0.05, 0.2))
+                 for _ in range(2)]
+        s = P.swe2d(Ne=Ne, N=N, bumps=bumps, periodic=per, gravity=g)
+        out.append((s, P.swe_courant(s, s.Q)))
+    return out
+
+
+def euler2d_mix():
+    out = []
+    for N, Ne, seed in ((3, 4, 1), (4, 3, 2), (6, 2, 3)):
+        s = P.bubble2d(Ne=Ne, N=N)
+        s.Q = np.asfortranarray(smooth_euler_state(s, seed=seed))
+        out.append((s, P.cfl(s, s.Q) / s.N ** np.sqrt(2)))
+    v = P.vortex2d(Ne=4, N=4)  # other constants (p0 = R = 1, g = 0)
+    out.append((v, P.euler_courant(v, v.Q)))
+    return out
+
+
+def euler3d_mix():
+    out = []
+    for N, Ne, seed in ((2, 3, 1), (3, 2, 2), (4, 2, 3), (5, 2, 4)):
+        s = P.bubble3d(Ne=Ne, N=N, jitter=1e-3, seed=seed)
+        s.Q = np.asfortranarray(smooth_euler_state(s, seed=seed))
+        out.append((s, P.cfl(s, s.Q) / s.N ** np.sqrt(2)))
+    return out
+
+
+def single_run(s, dt, nsteps):
+    """nsteps LSRK54 steps of one solve through the single-solve fused stage."""
+    A, B, _ = P.rk_coefs()
+    ds = DeviceSetup(s)
+    q, q2, rhs = dev(s.Q), dev(s.Q), dev(np.zeros_like(s.Q))
+    for _ in range(nsteps):
+        for st in range(5):
+            K.stage_fused(s.dim, s.N, rhs, q, q2, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM,
+                          ds.vmapP, ds.bndy, A[(st + 1) % 5], B[st], dt, eqn=s.eqn, phys=s.phys)
+            q, q2 = q2, q
+    K.check_errors()
+    out = host(q, s.Q.shape)
+    assert np.isfinite(out).all()
+    return out
+
+
+def make_batch(eqn, dim, mix, prec=FP64, epw=None):
+    b = Batch(eqn, dim, prec)
+    ids = [b.add(s, dt) for s, dt in mix]
+    b.finalize(epw=epw)
+    return b, ids
+
+
+@pytest.mark.parametrize("eqn,dim,mixf", [(SWE, 2, swe_mix), (EULER, 2, euler2d_mix),
+                                          (EULER, 3, euler3d_mix)])
+def test_batch_equals_single_solves(cuda, eqn, dim, mixf):
+    mix = mixf()
+    print("batch eqn=%d dim=%d N=%s" % (eqn, dim, [s.N for s, _ in mix]), flush=True)
+    b, ids = make_batch(eqn, dim, mix)
+    b.run(3, graph=False)
+    b.check()
+    for (s, dt), i in zip(mix, ids):
+        want = single_run(s, dt, 3)
+        assert max_ulp(b.state(i), want) == 0, "solve N=%d" % s.N
+
+
+def test_batch_swe_matches_oracle(cuda):
+    s, dt = swe_mix()[1]
+    b, ids = make_batch(SWE, 2, [(s, dt)])
+    b.run(2, graph=False)
+    Q = np.array(s.Q, order="F", copy=True)
+    rhs = np.zeros_like(Q, order="F")
+    O.lsrk(s, Q, rhs, dt, 2)
+    assert np.isfinite(Q).all()
+    assert max_ulp(b.state(ids[0]), Q) == 0
+
+
+@pytest.mark.parametrize("eqn,dim,mixf", [(SWE, 2, swe_mix), (EULER, 3, euler3d_mix)])
+def test_every_variant_same_bits(cuda, eqn, dim, mixf):
+    mix = mixf()
+    ref, rids = make_batch(eqn, dim, mix)
+    ref.run(2, graph=False)
+    Ns = sorted({s.N for s, _ in mix})
+    nvar = max(len(variants(eqn, dim, FP64, N)) for N in Ns)
+    for k in range(1, nvar):
+        epw = {}
+        for N in Ns:
+            c = variants(eqn, dim, FP64, N)
+            epw[N] = c[min(k, len(c) - 1)][0]
+        print("variant", epw, flush=True)
+        b, ids = make_batch(eqn, dim, mix, epw=epw)
+        b.run(2, graph=False)
+        b.check()
+        for i, j in zip(ids, rids):
+            assert max_ulp(b.state(i), ref.state(j)) == 0
+
+
+def test_graph_replays_match_eager(cuda):
+    mix = swe_mix()
+    eager, eids = make_batch(SWE, 2, mix)
+    graph, gids = make_batch(SWE, 2, mix)
+    for rep in range(4):
+        graph.run(1, graph=True)       # replays alternate between the two parity graphs
+        eager.run(1, graph=False)
+        x = torch.ones(1000, device="cuda").sum()  # eager work between replays
+        torch.cuda.synchronize()
+        assert float(x) == 1000.0
+        for i, j in zip(gids, eids):
+            assert max_ulp(graph.state(i), eager.state(j)) == 0, "replay %d" % rep
+    graph.check()
+
+
+def test_autotune_keeps_results(cuda):
+    mix = swe_mix()
+    ref, rids = make_batch(SWE, 2, mix)
+    ref.run(2, graph=False)
+    b = Batch(SWE, 2)
+    ids = [b.add(s, dt) for s, dt in mix]
+    chosen = b.finalize(autotune=True).epw
+    print("autotune", chosen, b.tuning, flush=True)
+    for N, res in b.tuning.items():
+        assert chosen[N] == min(res, key=res.get)
+        assert all(t > 0 for t in res.values())
+    b.run(2)
+    for i, j in zip(ids, rids):
+        assert max_ulp(b.state(i), ref.state(j)) == 0
+
+
+def test_join_and_leave(cuda):
+    """Continuous batching: a solve joining a running batch, and one leaving
+    it, get exactly the bits they get alone."""
+    mix = swe_mix()
+    b = Batch(SWE, 2)
+    a = b.add(*mix[0])
+    c = b.add(*mix[2])
+    b.finalize()
+    b.run(2)
+    d = b.add(*mix[3])           # joins after 2 steps
+    b.finalize()
+    b.run(1)
+    Qc = b.remove(c)             # leaves after 3 steps
+    b.finalize()
+    b.run(2)
+    assert max_ulp(b.state(a), single_run(*mix[0], 5)) == 0
+    assert max_ulp(Qc, single_run(*mix[2], 3)) == 0
+    assert max_ulp(b.state(d), single_run(*mix[3], 3)) == 0
+    b.check()
+
+
+def test_fp32_swe_tracks_fp64(cuda):
+    mix = swe_mix()
+    b64, ids64 = make_batch(SWE, 2, mix)
+    b32, ids32 = make_batch(SWE, 2, mix, prec=FP32)
+    for n in (1, 10):
+        b64.run(n)
+        b32.run(n)
+        b32.check()
+        for (s, _), i, j in zip(mix, ids32, ids64):
+            got, want = b32.state(i), b64.state(j)
+            # states (U, V, h, b): momenta and the height perturbation h over
+            # the depth b (b is copied through unchanged)
+            eh = rel_l2(got[:, 2, :], want[:, 2, :])
+            eU = rel_l2(got[:, 0:2, :], want[:, 0:2, :])
+            print("N=%d steps %d: h %.2e U %.2e" % (s.N, n, eh, eU), flush=True)
+            assert eh <= 1e-4 and eU <= 1e-4
+            assert max_ulp(got[:, 3, :], want[:, 3, :].astype(np.float32).astype(np.float64)) == 0
+
+
+def test_bad_table_entry_is_reported(cuda):
+    """An entry whose N does not match its kernel is skipped and reported."""
+    s, dt = swe_mix()[0]
+    b, ids = make_batch(SWE, 2, [(s, dt)])
+    items = np.zeros(1, dtype=ITEM_DTYPE)
+    items["solve"], items["N"], items["epw"], items["e0"] = 0, s.N + 1, b.epw[s.N], 0
+    t = torch.from_numpy(items.view(np.uint8).copy()).cuda()
+    before = b.state(ids[0])
+    b._launch(s.N, b.epw[s.N], t.data_ptr(), 1, b.parity, 0.0, 1.0, b._st())
+    with pytest.raises(CanaryError) as ei:
+        b.check()
+    assert ei.value.code == ERR_ARG
+    b.check()  # cleared
+    assert max_ulp(b.state(ids[0]), before) == 0

@@ -1,7 +1,9 @@
-"""Precision form (FP32 perturbation about an FP64 hydrostatic background)
+"""Precision form (FP32 perturbation about an FP64 background at rest)
 against the FP64 GPU path, itself bit-compared with the CPU oracle in
-test_gpu_parity.py.  Bar: relative L2 error of rho, rho*u and rho*theta
-<= 1e-5 after 1, 10 and 100 LSRK54 steps."""
+test_gpu_parity.py.  Bar: relative L2 error of rho, the momenta and
+rho*theta <= 1e-5 after 1, 10 and 100 LSRK54 steps, for the 3D bubble
+(hydrostatic background) and the 2D isentropic vortex (uniform background).
+The batched precision form is bit-identical to the single-solve one."""
 import numpy as np
 import pytest
 import torch
@@ -10,8 +12,10 @@ import canary_amd.kernels as K
 import canary_amd.problems as P
 from canary_amd._lib import CanaryError
 from canary_amd.perturb import Perturb32
+from canary_amd.batch import Batch32
 
 from gpu_util import DeviceSetup, dev, host
+from states import max_ulp
 
 pytestmark = pytest.mark.gpu
 
@@ -21,8 +25,8 @@ def fp64_run(s, ds, Q, dt, nsteps):
     q, q2, rhs = dev(Q), torch.empty(Q.size, dtype=torch.float64, device="cuda"), dev(np.zeros_like(Q))
     for _ in range(nsteps):
         for st in range(5):
-            K.stage_fused(3, s.N, rhs, q, q2, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM, ds.vmapP,
-                          ds.bndy, A[(st + 1) % 5], B[st], dt, phys=s.phys)
+            K.stage_fused(s.dim, s.N, rhs, q, q2, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM,
+                          ds.vmapP, ds.bndy, A[(st + 1) % 5], B[st], dt, phys=s.phys)
             q, q2 = q2, q
     K.check_errors()
     return host(q, Q.shape)
@@ -62,3 +66,78 @@ def test_perturb32_rejects_noncanonical_vmapM(cuda):
     s.vmapM[0, 0, 0] += 1
     with pytest.raises(CanaryError):
         Perturb32(s, Q0, s.Q)
+
+
+def state_errors(got, want, dim):
+    names = ["rhou", "rhov", "rhow"][:dim] + ["rho", "rhotheta"]
+    return {nm: rel(got[:, k, :], want[:, k, :]) for k, nm in enumerate(names)}
+
+
+@pytest.mark.parametrize("N", [4, 7])
+def test_perturb32_vortex2d_tracks_fp64(cuda, N):
+    s = P.vortex2d(Ne=8, N=N)
+    dt = P.euler_courant(s, s.Q)
+    ds = DeviceSetup(s)
+    pf = Perturb32(s, s.Q0, s.Q)
+    done = 0
+    for n in (1, 10, 100):
+        pf.run(dt, n - done)
+        done = n
+        pf.check()
+        got = host(pf.full_state(), s.Q.shape)
+        want = fp64_run(s, ds, s.Q, dt, n)
+        e = state_errors(got, want, 2)
+        print("vortex N=%d steps=%d" % (N, n), {k: "%.2e" % v for k, v in e.items()}, flush=True)
+        assert all(v <= 1e-5 for v in e.values()), e
+
+
+def p32_mix():
+    out = []
+    for Ne, N, seed in ((3, 3, 1), (2, 4, 2), (4, 2, 3), (2, 5, 4)):
+        s = P.bubble3d(Ne=Ne, N=N, dtheta=2.0, jitter=1e-3, seed=seed)
+        out.append((s, P.cfl(s, s.Q) / s.N ** np.sqrt(2)))
+    return out
+
+
+def vortex_mix():
+    out = []
+    for Ne, N in ((4, 3), (3, 5), (2, 7), (5, 4)):
+        s = P.vortex2d(Ne=Ne, N=N)
+        out.append((s, P.euler_courant(s, s.Q)))
+    return out
+
+
+@pytest.mark.parametrize("dim,mixf", [(3, p32_mix), (2, vortex_mix)])
+def test_batch32_equals_single(cuda, dim, mixf):
+    mix = mixf()
+    b = Batch32(dim)
+    ids = [b.add(s, dt) for s, dt in mix]
+    b.finalize(autotune=True)
+    print("autotune", b.epw, flush=True)
+    b.run(3)
+    b.check()
+    for (s, dt), i in zip(mix, ids):
+        pf = Perturb32(s, s.Q0, s.Q)
+        pf.run(dt, 3)
+        pf.check()
+        want = host(pf.full_state(), s.Q.shape)
+        assert np.isfinite(want).all()
+        assert max_ulp(b.state(i), want) == 0, "solve N=%d" % s.N
+
+
+def test_batch32_join_and_leave(cuda):
+    mix = vortex_mix()
+    b = Batch32(2)
+    a = b.add(*mix[0])
+    b.finalize()
+    b.run(2)
+    c = b.add(*mix[2])
+    b.finalize()
+    b.run(3)
+    Qa = b.remove(a)
+    b.finalize()
+    b.run(1)
+    for (s, dt), n, got in ((mix[0], 5, Qa), (mix[2], 4, b.state(c))):
+        pf = Perturb32(s, s.Q0, s.Q)
+        pf.run(dt, n)
+        assert max_ulp(got, host(pf.full_state(), s.Q.shape)) == 0
