@@ -61,6 +61,12 @@ def main():
                     help="elements per direction of the FP32-vs-FP64 precision-form run (0: skip)")
     ap.add_argument("--precision-steps", type=str, default="1,100,1000",
                     help="LSRK steps after which the FP32 form's error is reported")
+    ap.add_argument("--batched-solves", type=int, default=64,
+                    help="shallow-water solves of the batched-form run (0: skip)")
+    ap.add_argument("--batched-steps", type=int, default=20)
+    ap.add_argument("--served-requests", type=int, default=64,
+                    help="requests of the served-form run (0: skip)")
+    ap.add_argument("--served-rate", type=float, default=200.0)
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also run one device-to-device copy of this many GiB (untimed), the "
                          "known-size reference for HBM byte counters")
@@ -169,6 +175,12 @@ def main():
         del q, q2, rhs, vgeo, sgeo, vM, vP, bnd, Q0, holder, qf
         torch.cuda.empty_cache()
         extra["precision_form"] = precision_form(args.precision_ne, s.N, args.precision_steps)
+    if args.batched_solves > 0:
+        torch.cuda.empty_cache()
+        extra["batched_form"] = batched_form(args.batched_solves, args.batched_steps)
+    if args.served_requests > 0:
+        torch.cuda.empty_cache()
+        extra["served_form"] = served_form(args.served_requests, args.served_rate)
 
     traffic = measured_traffic("euler3d_N%d_%d" % (s.N, args.ne))
     line = {
@@ -282,6 +294,160 @@ def precision_form(ne, N, steps_csv, timed_steps=10):
             "fp32_bytes_per_element_stage": b32,
             "fp32_hbm_frac": round(b32 * s.nelem / t32 / 1e9 / HBM_PEAK_GBS, 4),
             "rel_l2_vs_fp64": errors, "dt": dt}
+
+
+def _gpu_timer():
+    import torch
+
+    def timed(fn):
+        stream = torch.cuda.current_stream()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3
+    return timed
+
+
+def batched_form(nsolves, steps, seed=42):
+    """Ensemble of independent periodic shallow-water solves (Ne^2 quads, Ne
+    uniform in 8..64, N in {3,4,5,7}, seeded bumps) advanced together:
+    FP64 and FP32 batches (autotuned elements per workgroup, one graph per
+    step) against the same solves run one after another with the
+    single-solve fused stage."""
+    import numpy as np
+    import torch
+    import canary_amd.kernels as K
+    import canary_amd.problems as P
+    from canary_amd._lib import SWE, FP32, FP64
+    from canary_amd.batch import Batch
+
+    t0 = time.time()
+    ens = P.swe_ensemble(n=nsolves, seed=seed)
+    print("setup batched form: %d solves, %.1f s" % (nsolves, time.time() - t0), file=sys.stderr,
+          flush=True)
+    timed = _gpu_timer()
+    dof = sum(s.ndof() for s, _ in ens)
+    out = {"workload": "%d periodic shallow-water solves, Ne^2 quads (Ne uniform 8..64), "
+                       "N in {3,4,5,7}, depth 10, 2 seeded Gaussian bumps (height 0.01-0.1, "
+                       "width 0.05-0.2)" % nsolves,
+           "solves": nsolves, "dof": dof, "elements": sum(s.nelem for s, _ in ens),
+           "orders": sorted({s.N for s, _ in ens}), "steps": steps}
+    states = {}
+    for name, prec in (("fp64", FP64), ("fp32", FP32)):
+        b = Batch(SWE, 2, prec)
+        ids = [b.add(s, dt) for s, dt in ens]
+        b.finalize(autotune=True)
+        b.run(2)
+        b.check()
+        t = timed(lambda: b.run(steps))
+        b.check()
+        out[name] = {"gdof_stage_s": round(dof * 5 * steps / t / 1e9, 3),
+                     "ms_per_step": round(t / steps * 1e3, 4),
+                     "hbm_frac": round(b.bytes_per_stage() * 5 * steps / t / 1e9 / HBM_PEAK_GBS, 4),
+                     "epw": {str(k): v for k, v in sorted(b.epw.items())}}
+        states[name] = [b.state(i) for i in ids]
+        del b
+        torch.cuda.empty_cache()
+    errs = [np.linalg.norm(a[:, :3] - c[:, :3]) / np.linalg.norm(c[:, :3])
+            for a, c in zip(states["fp32"], states["fp64"])]
+    out["fp32_rel_l2_vs_fp64_max"] = float("%.3e" % max(errs))
+    # the same solves one after another, single-solve fused stage, FP64
+    A, B, _ = P.rk_coefs()
+    dev = torch.device("cuda:0")
+
+    def d(a):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to(dev)
+
+    seq = []
+    for s, dt in ens:
+        arrs = [d(x) for x in (s.vgeo, s.sgeo, s.D, s.vmapM, s.vmapP, s.elemtobndy)]
+        seq.append((s, dt, arrs, d(s.Q), torch.empty(s.Q.size, dtype=torch.float64, device=dev),
+                    torch.zeros(s.Q.size, dtype=torch.float64, device=dev)))
+
+    def run_seq(nsteps):
+        for s, dt, (vg, sg, D, vM, vP, bd), q, q2, rhs in seq:
+            for _ in range(nsteps):
+                for k in range(5):
+                    K.stage_fused(2, s.N, rhs, q, q2, vg, sg, D, s.nelem, vM, vP, bd,
+                                  A[(k + 1) % 5], B[k], dt, eqn=SWE, phys=s.phys)
+                    q, q2 = q2, q
+    run_seq(1)
+    t = timed(lambda: run_seq(steps))
+    K.check_errors()
+    out["sequential_fp64"] = {"gdof_stage_s": round(dof * 5 * steps / t / 1e9, 3),
+                              "ms_per_step": round(t / steps * 1e3, 4)}
+    out["speedup_batched_fp64_vs_sequential"] = round(
+        out["fp64"]["gdof_stage_s"] / out["sequential_fp64"]["gdof_stage_s"], 3)
+    return out
+
+
+def served_form(nreq, rate, seed=2024):
+    """Poisson request stream (half 2D vortices, half 3D bubbles) through the
+    continuous-batching server (FP32 perturbation form; host preparation of
+    each request inside its latency), against the same requests solved one
+    after another with the plain FP64 fused stage (preparation excluded)."""
+    import numpy as np
+    import torch
+    import canary_amd.kernels as K
+    import canary_amd.problems as P
+    from canary_amd.serve import Server, poisson_stream, summary
+
+    reqs = poisson_stream(n=nreq, rate=rate, seed=seed)
+    # an untimed tiny stream first (first launches, allocator); the measured
+    # server starts with an empty elements-per-workgroup table and autotunes
+    # each new order inside the stream
+    Server("fp32").serve(poisson_stream(n=8, rate=1000.0, seed=1, vortex_ne=(2, 3),
+                                        bubble_ne=(2, 2), steps=(2, 3)))
+    srv = Server("fp32", keep_setups=True)
+    t0 = time.time()
+    out_reqs = srv.serve(reqs)
+    wall = time.time() - t0
+    st = summary(out_reqs, srv.makespan)
+    print("served %d requests in %.1f s" % (nreq, wall), file=sys.stderr, flush=True)
+    # sequential plain FP64 solves of the same requests
+    timed = _gpu_timer()
+    A, B, _ = P.rk_coefs()
+    dev = torch.device("cuda:0")
+
+    def d(a):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to(dev)
+
+    t_seq, errs = 0.0, []
+    for r in out_reqs:
+        s, dt = r.setup
+        vg, sg, D, vM, vP, bd = (d(x) for x in (s.vgeo, s.sgeo, s.D, s.vmapM, s.vmapP,
+                                                s.elemtobndy))
+        q, q2, rhs = d(s.Q), torch.empty(s.Q.size, dtype=torch.float64, device=dev), \
+            torch.zeros(s.Q.size, dtype=torch.float64, device=dev)
+        hold = {}
+
+        def go():
+            a, b2 = q, q2
+            for _ in range(r.nsteps):
+                for k in range(5):
+                    K.stage_fused(s.dim, s.N, rhs, a, b2, vg, sg, D, s.nelem, vM, vP, bd,
+                                  A[(k + 1) % 5], B[k], dt, phys=s.phys)
+                    a, b2 = b2, a
+            hold["q"] = a
+        t_seq += timed(go)
+        ref = hold["q"].cpu().numpy().reshape(s.Q.shape, order="F")
+        errs.append(np.linalg.norm(r.result - ref) / np.linalg.norm(ref))
+        del vg, sg, D, vM, vP, bd, q, q2, rhs, hold
+        r.setup = None
+    K.check_errors()
+    nk = {"vortex2d": sum(r.kind == "vortex2d" for r in out_reqs)}
+    return {"workload": "%d requests, Poisson arrivals (mean %g/s, seed %d): %d 2D isentropic "
+                        "vortices (16^2-64^2 quads, N 3..7), %d 3D rising bubbles (8^3-24^3 "
+                        "hexes, N 3..5), 50-500 LSRK54 steps each; FP32 perturbation form"
+                        % (nreq, rate, seed, nk["vortex2d"], nreq - nk["vortex2d"]),
+            **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()},
+            "sequential_fp64_gpu_s": round(t_seq, 4),
+            "sequential_fp64_gdof_stage_s": round(st["dof_stages"] / t_seq / 1e9, 3),
+            "fp32_rel_l2_vs_fp64": {"max": float("%.3e" % max(errs)),
+                                    "median": float("%.3e" % float(np.median(errs)))}}
 
 
 def measured_traffic(tag):

@@ -16,6 +16,7 @@ ERR_NOCONVERGENCE, ERR_LOBATTO_N1, ERR_ORDERING, ERR_ORIENTATION = 6, 7, 8, 9
 ERR_NOMEM, ERR_CAPACITY = 10, 11
 EULER, SWE = 0, 1
 FP64, FP32 = 0, 1
+REDUCE_WORK = 1024  # CANARY_REDUCE_WORK
 ENDPT_NEITHER, ENDPT_LEFT, ENDPT_RIGHT, ENDPT_BOTH = 0, 1, 2, 3
 
 ERRNAMES = {
@@ -94,6 +95,11 @@ _SIGS = {
     "canary_batch_variant": (INT, [INT, INT, INT, INT, INT, C.POINTER(INT), C.POINTER(INT)]),
     "canary_batch_stage": (INT, [INT, INT, INT, INT, INT, P, P, I64, INT, D, D, P]),
     "canary_device_count": (INT, []),
+    "canary_computegeometry_device": (INT, [INT, INT, I64, P, P, P, P, P]),
+    "canary_fillsendQ": (INT, [INT, INT, INT, P, P, P, I64, I64, INT, P]),
+    "canary_transferrecvQ": (INT, [INT, INT, INT, P, P, I64, I64, INT, P]),
+    "canary_l2energysquared_device": (INT, [INT, INT, INT, INT, P, P, I64, P, P, P]),
+    "canary_cfl_device": (INT, [INT, INT, INT, P, P, I64, C.POINTER(Phys), P, P, P]),
     "canary_gauss_rule": (INT, [INT, INT, INT, P, P]),
     "canary_lglpoints": (INT, [INT, P, P]),
     "canary_lgpoints": (INT, [INT, P, P]),

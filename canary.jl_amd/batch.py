@@ -234,14 +234,21 @@ class _BatchBase:
         g = self._graphs.get(key)
         if g is None:
             p0 = self.parity
+            cur = torch.cuda.current_stream(self.device)
             side = torch.cuda.Stream(device=self.device)
-            side.wait_stream(torch.cuda.current_stream(self.device))
+            side.wait_stream(cur)
             g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g, stream=side):
-                A, B, _ = rk_coefs()
-                for s in range(5):
-                    self.stage(A[(s + 1) % 5], B[s], concurrent=True)
-            torch.cuda.current_stream(self.device).wait_stream(side)
+            A, B, _ = rk_coefs()
+            # capture_begin/end directly: the torch.cuda.graph context would also
+            # empty the allocator cache on every capture
+            with torch.cuda.stream(side):
+                g.capture_begin()
+                try:
+                    for s in range(5):
+                        self.stage(A[(s + 1) % 5], B[s], concurrent=True)
+                finally:
+                    g.capture_end()
+            cur.wait_stream(side)
             self.parity = p0  # capture recorded the launches, it did not run them
             self._graphs[key] = g
         return g

@@ -1,0 +1,267 @@
+// Kernels beside the stage: ghost-element exchange buffers and the two
+// reductions the drivers print.
+//
+//   fillsendQ     sendQ[:, :, e] = Q[:, :, sendelems[e]]   (euler3d_set2c.jl:645-659)
+//   transferrecvQ Q[:, :, nrealelem + e] = recvQ[:, :, e]  (euler3d_set2c.jl:663-676)
+//   l2energysquared  sum_e sum_s sum_i MJ q^2               (euler3d_set2c.jl:738-750)
+//   cfl              min over nodes of the local step       (euler3d_set2c.jl:67-97)
+//
+// The copies move whole elements (Np * nstate contiguous words) with one
+// lane per word.  The reductions are deterministic: a fixed grid of
+// RED_BLOCKS workgroups each sums a fixed contiguous range in LDS tree
+// order, then one workgroup combines the RED_BLOCKS partials; no atomics.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "../../../include/canary_dg.h"
+
+namespace canary {
+namespace aux {
+
+static __device__ unsigned long long g_aux_status[4];
+
+__device__ __forceinline__ void aux_error(unsigned long long code, long long a, long long b,
+                                          long long value) {
+  if (atomicCAS(&g_aux_status[0], 0ull, code) == 0ull) {
+    g_aux_status[1] = (unsigned long long)a;
+    g_aux_status[2] = (unsigned long long)b;
+    g_aux_status[3] = (unsigned long long)value;
+  }
+}
+
+constexpr int RED_BLOCKS = CANARY_REDUCE_WORK;
+constexpr int RED_THREADS = 256;
+
+template <class W>
+__global__ __launch_bounds__(256) void fill_send_kernel(W* __restrict__ sendQ,
+                                                        const W* __restrict__ Q,
+                                                        const int64_t* __restrict__ sendelems,
+                                                        int64_t nsend, int64_t words,
+                                                        int64_t nelem_total) {
+  const int64_t total = nsend * words;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = g / words, w = g - e * words;
+    const int64_t re = sendelems[e];
+    if (re < 1 || re > nelem_total) {
+      if (w == 0) aux_error(CANARY_ERR_ARG, e + 1, -1, re);
+      continue;
+    }
+    sendQ[g] = Q[(size_t)(re - 1) * words + w];
+  }
+}
+
+template <class W>
+__global__ __launch_bounds__(256) void recv_kernel(W* __restrict__ Q, const W* __restrict__ recvQ,
+                                                   int64_t total, int64_t offset) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+       g += (int64_t)gridDim.x * blockDim.x)
+    Q[offset + g] = recvQ[g];
+}
+
+// LDS tree sum / min of the block's values; result in lane 0
+template <bool MIN>
+__device__ __forceinline__ double block_reduce(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] = MIN ? fmin(sh[t], sh[t + s]) : sh[t] + sh[t + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+__global__ __launch_bounds__(RED_THREADS) void energy_kernel(
+    const double* __restrict__ Q, const double* __restrict__ vgeo, int64_t Np, int64_t nstate,
+    int64_t nvgeo, int64_t iMJ, int64_t nelem, double* __restrict__ work) {
+  __shared__ double sh[RED_THREADS];
+  const int64_t total = Np * nstate * nelem;
+  const int64_t chunk = (total + RED_BLOCKS - 1) / RED_BLOCKS;
+  const int64_t lo = blockIdx.x * chunk;
+  const int64_t hi = lo + chunk < total ? lo + chunk : total;
+  double acc = 0.0;
+  for (int64_t g = lo + threadIdx.x; g < hi; g += RED_THREADS) {
+    const int64_t e = g / (Np * nstate), r = g - e * Np * nstate;
+    const int64_t i = r % Np;
+    const double q = Q[g];
+    acc += vgeo[i + Np * (iMJ + nvgeo * e)] * (q * q);
+  }
+  const double s = block_reduce<false>(acc, sh);
+  if (threadIdx.x == 0) work[blockIdx.x] = s;
+}
+
+// local time step of node n of element e (host/problems.cpp canary_cfl)
+__device__ __forceinline__ double local_dt(int eqn, int dim, int64_t Np, const double* Q,
+                                           const double* vgeo, int64_t e, int64_t n,
+                                           const canary_phys& ph) {
+  if (eqn == CANARY_EULER && dim == 3) {
+    const double* q = Q + (size_t)Np * 5 * e;
+    const double* vg = vgeo + (size_t)Np * 14 * e;
+    const double rho = q[n + Np * 3], U = q[n], V = q[n + Np], W = q[n + 2 * Np];
+    const double E = q[n + Np * 4];
+    const double P = ph.p0 * pow(ph.R_gas * E / ph.p0, ph.c_p / ph.c_v);
+    const double c = rho * sqrt(ph.gamma * P / rho);
+    const double a = fabs(U * vg[n] + V * vg[n + 3 * Np] + W * vg[n + 6 * Np]) + c;
+    const double b = fabs(U * vg[n + Np] + V * vg[n + 4 * Np] + W * vg[n + 7 * Np]) + c;
+    const double d = fabs(U * vg[n + 2 * Np] + V * vg[n + 5 * Np] + W * vg[n + 8 * Np]) + c;
+    double m = a > b ? a : b;
+    m = m > d ? m : d;
+    return 2 * rho / m;
+  }
+  if (eqn == CANARY_EULER && dim == 2) {
+    const double* q = Q + (size_t)Np * 4 * e;
+    const double* vg = vgeo + (size_t)Np * 8 * e;
+    const double rho = q[n + Np * 2], U = q[n], V = q[n + Np], E = q[n + Np * 3];
+    const double P = ph.p0 * pow(ph.R_gas * E / ph.p0, ph.c_p / ph.c_v);
+    const double c = rho * sqrt(ph.gamma * P / rho);
+    const double a = fabs(U * vg[n] + V * vg[n + 2 * Np]) + c;
+    const double b = fabs(U * vg[n + Np] + V * vg[n + 3 * Np]) + c;
+    return 2 * rho / (a > b ? a : b);
+  }
+  const double* q = Q + (size_t)Np * 4 * e;
+  const double* vg = vgeo + (size_t)Np * 8 * e;
+  const double h = q[n + 2 * Np], b = q[n + 3 * Np], U = q[n], V = q[n + Np];
+  const double H = h + b;
+  const double s = H * sqrt(ph.swe_gravity * H) * ph.delta_nl;
+  const double a = fabs(U * vg[n] + V * vg[n + 2 * Np]) + s;
+  const double c = fabs(U * vg[n + Np] + V * vg[n + 3 * Np]) + s;
+  return 2 * H / (a > c ? a : c);
+}
+
+__global__ __launch_bounds__(RED_THREADS) void cfl_kernel(int eqn, int dim, int64_t Np,
+                                                          const double* __restrict__ vgeo,
+                                                          const double* __restrict__ Q,
+                                                          int64_t nelem, canary_phys ph,
+                                                          double* __restrict__ work) {
+  __shared__ double sh[RED_THREADS];
+  const int64_t total = Np * nelem;
+  const int64_t chunk = (total + RED_BLOCKS - 1) / RED_BLOCKS;
+  const int64_t lo = blockIdx.x * chunk;
+  const int64_t hi = lo + chunk < total ? lo + chunk : total;
+  double acc = 1.7976931348623157e308;
+  for (int64_t g = lo + threadIdx.x; g < hi; g += RED_THREADS) {
+    const int64_t e = g / Np, n = g - e * Np;
+    const double v = local_dt(eqn, dim, Np, Q, vgeo, e, n, ph);
+    acc = v < acc ? v : acc;
+  }
+  const double s = block_reduce<true>(acc, sh);
+  if (threadIdx.x == 0) work[blockIdx.x] = s;
+}
+
+template <bool MIN>
+__global__ __launch_bounds__(RED_THREADS) void finish_kernel(const double* __restrict__ work,
+                                                             double* __restrict__ out) {
+  __shared__ double sh[RED_THREADS];
+  double acc = MIN ? 1.7976931348623157e308 : 0.0;
+  for (int b = threadIdx.x; b < RED_BLOCKS; b += RED_THREADS)
+    acc = MIN ? fmin(acc, work[b]) : acc + work[b];
+  const double s = block_reduce<MIN>(acc, sh);
+  if (threadIdx.x == 0) out[0] = s;
+}
+
+int status() { return hipGetLastError() == hipSuccess ? 0 : CANARY_ERR_LAUNCH; }
+
+int64_t grid_for(int64_t work) {
+  int64_t b = (work + 255) / 256;
+  return b < 1 ? 1 : (b > 65536 ? 65536 : b);
+}
+
+}  // namespace aux
+}  // namespace canary
+
+using namespace canary::aux;
+
+extern "C" int canary_fillsendQ(int dim, int N, int nstate, void* sendQ, const void* Q,
+                                const int64_t* sendelems, int64_t nsend, int64_t nelem_total,
+                                int word_bytes, void* stream) {
+  if ((dim != 2 && dim != 3) || N < 1 || nstate < 1 || nsend < 0 || nelem_total < 0)
+    return CANARY_ERR_ARG;
+  if (nsend == 0) return 0;
+  if (!sendQ || !Q || !sendelems) return CANARY_ERR_ARG;
+  int64_t Np = 1;
+  for (int d = 0; d < dim; ++d) Np *= N + 1;
+  const int64_t words = Np * nstate;
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned g = (unsigned)grid_for(nsend * words);
+  if (word_bytes == 8)
+    fill_send_kernel<double><<<g, 256, 0, st>>>((double*)sendQ, (const double*)Q, sendelems,
+                                                nsend, words, nelem_total);
+  else if (word_bytes == 4)
+    fill_send_kernel<float><<<g, 256, 0, st>>>((float*)sendQ, (const float*)Q, sendelems, nsend,
+                                               words, nelem_total);
+  else
+    return CANARY_ERR_ARG;
+  return status();
+}
+
+extern "C" int canary_transferrecvQ(int dim, int N, int nstate, void* Q, const void* recvQ,
+                                    int64_t nrecv, int64_t nrealelem, int word_bytes,
+                                    void* stream) {
+  if ((dim != 2 && dim != 3) || N < 1 || nstate < 1 || nrecv < 0 || nrealelem < 0)
+    return CANARY_ERR_ARG;
+  if (nrecv == 0) return 0;
+  if (!Q || !recvQ) return CANARY_ERR_ARG;
+  int64_t Np = 1;
+  for (int d = 0; d < dim; ++d) Np *= N + 1;
+  const int64_t words = Np * nstate;
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned g = (unsigned)grid_for(nrecv * words);
+  if (word_bytes == 8)
+    recv_kernel<double><<<g, 256, 0, st>>>((double*)Q, (const double*)recvQ, nrecv * words,
+                                           nrealelem * words);
+  else if (word_bytes == 4)
+    recv_kernel<float><<<g, 256, 0, st>>>((float*)Q, (const float*)recvQ, nrecv * words,
+                                          nrealelem * words);
+  else
+    return CANARY_ERR_ARG;
+  return status();
+}
+
+extern "C" int canary_l2energysquared_device(int Np, int nstate, int nvgeo, int iMJ,
+                                             const double* Q, const double* vgeo, int64_t nelem,
+                                             double* work, double* out, void* stream) {
+  if (Np < 1 || nstate < 1 || nvgeo < 1 || iMJ < 0 || iMJ >= nvgeo || nelem < 0)
+    return CANARY_ERR_ARG;
+  if (!work || !out || (nelem > 0 && (!Q || !vgeo))) return CANARY_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  energy_kernel<<<RED_BLOCKS, RED_THREADS, 0, st>>>(Q, vgeo, Np, nstate, nvgeo, iMJ, nelem, work);
+  finish_kernel<false><<<1, RED_THREADS, 0, st>>>(work, out);
+  return status();
+}
+
+extern "C" int canary_cfl_device(int eqn, int dim, int N, const double* vgeo, const double* Q,
+                                 int64_t nelem, const canary_phys* phys, double* work,
+                                 double* out, void* stream) {
+  const bool ok = (eqn == CANARY_EULER && (dim == 2 || dim == 3)) || (eqn == CANARY_SWE && dim == 2);
+  if (!ok || N < 1 || nelem < 0 || !work || !out || (nelem > 0 && (!Q || !vgeo)))
+    return CANARY_ERR_ARG;
+  canary_phys ph;
+  if (phys)
+    ph = *phys;
+  else
+    canary_phys_default(&ph);
+  int64_t Np = 1;
+  for (int d = 0; d < dim; ++d) Np *= N + 1;
+  hipStream_t st = (hipStream_t)stream;
+  cfl_kernel<<<RED_BLOCKS, RED_THREADS, 0, st>>>(eqn, dim, Np, vgeo, Q, nelem, ph, work);
+  finish_kernel<true><<<1, RED_THREADS, 0, st>>>(work, out);
+  return status();
+}
+
+extern "C" int canary_k_aux_status(int64_t* info, int clear) {
+  unsigned long long h[4] = {0, 0, 0, 0};
+  if (hipMemcpyFromSymbol(h, HIP_SYMBOL(canary::aux::g_aux_status), sizeof(h), 0,
+                          hipMemcpyDeviceToHost) != hipSuccess)
+    return CANARY_ERR_LAUNCH;
+  if (clear && h[0] != 0) {
+    unsigned long long z[4] = {0, 0, 0, 0};
+    if (hipMemcpyToSymbol(HIP_SYMBOL(canary::aux::g_aux_status), z, sizeof(z), 0,
+                          hipMemcpyHostToDevice) != hipSuccess)
+      return CANARY_ERR_LAUNCH;
+  }
+  if (info)
+    for (int i = 0; i < 4; ++i) info[i] = (int64_t)h[i];
+  return (int)h[0];
+}

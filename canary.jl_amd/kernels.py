@@ -139,3 +139,81 @@ def check_errors(stream=None):
                                % (info[1], info[2], info[3]))
         err.info = tuple(info)
         raise err
+
+
+# ------------------------------------------------ geometry, halos, reductions
+def computegeometry(dim, N, elemtocoord, vmapM, stream=None):
+    """computegeometry on the GPU: elemtocoord (CUDA float64, the reference's
+    (d, 2^d, nelem) column-major array flattened), vmapM (CUDA int64) ->
+    (vgeo, sgeo) flat float64 tensors in the reference layouts."""
+    nelem = elemtocoord.numel() // (dim * 2 ** dim)
+    Np, Nfp, nf = (N + 1) ** dim, (N + 1) ** (dim - 1), 2 * dim
+    _need(elemtocoord, "elemtocoord", dim * 2 ** dim * nelem)
+    _need(vmapM, "vmapM", Nfp * nf * nelem, torch.int64)
+    vgeo = torch.empty(Np * _NV[dim] * nelem, dtype=torch.float64, device=elemtocoord.device)
+    sgeo = torch.empty(_NSG[dim] * Nfp * nf * nelem, dtype=torch.float64,
+                       device=elemtocoord.device)
+    rc = lib().canary_computegeometry_device(dim, N, nelem, _ptr(elemtocoord), _ptr(vmapM),
+                                             _ptr(vgeo), _ptr(sgeo), _stream(stream))
+    check(rc, "computegeometry_device")
+    return vgeo, sgeo
+
+
+def _word(t):
+    if t.dtype == torch.float64:
+        return 8
+    if t.dtype == torch.float32:
+        return 4
+    raise TypeError("states must be float64 or float32")
+
+
+def fillsendQ(dim, N, nstate, sendQ, Q, sendelems, nelem_total, stream=None):
+    """fillsendQ!: sendQ[:, :, e] = Q[:, :, sendelems[e]] (1-based sendelems)."""
+    nsend = sendelems.numel()
+    w = (N + 1) ** dim * nstate
+    _need(sendelems, "sendelems", nsend, torch.int64)
+    _need(sendQ, "sendQ", w * nsend, Q.dtype)
+    _need(Q, "Q", w * nelem_total, Q.dtype)
+    rc = lib().canary_fillsendQ(dim, N, nstate, _ptr(sendQ), _ptr(Q), _ptr(sendelems), nsend,
+                                int(nelem_total), _word(Q), _stream(stream))
+    check(rc, "fillsendQ")
+
+
+def transferrecvQ(dim, N, nstate, Q, recvQ, nrealelem, stream=None):
+    """transferrecvQ!: Q[:, :, nrealelem+1:end] = recvQ."""
+    w = (N + 1) ** dim * nstate
+    nrecv = recvQ.numel() // w
+    _need(recvQ, "recvQ", w * nrecv, Q.dtype)
+    _need(Q, "Q", w * (nrealelem + nrecv), Q.dtype)
+    rc = lib().canary_transferrecvQ(dim, N, nstate, _ptr(Q), _ptr(recvQ), nrecv, int(nrealelem),
+                                    _word(Q), _stream(stream))
+    check(rc, "transferrecvQ")
+
+
+def l2energysquared(dim, N, Q, vgeo, nelem, nstate=None, stream=None):
+    """L2energysquared on the GPU (deterministic tree sum); returns a
+    0-dimensional CUDA tensor."""
+    Np = (N + 1) ** dim
+    ns = nstate if nstate is not None else dim + 2
+    _need(Q, "Q", Np * ns * nelem)
+    _need(vgeo, "vgeo", Np * _NV[dim] * nelem)
+    work = torch.empty(_lib.REDUCE_WORK, dtype=torch.float64, device=Q.device)
+    out = torch.empty((), dtype=torch.float64, device=Q.device)
+    imj = 9 if dim == 3 else 4
+    rc = lib().canary_l2energysquared_device(Np, ns, _NV[dim], imj, _ptr(Q), _ptr(vgeo), nelem,
+                                             _ptr(work), _ptr(out), _stream(stream))
+    check(rc, "l2energysquared_device")
+    return out
+
+
+def cfl(dim, N, vgeo, Q, nelem, eqn=EULER, phys=None, stream=None):
+    """The drivers' cfl() on the GPU (min over nodes); 0-dimensional tensor."""
+    Np = (N + 1) ** dim
+    _need(Q, "Q", Np * _NS[(eqn, dim)] * nelem)
+    _need(vgeo, "vgeo", Np * _NV[dim] * nelem)
+    work = torch.empty(_lib.REDUCE_WORK, dtype=torch.float64, device=Q.device)
+    out = torch.empty((), dtype=torch.float64, device=Q.device)
+    rc = lib().canary_cfl_device(eqn, dim, N, _ptr(vgeo), _ptr(Q), nelem, _phys(phys), _ptr(work),
+                                 _ptr(out), _stream(stream))
+    check(rc, "cfl_device")
+    return out

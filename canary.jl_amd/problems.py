@@ -243,6 +243,24 @@ def swe2d(Ne=10, N=4, depth=10.0, bumps=((0.5, 0.5, 0.05, 0.1),), periodic=True,
     return s
 
 
+def swe_ensemble(n=64, seed=42, ne=(8, 64), orders=(3, 4, 5, 7), depth=10.0,
+                 height=(0.01, 0.1), width=(0.05, 0.2), nbumps=2, gravity=10.0):
+    """n independent periodic shallow-water solves on [0,1]^2: Ne^2 quads with
+    Ne uniform in ne, N drawn from orders, still depth `depth` with nbumps
+    Gaussian bumps of seeded height and width (domain widths).  Returns
+    [(setup, dt)] with the shallow-water example's Courant step."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        Ne = int(rng.integers(ne[0], ne[1] + 1))
+        N = int(rng.choice(orders))
+        bumps = [(rng.uniform(0, 1), rng.uniform(0, 1), rng.uniform(*height), rng.uniform(*width))
+                 for _ in range(nbumps)]
+        s = swe2d(Ne=Ne, N=N, depth=depth, bumps=bumps, periodic=True, gravity=gravity)
+        out.append((s, swe_courant(s, s.Q)))
+    return out
+
+
 def rk_coefs():
     """LSRK54 (Carpenter & Kennedy 1994) as DFloat(num)/DFloat(den):
     euler3d_set2c.jl:772-788."""

@@ -1,0 +1,237 @@
+"""Served form: a request loop with continuous batching (no reference
+counterpart; the reference driver runs one solve to completion per process,
+euler3d_set2c.jl:1049-1160).
+
+A request names an equation set / problem, a mesh size, N and an end time
+(a number of LSRK54 steps of the problem's own dt).  The server
+
+  * prepares each request on host worker threads as soon as it arrives
+    (mesh, maps, geometry, initial and background states),
+  * admits prepared requests in arrival order while the HBM footprint of the
+    resident solves stays within a budget and a slot is free,
+  * keeps every admitted state resident on the GPU in one batch per
+    dimension (Batch32: FP32 perturbation form; or Batch: plain FP64), so
+    the requests on the GPU advance together, one captured graph per step
+    and batch, the 2D and 3D batches on their own streams,
+  * retires a request when it reaches its end time (its state is copied to
+    the host) and admits waiting ones in the freed room.
+
+Latency of a request = time its result is on the host - its arrival time.
+"""
+import concurrent.futures as cf
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import problems as P
+from .batch import Batch, Batch32
+from ._lib import FP64
+
+
+@dataclass
+class Request:
+    rid: int
+    kind: str            # "vortex2d" or "bubble3d"
+    Ne: int
+    N: int
+    nsteps: int          # end time in LSRK54 steps of the problem's dt
+    arrival: float       # seconds after the stream starts
+    seed: int = 0
+    # filled in by the server
+    prepared: float = None
+    admitted: float = None
+    done: float = None
+    result: np.ndarray = field(default=None, repr=False)
+    setup: object = field(default=None, repr=False)  # (setup, dt) when the server keeps them
+    dof: int = 0
+    dt: float = 0.0
+
+    @property
+    def latency(self):
+        return None if self.done is None else self.done - self.arrival
+
+
+def poisson_stream(n=64, rate=200.0, seed=2024, vortex_ne=(16, 64), vortex_n=(3, 4, 5, 6, 7),
+                   bubble_ne=(8, 24), bubble_n=(3, 4, 5), steps=(50, 500)):
+    """Seeded Poisson arrivals (mean `rate` per second): half 2D isentropic
+    vortices (vortex_ne[0]^2..vortex_ne[1]^2 quads, N in vortex_n), half 3D
+    rising bubbles (bubble_ne[0]^3..bubble_ne[1]^3 hexes, N in bubble_n),
+    each to an end time of steps[0]..steps[1] steps."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    out = []
+    for i in range(n):
+        if rng.random() < 0.5:
+            kind, ne, N = "vortex2d", int(rng.integers(vortex_ne[0], vortex_ne[1] + 1)), \
+                int(rng.choice(vortex_n))
+        else:
+            kind, ne, N = "bubble3d", int(rng.integers(bubble_ne[0], bubble_ne[1] + 1)), \
+                int(rng.choice(bubble_n))
+        out.append(Request(rid=i, kind=kind, Ne=ne, N=N,
+                           nsteps=int(rng.integers(steps[0], steps[1] + 1)),
+                           arrival=float(t[i]), seed=int(rng.integers(1, 1 << 30))))
+    return out
+
+
+def make_setup(req):
+    """Host preparation of one request: (setup, dt)."""
+    if req.kind == "vortex2d":
+        s = P.vortex2d(Ne=req.Ne, N=req.N)
+        dt = P.euler_courant(s, s.Q)
+    elif req.kind == "bubble3d":
+        s = P.bubble3d(Ne=req.Ne, N=req.N, dtheta=2.0, jitter=1e-3, seed=req.seed)
+        dt = P.cfl(s, s.Q) / s.N ** np.sqrt(2)  # the reference's step (euler3d_set2c.jl:1104)
+    else:
+        raise ValueError("unknown request kind %r" % req.kind)
+    return s, dt
+
+
+class Server:
+    def __init__(self, precision="fp32", hbm_budget=64 << 30, max_solves=256, step_chunk=8,
+                 prep_workers=8, graph=True, epw=None, keep_setups=False, device="cuda"):
+        if precision not in ("fp32", "fp64"):
+            raise ValueError("precision is 'fp32' (perturbation form) or 'fp64'")
+        self.precision = precision
+        self.hbm_budget = int(hbm_budget)
+        self.max_solves = int(max_solves)
+        self.step_chunk = int(step_chunk)
+        self.prep_workers = int(prep_workers)
+        self.graph = graph
+        self.keep_setups = keep_setups
+        self.device = torch.device(device)
+        self.epw = dict(epw or {})   # (dim, N) -> elements per workgroup
+        self.batches = {}
+        self.streams = {}
+        self.trace = []              # (time, active solves, resident bytes)
+
+    def _batch(self, dim):
+        b = self.batches.get(dim)
+        if b is None:
+            b = Batch32(dim, device=self.device) if self.precision == "fp32" else \
+                Batch(P._lib.EULER, dim, FP64, device=self.device)
+            self.batches[dim] = b
+            self.streams[dim] = torch.cuda.Stream(device=self.device)
+        return b
+
+    def footprint(self, s):
+        return self._batch(s.dim).footprint(s)
+
+    def serve(self, requests, setups=None, clock=time.perf_counter):
+        """Run the stream; returns the requests with timings and results.
+        setups: optional {rid: (setup, dt)} prepared beforehand (then host
+        preparation is not part of the latency)."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        pool = cf.ThreadPoolExecutor(max_workers=self.prep_workers)
+        futures, ready, active = {}, [], {}
+        resident = 0
+        nxt = 0
+        t0 = clock()
+        done = 0
+        try:
+            while done < len(reqs):
+                now = clock() - t0
+                while nxt < len(reqs) and reqs[nxt].arrival <= now:
+                    r = reqs[nxt]
+                    if setups is not None and r.rid in setups:
+                        fut = cf.Future()
+                        fut.set_result(setups[r.rid])
+                    else:
+                        fut = pool.submit(make_setup, r)
+                    futures[r.rid] = (r, fut)
+                    nxt += 1
+                for rid in list(futures):
+                    r, fut = futures[rid]
+                    if fut.done():
+                        s, dt = fut.result()
+                        r.prepared = clock() - t0
+                        r.dof, r.dt = s.ndof(), dt
+                        if self.keep_setups:
+                            r.setup = (s, dt)
+                        ready.append((r, s, dt))
+                        del futures[rid]
+                ready.sort(key=lambda x: x[0].arrival)
+                # admission: arrival order, by HBM footprint and free slots
+                admitted = False
+                while ready and len(active) < self.max_solves:
+                    r, s, dt = ready[0]
+                    need = self.footprint(s)
+                    if active and resident + need > self.hbm_budget:
+                        break
+                    ready.pop(0)
+                    b = self._batch(s.dim)
+                    sid = b.add(s, dt)
+                    active[r.rid] = (r, b, sid, need)
+                    resident += need
+                    r.admitted = clock() - t0
+                    admitted = True
+                if admitted:
+                    for dim, b in self.batches.items():
+                        if b.solves and b._table is None:
+                            self._finalize(dim, b)
+                if not active:
+                    if futures:
+                        cf.wait([f for _, f in futures.values()], timeout=0.002,
+                                return_when=cf.FIRST_COMPLETED)
+                    elif nxt < len(reqs):
+                        time.sleep(max(0.0, min(0.002, reqs[nxt].arrival - (clock() - t0))))
+                    continue
+                # advance every resident solve by k steps (k <= the nearest end time)
+                k = min(self.step_chunk, min(r.nsteps - b.solves[sid].steps
+                                             for r, b, sid, _ in active.values()))
+                cur = torch.cuda.current_stream(self.device)
+                for dim, b in self.batches.items():
+                    if not b.solves:
+                        continue
+                    st = self.streams[dim]
+                    st.wait_stream(cur)
+                    with torch.cuda.stream(st):
+                        b.run(k, graph=self.graph)
+                for dim, b in self.batches.items():
+                    if b.solves:
+                        cur.wait_stream(self.streams[dim])
+                self.trace.append((clock() - t0, len(active), resident))
+                # retire finished requests
+                fin = [rid for rid, (r, b, sid, _) in active.items()
+                       if b.solves[sid].steps >= r.nsteps]
+                for rid in fin:
+                    r, b, sid, need = active.pop(rid)
+                    r.result = b.remove(sid)
+                    r.done = clock() - t0
+                    resident -= need
+                    done += 1
+                if fin:
+                    for dim, b in self.batches.items():
+                        if b.solves and b._table is None:
+                            self._finalize(dim, b)
+            for b in self.batches.values():
+                b.check()
+        finally:
+            pool.shutdown(wait=True)
+        self.makespan = clock() - t0
+        return reqs
+
+    def _finalize(self, dim, b):
+        Ns = sorted({v.N for v in b.solves.values()})
+        known = {N: self.epw[(dim, N)] for N in Ns if (dim, N) in self.epw}
+        tune = [N for N in Ns if N not in known]
+        b.finalize(epw=known, autotune=bool(tune))
+        for N in Ns:
+            self.epw[(dim, N)] = b.epw[N]
+
+
+def summary(reqs, makespan):
+    """p50 / p99 latency, throughput figures of a served stream."""
+    lat = np.array([r.latency for r in reqs])
+    dof_stages = sum(5.0 * r.dof * r.nsteps for r in reqs)
+    return {
+        "requests": len(reqs),
+        "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "max_latency_s": float(lat.max()),
+        "mean_queue_s": float(np.mean([r.admitted - r.arrival for r in reqs])),
+        "makespan_s": float(makespan),
+        "aggregate_gdof_stage_s": dof_stages / makespan / 1e9,
+        "dof_stages": dof_stages,
+    }

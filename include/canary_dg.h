@@ -336,6 +336,39 @@ double canary_cfl(int eqn, int dim, int N, const double* vgeo, const double* Q,
 double canary_l2energysquared(int Np, int nstate, int nvgeo, int iMJ, const double* Q,
                               const double* vgeo, int64_t nelem);
 
+/* ---- device side of the driver's other work (device pointers) ------------ */
+
+/* computegeometry on the GPU (euler3d_set2c.jl:101-139 / euler2d_set2c.jl:97-134):
+ * elemtocoord (d, 2^d, nelem), vmapM (1-based) -> vgeo, sgeo, bit-identical
+ * to canary_computegeometry.  Errors (vmapM out of range) are recorded for
+ * canary_check_errors. */
+int canary_computegeometry_device(int d, int N, int64_t nelem, const double* elemtocoord,
+                                  const int64_t* vmapM, double* vgeo, double* sgeo,
+                                  void* stream);
+
+/* Ghost-element exchange buffers (word_bytes 8 = FP64 states, 4 = FP32):
+ * fillsendQ!:     sendQ[:, :, e] = Q[:, :, sendelems[e]], sendelems 1-based
+ *                 (euler3d_set2c.jl:645-659, 682-693);
+ * transferrecvQ!: Q[:, :, nrealelem + e] = recvQ[:, :, e]
+ *                 (euler3d_set2c.jl:663-676, 695-709). */
+int canary_fillsendQ(int dim, int N, int nstate, void* sendQ, const void* Q,
+                     const int64_t* sendelems, int64_t nsend, int64_t nelem_total,
+                     int word_bytes, void* stream);
+int canary_transferrecvQ(int dim, int N, int nstate, void* Q, const void* recvQ, int64_t nrecv,
+                         int64_t nrealelem, int word_bytes, void* stream);
+
+/* Deterministic device reductions into out[0] (device memory); work holds
+ * CANARY_REDUCE_WORK doubles of scratch.  Same values as the host
+ * canary_l2energysquared / canary_cfl up to summation order (energy) and
+ * the last bit of pow (cfl, Euler). */
+#define CANARY_REDUCE_WORK 1024
+int canary_l2energysquared_device(int Np, int nstate, int nvgeo, int iMJ, const double* Q,
+                                  const double* vgeo, int64_t nelem, double* work, double* out,
+                                  void* stream);
+int canary_cfl_device(int eqn, int dim, int N, const double* vgeo, const double* Q,
+                      int64_t nelem, const canary_phys* phys, double* work, double* out,
+                      void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,79 @@
+"""Served form on the GPU: continuous batching gives every request exactly
+the result it gets alone (the batched kernels compute each element with the
+same code whatever else is resident), admission keeps the resident HBM
+footprint within the budget, and every request finishes."""
+import numpy as np
+import pytest
+import torch
+
+import canary_amd.kernels as K
+import canary_amd.problems as P
+from canary_amd.perturb import Perturb32
+from canary_amd.serve import Request, Server, make_setup, poisson_stream, summary
+
+from gpu_util import DeviceSetup, dev, host
+from states import max_ulp, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def small_stream(seed=7, n=8):
+    return poisson_stream(n=n, rate=200.0, seed=seed, vortex_ne=(3, 6), vortex_n=(3, 4, 5, 7),
+                          bubble_ne=(2, 3), bubble_n=(2, 3, 4), steps=(3, 12))
+
+
+def fp32_alone(s, dt, nsteps):
+    pf = Perturb32(s, s.Q0, s.Q)
+    pf.run(dt, nsteps)
+    pf.check()
+    return host(pf.full_state(), s.Q.shape)
+
+
+def fp64_alone(s, dt, nsteps):
+    A, B, _ = P.rk_coefs()
+    ds = DeviceSetup(s)
+    q, q2, rhs = dev(s.Q), dev(s.Q), dev(np.zeros_like(s.Q))
+    for _ in range(nsteps):
+        for st in range(5):
+            K.stage_fused(s.dim, s.N, rhs, q, q2, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM,
+                          ds.vmapP, ds.bndy, A[(st + 1) % 5], B[st], dt, eqn=s.eqn, phys=s.phys)
+            q, q2 = q2, q
+    K.check_errors()
+    return host(q, s.Q.shape)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_served_results_equal_alone(cuda, precision):
+    reqs = small_stream()
+    setups = {r.rid: make_setup(r) for r in reqs}
+    sizes = sorted(Server(precision).footprint(s) for s, _ in setups.values())
+    budget = sizes[-1] + sizes[-2]  # at most about two of the largest resident at once
+    srv = Server(precision, hbm_budget=budget, step_chunk=4)
+    out = srv.serve(reqs)
+    st = summary(out, srv.makespan)
+    print(precision, {k: round(v, 4) if isinstance(v, float) else v for k, v in st.items()},
+          flush=True)
+    assert all(r.done is not None and r.latency > 0 for r in out)
+    assert all(r.admitted >= r.prepared >= r.arrival for r in out)
+    assert max(res for _, _, res in srv.trace) <= max(budget, sizes[-1])
+    alone = fp32_alone if precision == "fp32" else fp64_alone
+    for r in out:
+        s, dt = setups[r.rid]
+        want = alone(s, dt, r.nsteps)
+        assert np.isfinite(want).all()
+        assert max_ulp(r.result, want) == 0, "request %d (%s N=%d)" % (r.rid, r.kind, r.N)
+        if precision == "fp32":
+            ref = fp64_alone(s, dt, r.nsteps)
+            err = rel_l2(r.result, ref)
+            assert err <= 1e-5, (r.rid, err)
+
+
+def test_served_in_mix_equals_served_alone(cuda):
+    reqs = small_stream(seed=11, n=6)
+    setups = {r.rid: make_setup(r) for r in reqs}
+    mixed = Server("fp32", step_chunk=3).serve(reqs, setups=setups)
+    for r in mixed:
+        solo = Request(rid=r.rid, kind=r.kind, Ne=r.Ne, N=r.N, nsteps=r.nsteps, arrival=0.0,
+                       seed=r.seed)
+        got = Server("fp32").serve([solo], setups=setups)[0]
+        assert max_ulp(got.result, r.result) == 0
