@@ -136,6 +136,15 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    # measured copy bandwidth of the box (1 GiB device-to-device, read + write)
+    cw = (1 << 30) // 8
+    csrc = torch.ones(cw, dtype=torch.float64, device=dev)
+    cdst = torch.empty_like(csrc)
+    cdst.copy_(csrc)
+    t_copy, _ = timed(lambda: [cdst.copy_(csrc) for _ in range(10)])
+    copy_gbs = 2 * (1 << 30) * 10 / t_copy / 1e9
+    del csrc, cdst
+
     # fused path (the measured value)
     q, q2 = Q0.clone(), torch.empty_like(Q0)
     rhs = torch.zeros_like(Q0)
@@ -206,7 +215,11 @@ def main():
                      "traffic": traffic,
                      "kernel": "stage_kernel<Euler3D,5,1> (fused volume+face+update)",
                      "bytes_per_element_stage": algorithmic_bytes(3, s.N),
-                     "avg_launch_us": round(launch_s * 1e6, 2)},
+                     "bytes_per_element_stage_without_vmapM": algorithmic_bytes(3, s.N)
+                     - 6 * (s.N + 1) ** 2 * 8,
+                     "avg_launch_us": round(launch_s * 1e6, 2),
+                     "measured_copy_gbs": round(copy_gbs, 1),
+                     "frac_of_measured_copy": round(achieved / copy_gbs, 4)},
         "cpu_baseline": cpu,
         "wall_s_timed_region": round(wall_fused, 4),
         **extra,

@@ -41,7 +41,7 @@ def _cmds():
     cmds = []
     for s in HOST_SOURCES:
         o = os.path.join(OBJ, s.replace("/", "_") + ".o")
-        cmds.append((o, ["g++", "-std=c++17", "-O2", "-fPIC", "-ffp-contract=off",
+        cmds.append((o, ["g++", "-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-fopenmp",
                          "-fno-fast-math", "-Wall", "-Wno-unused-function", *inc,
                          "-c", os.path.join(CSRC, s), "-o", o]))
     hip = _hipcc()
@@ -72,7 +72,7 @@ def build(jobs=None, verbose=False):
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(run, cmds))
     tmp = LIB + ".tmp"
-    link = [_hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, *objs, "-o", tmp]
+    link = [_hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, *objs, "-lgomp", "-o", tmp]
     r = subprocess.run(link, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))

@@ -53,6 +53,7 @@ extern "C" int canary_creategrid(int d, int Nq, int64_t nelem, const double* e2c
   const size_t nn = Np * nelem;
   for (int n = 0; n < d; ++n) {
     double* f = X + n * nn;
+#pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < nelem; ++e) {
       const double* c = e2c + (size_t)d * nvert * e;
       auto C = [&](int v) { return c[n + d * v]; };
@@ -114,6 +115,7 @@ extern "C" int canary_computemetric(int d, int Nq, int64_t nelem, const double* 
     const double *x = X, *y = X + nn;
     double *J = vol, *xix = vol + nn, *etax = vol + 2 * nn, *xiy = vol + 3 * nn, *etay = vol + 4 * nn;
     double *sJ = surf, *nx = surf + ns, *ny = surf + 2 * ns;
+#pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < nelem; ++e) {
       auto V = [&](int i, int j) { return (size_t)i + Nq * (j + (size_t)Nq * e); };
       for (int j = 0; j < Nq; ++j)
@@ -160,9 +162,12 @@ extern "C" int canary_computemetric(int d, int Nq, int64_t nelem, const double* 
   double *xiy = vol + 4 * nn, *etay = vol + 5 * nn, *zetay = vol + 6 * nn;
   double *xiz = vol + 7 * nn, *etaz = vol + 8 * nn, *zetaz = vol + 9 * nn;
   double *sJ = surf, *nx = surf + ns, *ny = surf + 2 * ns, *nz = surf + 3 * ns;
+  auto L = [&](int i, int j, int k) { return (size_t)i + Nq * (j + (size_t)Nq * k); };
+#pragma omp parallel
+  {
   std::vector<double> JI2(Np), yzr(Np), yzs(Np), yzt(Np), zxr(Np), zxs(Np), zxt(Np), xyr(Np),
       xys(Np), xyt(Np);
-  auto L = [&](int i, int j, int k) { return (size_t)i + Nq * (j + (size_t)Nq * k); };
+#pragma omp for schedule(static)
   for (int64_t e = 0; e < nelem; ++e) {
     const size_t eo = Np * e;
     for (int k = 0; k < Nq; ++k)
@@ -262,6 +267,7 @@ extern "C" int canary_computemetric(int d, int Nq, int64_t nelem, const double* 
         }
       }
   }
+  }  // omp parallel
   return 0;
 }
 
@@ -290,6 +296,7 @@ extern "C" int canary_computegeometry(int d, int N, int64_t nelem, const double*
     int i = p % Nq, j = (int)(p / Nq);
     sM[p] = d == 3 ? w[j] * w[i] : w[i];
   }
+#pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < nelem; ++e)
     for (size_t p = 0; p < Np; ++p) {
       size_t g = p + Np * e;
@@ -300,6 +307,7 @@ extern "C" int canary_computegeometry(int d, int N, int64_t nelem, const double*
       v[Np * iMJI] = 1 / mj;
       for (int c = 0; c < d; ++c) v[Np * (ix + c)] = X[c * nn + g];
     }
+#pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < nelem; ++e)
     for (int f = 0; f < nface; ++f)
       for (size_t n = 0; n < Nfp; ++n) {

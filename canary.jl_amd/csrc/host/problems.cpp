@@ -70,6 +70,7 @@ extern "C" int canary_ic_bubble(int dim, int N, int64_t nelem, const double* vge
   const int Np = dim == 3 ? Nq * Nq * Nq : Nq * Nq;
   const int ns = dim == 3 ? 5 : 4, nv = dim == 3 ? 14 : 8, ix = dim == 3 ? 11 : 6;
   const double p0 = ph->p0, R_gas = ph->R_gas, c_p = ph->c_p, c_v = ph->c_v, g = ph->gravity;
+#pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < nelem; ++e)
     for (int n = 0; n < Np; ++n) {
       const double* vg = vgeo + (size_t)Np * nv * e;
@@ -116,6 +117,7 @@ extern "C" int canary_ic_vortex2d(int N, int64_t nelem, const double* vgeo,
   const int Nq = N + 1, Np = Nq * Nq, ns = 4, nv = 8;
   const double gam = v->gamma;
   double xc = v->x0 + v->u0 * t, yc = v->y0 + v->v0 * t;
+#pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < nelem; ++e)
     for (int n = 0; n < Np; ++n) {
       const double* vg = vgeo + (size_t)Np * nv * e;
@@ -151,6 +153,7 @@ extern "C" int canary_ic_swe_bumps(int N, int64_t nelem, const double* vgeo, dou
                                    double* Q) {
   if (N < 1 || nelem < 0 || nbump < 0) return CANARY_ERR_ARG;
   const int Nq = N + 1, Np = Nq * Nq, ns = 4, nv = 8;
+#pragma omp parallel for schedule(static)
   for (int64_t e = 0; e < nelem; ++e)
     for (int n = 0; n < Np; ++n) {
       const double* vg = vgeo + (size_t)Np * nv * e;
