@@ -67,6 +67,9 @@ def main():
     ap.add_argument("--served-requests", type=int, default=64,
                     help="requests of the served-form run (0: skip)")
     ap.add_argument("--served-rate", type=float, default=200.0)
+    ap.add_argument("--form", choices=["batched", "served"], default=None,
+                    help="run only that added form and print its JSON (used by the main run, "
+                         "which measures each added form in a child process)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also run one device-to-device copy of this many GiB (untimed), the "
                          "known-size reference for HBM byte counters")
@@ -76,6 +79,15 @@ def main():
               "multi-GPU sharding is described, not run (got --gpus %d)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+
+    if args.form is not None:
+        load_package()
+        if args.form == "batched":
+            res = batched_form(args.batched_solves, args.batched_steps)
+        else:
+            res = served_form(args.served_requests, args.served_rate)
+        print(json.dumps(res), flush=True)
+        return
 
     import numpy as np
     import torch
@@ -184,12 +196,13 @@ def main():
         del q, q2, rhs, vgeo, sgeo, vM, vP, bnd, Q0, holder, qf
         torch.cuda.empty_cache()
         extra["precision_form"] = precision_form(args.precision_ne, s.N, args.precision_steps)
+    torch.cuda.empty_cache()
     if args.batched_solves > 0:
-        torch.cuda.empty_cache()
-        extra["batched_form"] = batched_form(args.batched_solves, args.batched_steps)
+        extra["batched_form"] = form_child("batched", ["--batched-solves", str(args.batched_solves),
+                                                       "--batched-steps", str(args.batched_steps)])
     if args.served_requests > 0:
-        torch.cuda.empty_cache()
-        extra["served_form"] = served_form(args.served_requests, args.served_rate)
+        extra["served_form"] = form_child("served", ["--served-requests", str(args.served_requests),
+                                                     "--served-rate", str(args.served_rate)])
 
     traffic = measured_traffic("euler3d_N%d_%d" % (s.N, args.ne))
     line = {
@@ -309,6 +322,26 @@ def precision_form(ne, N, steps_csv, timed_steps=10):
             "rel_l2_vs_fp64": errors, "dt": dt}
 
 
+def form_child(name, extra_args, timeout=480):
+    """Measure one added form in a child process of its own (fresh device
+    state, as in the tests); a failure there is reported in the JSON line
+    instead of ending the benchmark."""
+    import subprocess
+    cmd = [sys.executable, "-X", "faulthandler", os.path.abspath(__file__), "--form", name,
+           *extra_args]
+    sys.stderr.flush()
+    try:  # the child's progress lines go straight to this process's stderr
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=timeout)
+    except subprocess.TimeoutExpired:
+        return {"error": "timed out after %d s" % timeout}
+    if r.returncode != 0:
+        return {"error": "exit status %d" % r.returncode}
+    try:
+        return json.loads(r.stdout.strip().splitlines()[-1])
+    except (ValueError, IndexError):
+        return {"error": "no JSON from the child", "stdout_tail": r.stdout[-500:]}
+
+
 def _gpu_timer():
     import torch
 
@@ -362,6 +395,8 @@ def batched_form(nsolves, steps, seed=42):
                      "hbm_frac": round(b.bytes_per_stage() * 5 * steps / t / 1e9 / HBM_PEAK_GBS, 4),
                      "epw": {str(k): v for k, v in sorted(b.epw.items())}}
         states[name] = [b.state(i) for i in ids]
+        print("batched form %s: %.3f GDOF-stage/s" % (name, out[name]["gdof_stage_s"]),
+              file=sys.stderr, flush=True)
         del b
         torch.cuda.empty_cache()
     errs = [np.linalg.norm(a[:, :3] - c[:, :3]) / np.linalg.norm(c[:, :3])
@@ -446,6 +481,8 @@ def served_form(nreq, rate, seed=2024):
                     a, b2 = b2, a
             hold["q"] = a
         t_seq += timed(go)
+        if r.rid % 8 == 7:
+            print("sequential FP64 baseline: request %d done" % r.rid, file=sys.stderr, flush=True)
         ref = hold["q"].cpu().numpy().reshape(s.Q.shape, order="F")
         errs.append(np.linalg.norm(r.result - ref) / np.linalg.norm(ref))
         del vg, sg, D, vM, vP, bd, q, q2, rhs, hold

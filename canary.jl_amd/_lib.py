@@ -95,6 +95,8 @@ _SIGS = {
     "canary_batch_variant": (INT, [INT, INT, INT, INT, INT, C.POINTER(INT), C.POINTER(INT)]),
     "canary_batch_stage": (INT, [INT, INT, INT, INT, INT, P, P, I64, INT, D, D, P]),
     "canary_device_count": (INT, []),
+    "canary_stream_create": (INT, [C.POINTER(P)]),
+    "canary_stream_destroy": (INT, [P]),
     "canary_computegeometry_device": (INT, [INT, INT, I64, P, P, P, P, P]),
     "canary_fillsendQ": (INT, [INT, INT, INT, P, P, P, I64, I64, INT, P]),
     "canary_transferrecvQ": (INT, [INT, INT, INT, P, P, I64, I64, INT, P]),

@@ -17,6 +17,7 @@ parallel).  All solves share one stage parity: buffer [parity] holds the
 current state and a stage writes buffer [1 - parity].
 """
 import ctypes as C
+import gc
 
 import numpy as np
 import torch
@@ -25,6 +26,7 @@ from . import _lib
 from ._lib import EULER, SWE, FP64, FP32, check, lib
 from .perturb import Perturb32
 from .problems import rk_coefs
+from .streams import OwnedStream
 
 _NS = {(EULER, 3): 5, (EULER, 2): 4, (SWE, 2): 4}
 _NSR = {(EULER, 3): 5, (EULER, 2): 4, (SWE, 2): 3}
@@ -98,7 +100,8 @@ class _BatchBase:
         self._table = None
         self._items = None
         self._graphs = {}
-        self._streams = []
+        self._streams = []        # OwnedStream per N class (forks of a captured stage)
+        self._capture = None      # OwnedStream the step graphs are captured on
         self._version = 0
 
     # -- to be provided by the concrete forms
@@ -190,7 +193,7 @@ class _BatchBase:
         self._graphs = {}
         self._version += 1
         while len(self._streams) < len(self.groups):
-            self._streams.append(torch.cuda.Stream(device=self.device))
+            self._streams.append(OwnedStream(self.device))
 
     # ------------------------------------------------------------ stages
     def _st(self):
@@ -209,13 +212,13 @@ class _BatchBase:
         if concurrent and len(self.groups) > 1:
             cur = torch.cuda.current_stream(self.device)
             for g, (N, epw, off, cnt) in enumerate(self.groups):
-                s = self._streams[g]
+                s = self._streams[g].stream
                 s.wait_stream(cur)
                 with torch.cuda.stream(s):
                     self._launch(N, epw, base + 16 * off, cnt, self.parity, rka, rkb,
                                  C.c_void_p(s.cuda_stream))
             for g in range(len(self.groups)):
-                cur.wait_stream(self._streams[g])
+                cur.wait_stream(self._streams[g].stream)
         else:
             st = self._st()
             for N, epw, off, cnt in self.groups:
@@ -234,13 +237,15 @@ class _BatchBase:
         g = self._graphs.get(key)
         if g is None:
             p0 = self.parity
-            cur = torch.cuda.current_stream(self.device)
-            side = torch.cuda.Stream(device=self.device)
-            side.wait_stream(cur)
+            if self._capture is None:
+                self._capture = OwnedStream(self.device)
+            side = self._capture.stream
+            # the capture protocol of torch.cuda.graph (idle device, no garbage
+            # left to be freed during the capture), without its cache flush
+            torch.cuda.synchronize(self.device)
+            gc.collect()
             g = torch.cuda.CUDAGraph()
             A, B, _ = rk_coefs()
-            # capture_begin/end directly: the torch.cuda.graph context would also
-            # empty the allocator cache on every capture
             with torch.cuda.stream(side):
                 g.capture_begin()
                 try:
@@ -248,7 +253,7 @@ class _BatchBase:
                         self.stage(A[(s + 1) % 5], B[s], concurrent=True)
                 finally:
                     g.capture_end()
-            cur.wait_stream(side)
+            torch.cuda.current_stream(self.device).wait_stream(side)
             self.parity = p0  # capture recorded the launches, it did not run them
             self._graphs[key] = g
         return g

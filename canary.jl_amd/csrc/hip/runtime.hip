@@ -188,3 +188,22 @@ extern "C" int canary_device_count(void) {
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
   return n;
 }
+
+// Streams owned by the batched / served runtimes: created here so that every
+// runtime stream is distinct (a stream borrowed from a shared pool can alias
+// another user's stream, which then joins a graph capture with unrelated
+// work pending).  Default priority, not synchronising with the null stream.
+extern "C" int canary_stream_create(void** out) {
+  if (!out) return CANARY_ERR_ARG;
+  hipStream_t s = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return CANARY_ERR_LAUNCH;
+  *out = (void*)s;
+  return 0;
+}
+
+extern "C" int canary_stream_destroy(void* s) {
+  if (!s) return CANARY_ERR_ARG;
+  hipStream_t st = (hipStream_t)s;
+  if (hipStreamSynchronize(st) != hipSuccess) return CANARY_ERR_LAUNCH;
+  return hipStreamDestroy(st) == hipSuccess ? 0 : CANARY_ERR_LAUNCH;
+}

@@ -28,6 +28,7 @@ import torch
 from . import problems as P
 from .batch import Batch, Batch32
 from ._lib import FP64
+from .streams import OwnedStream
 
 
 @dataclass
@@ -112,7 +113,7 @@ class Server:
             b = Batch32(dim, device=self.device) if self.precision == "fp32" else \
                 Batch(P._lib.EULER, dim, FP64, device=self.device)
             self.batches[dim] = b
-            self.streams[dim] = torch.cuda.Stream(device=self.device)
+            self.streams[dim] = OwnedStream(self.device)
         return b
 
     def footprint(self, s):
@@ -184,13 +185,13 @@ class Server:
                 for dim, b in self.batches.items():
                     if not b.solves:
                         continue
-                    st = self.streams[dim]
+                    st = self.streams[dim].stream
                     st.wait_stream(cur)
                     with torch.cuda.stream(st):
                         b.run(k, graph=self.graph)
                 for dim, b in self.batches.items():
                     if b.solves:
-                        cur.wait_stream(self.streams[dim])
+                        cur.wait_stream(self.streams[dim].stream)
                 self.trace.append((clock() - t0, len(active), resident))
                 # retire finished requests
                 fin = [rid for rid, (r, b, sid, _) in active.items()

@@ -220,6 +220,12 @@ int canary_check_errors(void* stream, int64_t* info);
 /* Number of HIP devices visible (0 without a GPU); never faults. */
 int canary_device_count(void);
 
+/* A stream of the caller's own (non-blocking, default priority) and its
+ * release (after the work on it completes); used by the batched and served
+ * runtimes so that their streams never alias one another. */
+int canary_stream_create(void** stream);
+int canary_stream_destroy(void* stream);
+
 /* ======================================================================
  * Host preparation (the inputs of the kernels).  All outputs are written
  * to caller-provided host arrays in the reference layouts.

@@ -4,7 +4,6 @@ same code whatever else is resident), admission keeps the resident HBM
 footprint within the budget, and every request finishes."""
 import numpy as np
 import pytest
-import torch
 
 import canary_amd.kernels as K
 import canary_amd.problems as P
