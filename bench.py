@@ -434,14 +434,15 @@ def batched_form(nsolves, steps, seed=42):
 
 def served_form(nreq, rate, seed=2024):
     """Poisson request stream (half 2D vortices, half 3D bubbles) through the
-    continuous-batching server (FP32 perturbation form; host preparation of
-    each request inside its latency), against the same requests solved one
-    after another with the plain FP64 fused stage (preparation excluded)."""
+    continuous-batching server (FP32 perturbation form; each request's
+    preparation -- host mesh, device geometry and states -- inside its
+    latency), against the same requests solved one after another with the
+    plain FP64 fused stage (preparation excluded)."""
     import numpy as np
     import torch
     import canary_amd.kernels as K
     import canary_amd.problems as P
-    from canary_amd.serve import Server, poisson_stream, summary
+    from canary_amd.serve import Server, make_setup_device, poisson_stream, summary
 
     reqs = poisson_stream(n=nreq, rate=rate, seed=seed)
     # an untimed tiny stream first (first launches, allocator); the measured
@@ -465,27 +466,27 @@ def served_form(nreq, rate, seed=2024):
 
     t_seq, errs = 0.0, []
     for r in out_reqs:
-        s, dt = r.setup
-        vg, sg, D, vM, vP, bd = (d(x) for x in (s.vgeo, s.sgeo, s.D, s.vmapM, s.vmapP,
-                                                s.elemtobndy))
-        q, q2, rhs = d(s.Q), torch.empty(s.Q.size, dtype=torch.float64, device=dev), \
-            torch.zeros(s.Q.size, dtype=torch.float64, device=dev)
+        s, dt = make_setup_device(r, r.setup)   # the same device preparation, FP64 arrays kept
+        a = P.device_arrays(s)
+        q, q2, rhs = s.Qd.clone(), torch.empty_like(s.Qd), torch.zeros_like(s.Qd)
         hold = {}
 
         def go():
-            a, b2 = q, q2
+            x, y = q, q2
             for _ in range(r.nsteps):
                 for k in range(5):
-                    K.stage_fused(s.dim, s.N, rhs, a, b2, vg, sg, D, s.nelem, vM, vP, bd,
-                                  A[(k + 1) % 5], B[k], dt, phys=s.phys)
-                    a, b2 = b2, a
-            hold["q"] = a
+                    K.stage_fused(s.dim, s.N, rhs, x, y, a["vgeo"], a["sgeo"], a["D"], s.nelem,
+                                  a["vmapM"], a["vmapP"], a["bndy"], A[(k + 1) % 5], B[k], dt,
+                                  phys=s.phys)
+                    x, y = y, x
+            hold["q"] = x
         t_seq += timed(go)
         if r.rid % 8 == 7:
             print("sequential FP64 baseline: request %d done" % r.rid, file=sys.stderr, flush=True)
-        ref = hold["q"].cpu().numpy().reshape(s.Q.shape, order="F")
+        ref = hold["q"].cpu().numpy().reshape((s.Np, s.nstate, s.nelem), order="F")
         errs.append(np.linalg.norm(r.result - ref) / np.linalg.norm(ref))
-        del vg, sg, D, vM, vP, bd, q, q2, rhs, hold
+        del a, q, q2, rhs, hold
+        s.dev = s.Qd = s.Q0d = None
         r.setup = None
     K.check_errors()
     nk = {"vortex2d": sum(r.kind == "vortex2d" for r in out_reqs)}

@@ -25,7 +25,7 @@ import torch
 from . import _lib
 from ._lib import EULER, SWE, FP64, FP32, check, lib
 from .perturb import Perturb32
-from .problems import rk_coefs
+from .problems import device_arrays, rk_coefs
 from .streams import OwnedStream
 
 _NS = {(EULER, 3): 5, (EULER, 2): 4, (SWE, 2): 4}
@@ -365,21 +365,20 @@ class Batch(_BatchBase):
             raise ValueError("solve (eqn %d, dim %d) does not match the batch" % (s.eqn, s.dim))
         if not self._variants(s.N):
             raise _lib.CanaryError(_lib.ERR_UNSUPPORTED, "batched form at N=%d" % s.N)
-        Q = s.Q if Q is None else Q
-        if Q is None or Q.shape != (s.Np, s.nstate, s.nelem):
-            raise ValueError("Q must have shape (Np, nstate, nelem)")
+        if Q is None:
+            Q = s.Q if s.Q is not None else getattr(s, "Qd", None)
+        if Q is None or Q.size != s.Np * s.nstate * s.nelem:
+            raise ValueError("Q must hold (Np, nstate, nelem) values")
         v = _Solve()
         v.setup, v.dt, v.N, v.nelem = s, float(dt), s.N, s.nelem
         dv, rd = self.device, self.rdtype
-        q = _dev(Q, dv, rd)
+        q = Q.to(dv, rd).clone() if isinstance(Q, torch.Tensor) else _dev(Q, dv, rd)
         v.Q = [q, torch.empty_like(q)] if self.parity == 0 else [torch.empty_like(q), q]
         v.rhs = torch.zeros_like(q)
-        v.vgeo = _dev(s.vgeo, dv, rd)
-        v.sgeo = _dev(s.sgeo, dv, rd)
-        v.D = _dev(s.D, dv, torch.float64)
-        v.vmapM = _dev(s.vmapM, dv, torch.int64)
-        v.vmapP = _dev(s.vmapP, dv, torch.int64)
-        v.bndy = _dev(s.elemtobndy, dv, torch.int64)
+        arr = device_arrays(s, dv)
+        v.vgeo = arr["vgeo"].to(rd)
+        v.sgeo = arr["sgeo"].to(rd)
+        v.D, v.vmapM, v.vmapP, v.bndy = arr["D"], arr["vmapM"], arr["vmapP"], arr["bndy"]
         v.phys = s.phys
         return self._register(v)
 
@@ -397,9 +396,12 @@ class Batch(_BatchBase):
     def footprint(self, s):
         """Device bytes a solve of this setup holds in the batch."""
         w = self._word()
-        nq = s.Np * s.nstate * s.nelem
-        return (w * (3 * nq + s.vgeo.size + s.sgeo.size) + 8 * (s.vmapM.size + s.vmapP.size
-                + s.elemtobndy.size + s.D.size))
+        d = s.dim
+        Np, Nfp, nf, ne = s.Np, (s.N + 1) ** (d - 1), 2 * d, s.nelem
+        nq = Np * s.nstate * ne
+        nvg, nsg = (14, 5) if d == 3 else (8, 4)
+        return (w * (3 * nq + nvg * Np * ne + nsg * Nfp * nf * ne)
+                + 8 * (2 * Nfp * nf * ne + nf * ne + (s.N + 1) ** 2))
 
     def state_tensor(self, sid):
         """Current state of a solve on the device (float64, flat)."""
@@ -444,8 +446,10 @@ class Batch32(_BatchBase):
             raise ValueError("solve (eqn %d, dim %d) does not match the batch" % (s.eqn, s.dim))
         if not self._variants(s.N):
             raise _lib.CanaryError(_lib.ERR_UNSUPPORTED, "batched precision form at N=%d" % s.N)
-        Q = s.Q if Q is None else Q
-        Q0 = s.Q0 if Q0 is None else Q0
+        if Q is None:
+            Q = s.Q if s.Q is not None else getattr(s, "Qd", None)
+        if Q0 is None:
+            Q0 = s.Q0 if s.Q0 is not None else getattr(s, "Q0d", None)
         if Q0 is None:
             raise ValueError("the precision form needs a background state (setup.Q0)")
         pf = Perturb32(s, Q0, Q, device=self.device)

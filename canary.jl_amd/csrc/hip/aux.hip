@@ -5,6 +5,7 @@
 //   transferrecvQ Q[:, :, nrealelem + e] = recvQ[:, :, e]  (euler3d_set2c.jl:663-676)
 //   l2energysquared  sum_e sum_s sum_i MJ q^2               (euler3d_set2c.jl:738-750)
 //   cfl              min over nodes of the local step       (euler3d_set2c.jl:67-97)
+//   courant          Courant-number step of the Euler sets (no reference counterpart)
 //
 // The copies move whole elements (Np * nstate contiguous words) with one
 // lane per word.  The reductions are deterministic: a fixed grid of
@@ -150,6 +151,42 @@ __global__ __launch_bounds__(RED_THREADS) void cfl_kernel(int eqn, int dim, int6
   if (threadIdx.x == 0) work[blockIdx.x] = s;
 }
 
+// Courant-number step of the Euler sets (problems.euler_courant): min over
+// nodes of courant * dx / (|u| + c) / N, dx = sqrt(sum_d (1 / (2 xi^d_d))^2)
+__global__ __launch_bounds__(RED_THREADS) void courant_kernel(int dim, int N, int64_t Np,
+                                                              const double* __restrict__ vgeo,
+                                                              const double* __restrict__ Q,
+                                                              int64_t nelem, canary_phys ph,
+                                                              double courant,
+                                                              double* __restrict__ work) {
+  __shared__ double sh[RED_THREADS];
+  const int ns = dim + 2, nv = dim == 3 ? 14 : 8;
+  const int64_t total = Np * nelem;
+  const int64_t chunk = (total + RED_BLOCKS - 1) / RED_BLOCKS;
+  const int64_t lo = blockIdx.x * chunk;
+  const int64_t hi = lo + chunk < total ? lo + chunk : total;
+  double acc = 1.7976931348623157e308;
+  for (int64_t g = lo + threadIdx.x; g < hi; g += RED_THREADS) {
+    const int64_t e = g / Np, n = g - e * Np;
+    const double* q = Q + (size_t)Np * ns * e;
+    const double* vg = vgeo + (size_t)Np * nv * e;
+    const double rho = q[n + Np * dim], rt = q[n + Np * (dim + 1)];
+    double vel2 = 0.0, dx2 = 0.0;
+    for (int c = 0; c < dim; ++c) {
+      const double u = q[n + Np * c] / rho;
+      vel2 = vel2 + u * u;
+      const double h = 1.0 / (2 * vg[n + Np * (c * (dim + 1))]);  // xi_x, eta_y(, zeta_z)
+      dx2 = dx2 + h * h;
+    }
+    const double P = ph.p0 * pow(ph.R_gas * rt / ph.p0, ph.c_p / ph.c_v);
+    const double cs = sqrt(ph.gamma * P / rho);
+    const double v = courant * sqrt(dx2) / (sqrt(vel2) + cs) / N;
+    acc = v < acc ? v : acc;
+  }
+  const double s = block_reduce<true>(acc, sh);
+  if (threadIdx.x == 0) work[blockIdx.x] = s;
+}
+
 template <bool MIN>
 __global__ __launch_bounds__(RED_THREADS) void finish_kernel(const double* __restrict__ work,
                                                              double* __restrict__ out) {
@@ -246,6 +283,26 @@ extern "C" int canary_cfl_device(int eqn, int dim, int N, const double* vgeo, co
   for (int d = 0; d < dim; ++d) Np *= N + 1;
   hipStream_t st = (hipStream_t)stream;
   cfl_kernel<<<RED_BLOCKS, RED_THREADS, 0, st>>>(eqn, dim, Np, vgeo, Q, nelem, ph, work);
+  finish_kernel<true><<<1, RED_THREADS, 0, st>>>(work, out);
+  return status();
+}
+
+extern "C" int canary_courant_device(int dim, int N, const double* vgeo, const double* Q,
+                                     int64_t nelem, const canary_phys* phys, double courant,
+                                     double* work, double* out, void* stream) {
+  if ((dim != 2 && dim != 3) || N < 1 || nelem < 0 || !work || !out ||
+      (nelem > 0 && (!Q || !vgeo)))
+    return CANARY_ERR_ARG;
+  canary_phys ph;
+  if (phys)
+    ph = *phys;
+  else
+    canary_phys_default(&ph);
+  int64_t Np = 1;
+  for (int d = 0; d < dim; ++d) Np *= N + 1;
+  hipStream_t st = (hipStream_t)stream;
+  courant_kernel<<<RED_BLOCKS, RED_THREADS, 0, st>>>(dim, N, Np, vgeo, Q, nelem, ph, courant,
+                                                     work);
   finish_kernel<true><<<1, RED_THREADS, 0, st>>>(work, out);
   return status();
 }

@@ -14,7 +14,7 @@ import torch
 
 from . import _lib
 from ._lib import check, lib
-from .problems import rk_coefs
+from .problems import device_arrays, rk_coefs
 
 
 def _ptr(t):
@@ -46,10 +46,12 @@ class Perturb32:
         self.q = torch.empty(Np * ns * ne, dtype=f32, device=dv)
         self.q2 = torch.empty_like(self.q)
         self.rhs = torch.zeros_like(self.q)
-        self.D = _dev(s.D, dv)
-        vgeo, sgeo = _dev(s.vgeo, dv), _dev(s.sgeo, dv)
-        vM, vP, bd = _dev(s.vmapM, dv), _dev(s.vmapP, dv), _dev(s.elemtobndy, dv)
-        q0, qf = _dev(Q0, dv), _dev(Q, dv)
+        # FP64 drop-in arrays: the device ones of a device-prepared setup, else uploads
+        arr = device_arrays(s, dv)
+        self.D = arr["D"]
+        vgeo, sgeo, vM, vP, bd = arr["vgeo"], arr["sgeo"], arr["vmapM"], arr["vmapP"], arr["bndy"]
+        q0 = Q0 if isinstance(Q0, torch.Tensor) else _dev(Q0, dv)
+        qf = Q if isinstance(Q, torch.Tensor) else _dev(Q, dv)
         rc = lib().canary_perturb32_prepare(d, s.N, ne, ne, _ptr(vgeo), _ptr(sgeo), _ptr(vM), _ptr(vP),
                                            _ptr(bd), _ptr(q0), _ptr(qf), C.byref(s.phys),
                                            _ptr(self.vg), _ptr(self.sg), _ptr(self.nbr),

@@ -79,16 +79,18 @@ class Setup:
         return self.nelem * self.Np * self.nstate
 
 
-def build(eqn, dim, N, x, periodic, boundary=None, phys=None, label=""):
+def build(eqn, dim, N, x, periodic, boundary=None, phys=None, label="", geometry=True):
     """brickmesh -> partition -> connectmesh -> mappings -> lglpoints /
-    spectralderivative -> computegeometry (one rank)."""
+    spectralderivative -> computegeometry (one rank).  geometry=False stops
+    before computegeometry (vgeo = sgeo = None): the device path builds them
+    on the GPU (to_device)."""
     etv, etc, etb, fcs = M.brickmesh(x, periodic, boundary=boundary)
     etv, etc, etb, fcs = M.partition(etv, etc, etb, fcs)
     mesh = M.connectmesh(etv, etc, etb, fcs)
     vmapM, vmapP = M.mappings(N, mesh["elemtoelem"], mesh["elemtoface"], mesh["elemtoordr"])
     r, w = M.lglpoints(N)
     D = M.spectralderivative(r)
-    vgeo, sgeo = M.computegeometry(dim, N, mesh["elemtocoord"], vmapM)
+    vgeo, sgeo = M.computegeometry(dim, N, mesh["elemtocoord"], vmapM) if geometry else (None, None)
     return Setup(eqn=eqn, dim=dim, N=N, nelem=etc.shape[2], D=D, r=r, w=w, vgeo=vgeo, sgeo=sgeo,
                  vmapM=vmapM, vmapP=vmapP, elemtobndy=mesh["elemtobndy"], mesh=mesh,
                  phys=phys if phys is not None else _lib.default_phys(), label=label)
@@ -186,6 +188,151 @@ def l2energysquared(s, Q):
     Qf = np.asfortranarray(Q)
     return lib().canary_l2energysquared(s.Np, s.nstate, s.nvgeo, IMJ[s.dim], _p(Qf), _p(s.vgeo),
                                         s.nelem)
+
+
+# ------------------------------------------------- device-side preparation
+def _torch():
+    import torch
+    return torch
+
+
+def to_device(s, device="cuda"):
+    """Upload the maps and build vgeo / sgeo on the GPU (device
+    computegeometry, bit-identical to the host one).  Sets s.dev = {vgeo,
+    sgeo, D, vmapM, vmapP, bndy} (flat CUDA tensors, reference layouts)."""
+    torch = _torch()
+    from . import kernels as K
+    dv = torch.device(device)
+
+    def up(a):
+        return torch.from_numpy(np.ascontiguousarray(np.asarray(a).reshape(-1, order="F"))).to(dv)
+
+    vM = up(s.vmapM)
+    vgeo, sgeo = K.computegeometry(s.dim, s.N, up(s.mesh["elemtocoord"]), vM)
+    s.dev = dict(vgeo=vgeo, sgeo=sgeo, D=up(s.D), vmapM=vM, vmapP=up(s.vmapP),
+                 bndy=up(s.elemtobndy))
+    return s
+
+
+def device_arrays(s, device="cuda"):
+    """The drop-in arrays of a setup on the device: s.dev when the setup was
+    prepared there, else uploads of the host arrays."""
+    if getattr(s, "dev", None) is not None:
+        return s.dev
+    torch = _torch()
+    dv = torch.device(device)
+
+    def up(a):
+        return torch.from_numpy(np.ascontiguousarray(np.asarray(a).reshape(-1, order="F"))).to(dv)
+
+    return dict(vgeo=up(s.vgeo), sgeo=up(s.sgeo), D=up(s.D), vmapM=up(s.vmapM),
+                vmapP=up(s.vmapP), bndy=up(s.elemtobndy))
+
+
+def device_state(s, which="Q", device="cuda"):
+    """Q (or Q0) of a setup as a flat CUDA float64 tensor."""
+    d = getattr(s, which + "d", None)
+    if d is not None:
+        return d
+    torch = _torch()
+    a = getattr(s, which)
+    return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to(torch.device(device))
+
+
+def _stream():
+    return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def bubble_state_device(s, theta0=300.0, dtheta=0.5, rc=250.0, center=(500.0, 500.0, 350.0),
+                        spherical=False, jitter=0.0, seed=0):
+    torch = _torch()
+    b = _lib.Bubble(theta0=theta0, dtheta=dtheta, rc=rc, xc=center[0], yc=center[1],
+                    zc=center[2] if s.dim == 3 else center[-1], spherical=1 if spherical else 0,
+                    jitter=jitter, seed=seed)
+    vgeo = s.dev["vgeo"]
+    Q = torch.empty(s.Np * s.nstate * s.nelem, dtype=torch.float64, device=vgeo.device)
+    check(lib().canary_ic_bubble_device(s.dim, s.N, s.nelem, C.c_void_p(vgeo.data_ptr()),
+                                        C.byref(b), C.byref(s.phys), C.c_void_p(Q.data_ptr()),
+                                        _stream()), "ic_bubble_device")
+    return Q
+
+
+def vortex_state_device(s, t=0.0, params=None):
+    torch = _torch()
+    params = params or vortex_params()
+    vgeo = s.dev["vgeo"]
+    Q = torch.empty(s.Np * 4 * s.nelem, dtype=torch.float64, device=vgeo.device)
+    check(lib().canary_ic_vortex2d_device(s.N, s.nelem, C.c_void_p(vgeo.data_ptr()),
+                                          C.byref(params), float(t), C.c_void_p(Q.data_ptr()),
+                                          _stream()), "ic_vortex2d_device")
+    return Q
+
+
+def _reduce_device(fn, *args):
+    torch = _torch()
+    work = torch.empty(_lib.REDUCE_WORK, dtype=torch.float64, device="cuda")
+    out = torch.empty((), dtype=torch.float64, device="cuda")
+    check(fn(*args, C.c_void_p(work.data_ptr()), C.c_void_p(out.data_ptr()), _stream()),
+          "device reduction")
+    return float(out.item())
+
+
+def cfl_device(s, Q):
+    """cfl() (before the / N^sqrt(2)) of a device state, on the device."""
+    return _reduce_device(lib().canary_cfl_device, s.eqn, s.dim, s.N,
+                          C.c_void_p(s.dev["vgeo"].data_ptr()), C.c_void_p(Q.data_ptr()),
+                          s.nelem, C.byref(s.phys))
+
+
+def euler_courant_device(s, Q, courant=0.5):
+    """euler_courant of a device state, on the device."""
+    return _reduce_device(lib().canary_courant_device, s.dim, s.N,
+                          C.c_void_p(s.dev["vgeo"].data_ptr()), C.c_void_p(Q.data_ptr()),
+                          s.nelem, C.byref(s.phys), float(courant))
+
+
+def bubble3d_mesh(Ne=32, N=4, L=1000.0, gravity=9.81, walls=True):
+    """Host part of bubble3d without geometry (no GPU work)."""
+    phys = _lib.default_phys(gravity=gravity)
+    x = linrange(0.0, L, Ne + 1)
+    per = (False, False, False) if walls else (True, True, False)
+    return build(_lib.EULER, 3, N, (x, x, x), per, phys=phys,
+                 label="euler3d bubble %d^3 N=%d" % (Ne, N), geometry=False)
+
+
+def bubble3d_on_device(s, dtheta=2.0, jitter=1e-3, seed=1, spherical=True):
+    """Device part of bubble3d: geometry, state and hydrostatic background on
+    the GPU (s.Qd, s.Q0d); s.Q = s.Q0 = None."""
+    to_device(s)
+    s.Qd = bubble_state_device(s, dtheta=dtheta, spherical=spherical, jitter=jitter, seed=seed)
+    s.Q0d = bubble_state_device(s, dtheta=0.0, spherical=spherical)
+    return s
+
+
+def vortex2d_mesh(Ne=10, N=4, half=5.0, params=None):
+    """Host part of vortex2d without geometry (no GPU work)."""
+    x = linrange(-half, half, Ne + 1)
+    params = params or vortex_params(Lx=2 * half, Ly=2 * half)
+    s = build(_lib.EULER, 2, N, (x, x), (True, True), phys=vortex_phys(params.gamma),
+              label="euler2d vortex %d^2 N=%d" % (Ne, N), geometry=False)
+    s.vortex = params
+    return s
+
+
+def vortex2d_on_device(s):
+    """Device part of vortex2d: geometry, exact state at t=0 and the uniform
+    background at rest on the GPU."""
+    torch = _torch()
+    to_device(s)
+    p = s.vortex
+    s.Qd = vortex_state_device(s, 0.0, p)
+    ph = s.phys
+    rt = (ph.p0 / ph.R_gas) * (p.p_inf / ph.p0) ** (ph.c_v / ph.c_p)
+    Q0 = torch.zeros((s.nelem, s.nstate, s.Np), dtype=torch.float64, device=s.Qd.device)
+    Q0[:, s.dim, :] = p.rho_inf
+    Q0[:, s.dim + 1, :] = rt
+    s.Q0d = Q0.reshape(-1)
+    return s
 
 
 # -------------------------------------------------------- named workloads

@@ -5,8 +5,9 @@ euler3d_set2c.jl:1049-1160).
 A request names an equation set / problem, a mesh size, N and an end time
 (a number of LSRK54 steps of the problem's own dt).  The server
 
-  * prepares each request on host worker threads as soon as it arrives
-    (mesh, maps, geometry, initial and background states),
+  * prepares each request's mesh, connectivity and maps on host worker
+    threads as soon as it arrives, and its geometry, initial and background
+    states and time step on the GPU when it is admitted,
   * admits prepared requests in arrival order while the HBM footprint of the
     resident solves stays within a budget and a slot is free,
   * keeps every admitted state resident on the GPU in one batch per
@@ -45,7 +46,7 @@ class Request:
     admitted: float = None
     done: float = None
     result: np.ndarray = field(default=None, repr=False)
-    setup: object = field(default=None, repr=False)  # (setup, dt) when the server keeps them
+    setup: object = field(default=None, repr=False)  # host setup when the server keeps them
     dof: int = 0
     dt: float = 0.0
 
@@ -76,17 +77,32 @@ def poisson_stream(n=64, rate=200.0, seed=2024, vortex_ne=(16, 64), vortex_n=(3,
     return out
 
 
-def make_setup(req):
-    """Host preparation of one request: (setup, dt)."""
+def make_setup_host(req):
+    """Host part of a request's preparation (mesh, connectivity, maps, 1D
+    operators; no geometry, no GPU work): runs on the server's worker
+    threads."""
     if req.kind == "vortex2d":
-        s = P.vortex2d(Ne=req.Ne, N=req.N)
-        dt = P.euler_courant(s, s.Q)
-    elif req.kind == "bubble3d":
-        s = P.bubble3d(Ne=req.Ne, N=req.N, dtheta=2.0, jitter=1e-3, seed=req.seed)
-        dt = P.cfl(s, s.Q) / s.N ** np.sqrt(2)  # the reference's step (euler3d_set2c.jl:1104)
+        return P.vortex2d_mesh(Ne=req.Ne, N=req.N)
+    if req.kind == "bubble3d":
+        return P.bubble3d_mesh(Ne=req.Ne, N=req.N)
+    raise ValueError("unknown request kind %r" % req.kind)
+
+
+def make_setup_device(req, s):
+    """Device part: geometry, initial and background states on the GPU and the
+    time step from a device reduction; returns (setup, dt)."""
+    if req.kind == "vortex2d":
+        P.vortex2d_on_device(s)
+        dt = P.euler_courant_device(s, s.Qd)
     else:
-        raise ValueError("unknown request kind %r" % req.kind)
+        P.bubble3d_on_device(s, dtheta=2.0, jitter=1e-3, seed=req.seed)
+        dt = P.cfl_device(s, s.Qd) / s.N ** np.sqrt(2)  # the reference's step (euler3d_set2c.jl:1104)
     return s, dt
+
+
+def make_setup(req):
+    """Both parts on the calling thread."""
+    return make_setup_device(req, make_setup_host(req))
 
 
 class Server:
@@ -121,8 +137,8 @@ class Server:
 
     def serve(self, requests, setups=None, clock=time.perf_counter):
         """Run the stream; returns the requests with timings and results.
-        setups: optional {rid: (setup, dt)} prepared beforehand (then host
-        preparation is not part of the latency)."""
+        setups: optional {rid: host setup (make_setup_host)} prepared
+        beforehand (then the host part is not in the latency)."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         pool = cf.ThreadPoolExecutor(max_workers=self.prep_workers)
         futures, ready, active = {}, [], {}
@@ -139,30 +155,33 @@ class Server:
                         fut = cf.Future()
                         fut.set_result(setups[r.rid])
                     else:
-                        fut = pool.submit(make_setup, r)
+                        fut = pool.submit(make_setup_host, r)
                     futures[r.rid] = (r, fut)
                     nxt += 1
                 for rid in list(futures):
                     r, fut = futures[rid]
                     if fut.done():
-                        s, dt = fut.result()
+                        s = fut.result()
                         r.prepared = clock() - t0
-                        r.dof, r.dt = s.ndof(), dt
+                        r.dof = s.ndof()
                         if self.keep_setups:
-                            r.setup = (s, dt)
-                        ready.append((r, s, dt))
+                            r.setup = s
+                        ready.append((r, s))
                         del futures[rid]
                 ready.sort(key=lambda x: x[0].arrival)
                 # admission: arrival order, by HBM footprint and free slots
                 admitted = False
                 while ready and len(active) < self.max_solves:
-                    r, s, dt = ready[0]
+                    r, s = ready[0]
                     need = self.footprint(s)
                     if active and resident + need > self.hbm_budget:
                         break
                     ready.pop(0)
+                    s, dt = make_setup_device(r, s)
+                    r.dt = dt
                     b = self._batch(s.dim)
                     sid = b.add(s, dt)
+                    s.dev = s.Qd = s.Q0d = None  # the batch holds what it needs
                     active[r.rid] = (r, b, sid, need)
                     resident += need
                     r.admitted = clock() - t0

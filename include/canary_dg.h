@@ -374,6 +374,21 @@ int canary_l2energysquared_device(int Np, int nstate, int nvgeo, int iMJ, const 
 int canary_cfl_device(int eqn, int dim, int N, const double* vgeo, const double* Q,
                       int64_t nelem, const canary_phys* phys, double* work, double* out,
                       void* stream);
+/* Courant-number step of the Euler sets: min over nodes of
+ * courant * dx / (|u| + c) / N with dx = sqrt(sum_d (1/(2 xi^d_d))^2), the
+ * form of the shallow-water example's courantnumber (swe2d.jl:100-124) with
+ * the Euler wave speed; no reference counterpart for Euler. */
+int canary_courant_device(int dim, int N, const double* vgeo, const double* Q, int64_t nelem,
+                          const canary_phys* phys, double courant, double* work, double* out,
+                          void* stream);
+/* The host initial states of canary_ic_bubble / canary_ic_vortex2d evaluated
+ * on the device from the coordinates in a device vgeo (same formulas; the
+ * device cos / pow / exp may differ from the host's in the last bit). */
+int canary_ic_bubble_device(int dim, int N, int64_t nelem, const double* vgeo,
+                            const canary_bubble* b, const canary_phys* ph, double* Q,
+                            void* stream);
+int canary_ic_vortex2d_device(int N, int64_t nelem, const double* vgeo, const canary_vortex* v,
+                              double t, double* Q, void* stream);
 
 #ifdef __cplusplus
 }

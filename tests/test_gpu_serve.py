@@ -8,9 +8,10 @@ import pytest
 import canary_amd.kernels as K
 import canary_amd.problems as P
 from canary_amd.perturb import Perturb32
-from canary_amd.serve import Request, Server, make_setup, poisson_stream, summary
+from canary_amd.serve import (Request, Server, make_setup_device, make_setup_host, poisson_stream,
+                              summary)
 
-from gpu_util import DeviceSetup, dev, host
+from gpu_util import host
 from states import max_ulp, rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -22,7 +23,7 @@ def small_stream(seed=7, n=8):
 
 
 def fp32_alone(s, dt, nsteps):
-    pf = Perturb32(s, s.Q0, s.Q)
+    pf = Perturb32(s, s.Q0d, s.Qd)
     pf.run(dt, nsteps)
     pf.check()
     return host(pf.full_state(), s.Q.shape)
@@ -30,22 +31,23 @@ def fp32_alone(s, dt, nsteps):
 
 def fp64_alone(s, dt, nsteps):
     A, B, _ = P.rk_coefs()
-    ds = DeviceSetup(s)
-    q, q2, rhs = dev(s.Q), dev(s.Q), dev(np.zeros_like(s.Q))
+    a = P.device_arrays(s)
+    q, q2, rhs = s.Qd.clone(), s.Qd.clone(), s.Qd.new_zeros(s.Qd.shape)
     for _ in range(nsteps):
         for st in range(5):
-            K.stage_fused(s.dim, s.N, rhs, q, q2, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM,
-                          ds.vmapP, ds.bndy, A[(st + 1) % 5], B[st], dt, eqn=s.eqn, phys=s.phys)
+            K.stage_fused(s.dim, s.N, rhs, q, q2, a["vgeo"], a["sgeo"], a["D"], s.nelem,
+                          a["vmapM"], a["vmapP"], a["bndy"], A[(st + 1) % 5], B[st], dt,
+                          eqn=s.eqn, phys=s.phys)
             q, q2 = q2, q
     K.check_errors()
-    return host(q, s.Q.shape)
+    return host(q, (s.Np, s.nstate, s.nelem))
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 def test_served_results_equal_alone(cuda, precision):
     reqs = small_stream()
-    setups = {r.rid: make_setup(r) for r in reqs}
-    sizes = sorted(Server(precision).footprint(s) for s, _ in setups.values())
+    hosts = {r.rid: make_setup_host(r) for r in reqs}
+    sizes = sorted(Server(precision).footprint(s) for s in hosts.values())
     budget = sizes[-1] + sizes[-2]  # at most about two of the largest resident at once
     srv = Server(precision, hbm_budget=budget, step_chunk=4)
     out = srv.serve(reqs)
@@ -57,7 +59,8 @@ def test_served_results_equal_alone(cuda, precision):
     assert max(res for _, _, res in srv.trace) <= max(budget, sizes[-1])
     alone = fp32_alone if precision == "fp32" else fp64_alone
     for r in out:
-        s, dt = setups[r.rid]
+        s, dt = make_setup_device(r, hosts[r.rid])
+        assert dt == r.dt
         want = alone(s, dt, r.nsteps)
         assert np.isfinite(want).all()
         assert max_ulp(r.result, want) == 0, "request %d (%s N=%d)" % (r.rid, r.kind, r.N)
@@ -69,10 +72,29 @@ def test_served_results_equal_alone(cuda, precision):
 
 def test_served_in_mix_equals_served_alone(cuda):
     reqs = small_stream(seed=11, n=6)
-    setups = {r.rid: make_setup(r) for r in reqs}
-    mixed = Server("fp32", step_chunk=3).serve(reqs, setups=setups)
+    hosts = {r.rid: make_setup_host(r) for r in reqs}
+    mixed = Server("fp32", step_chunk=3).serve(reqs, setups=hosts)
     for r in mixed:
         solo = Request(rid=r.rid, kind=r.kind, Ne=r.Ne, N=r.N, nsteps=r.nsteps, arrival=0.0,
                        seed=r.seed)
-        got = Server("fp32").serve([solo], setups=setups)[0]
+        got = Server("fp32").serve([solo], setups=hosts)[0]
         assert max_ulp(got.result, r.result) == 0
+
+
+@pytest.mark.parametrize("kind", ["bubble3d", "vortex2d"])
+def test_device_preparation_matches_host(cuda, kind):
+    """Geometry bit-identical; initial states within a few ulps (device libm);
+    the time-step reductions within a few ulps of the host formulas."""
+    r = Request(rid=0, kind=kind, Ne=3, N=4, nsteps=1, arrival=0.0, seed=5)
+    s, dt = make_setup_device(r, make_setup_host(r))
+    if kind == "bubble3d":
+        h = P.bubble3d(Ne=3, N=4, dtheta=2.0, jitter=1e-3, seed=5)
+        dth = P.cfl(h, h.Q) / h.N ** np.sqrt(2)
+    else:
+        h = P.vortex2d(Ne=3, N=4)
+        dth = P.euler_courant(h, h.Q)
+    assert max_ulp(host(s.dev["vgeo"], h.vgeo.shape), h.vgeo) == 0
+    assert max_ulp(host(s.dev["sgeo"], h.sgeo.shape), h.sgeo) == 0
+    Q, Q0 = host(s.Qd, h.Q.shape), host(s.Q0d, h.Q.shape)
+    assert rel_l2(Q, h.Q) <= 1e-15 and rel_l2(Q0, h.Q0) <= 1e-15
+    assert max_ulp(np.array([dt]), np.array([dth])) <= 8
