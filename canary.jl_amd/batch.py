@@ -99,6 +99,7 @@ class _BatchBase:
         self.groups = []          # [(N, epw, first item, item count)]
         self._table = None
         self._items = None
+        self.dirty = True         # solves added / removed since the tables were built
         self._graphs = {}
         self._streams = []        # OwnedStream per N class (forks of a captured stage)
         self._capture = None      # OwnedStream the step graphs are captured on
@@ -126,14 +127,14 @@ class _BatchBase:
         self._next_id += 1
         v.steps = 0
         self.solves[sid] = v
-        self._table = None
+        self.dirty = True
         return sid
 
     def remove(self, sid):
         """Drop a solve (its device arrays are freed); returns its final state."""
         Q = self.state(sid)
         del self.solves[sid]
-        self._table = None
+        self.dirty = True
         return Q
 
     # --------------------------------------------------------- work table
@@ -181,6 +182,14 @@ class _BatchBase:
         return np.concatenate(rows) if rows else np.zeros(0, dtype=ITEM_DTYPE)
 
     def _build_tables(self):
+        if self._table is not None or self._graphs:
+            # the step graphs and work tables about to be dropped may still be
+            # executing on the runtime's streams: destroying a graph (or
+            # handing its table memory back) under a running launch is a
+            # use after free
+            torch.cuda.synchronize(self.device)
+        self._graphs = {}
+        self.dirty = False
         self._table = self._solve_table()
         parts, self.groups, off = [], [], 0
         for N in sorted({v.N for v in self.solves.values()}):
@@ -206,7 +215,7 @@ class _BatchBase:
 
     def stage(self, rka, rkb, concurrent=False):
         """One LSRK stage of every solve: buffer [parity] -> [1 - parity]."""
-        if self._table is None:
+        if self.dirty:
             self.finalize()
         base = self._items.data_ptr()
         if concurrent and len(self.groups) > 1:
@@ -261,7 +270,7 @@ class _BatchBase:
     def run(self, nsteps, graph=True):
         """nsteps LSRK54 steps of every solve; graph=True replays one captured
         hipGraph per step (one per starting parity)."""
-        if self._table is None:
+        if self.dirty:
             self.finalize()
         for _ in range(nsteps):
             if graph:
@@ -303,7 +312,7 @@ class _BatchBase:
         """Time each elements-per-workgroup candidate per N on this batch's
         solves and keep the fastest.  The LSRK registers are restored after;
         buffer [parity] is only read."""
-        if self._table is None:
+        if self.dirty:
             self._build_tables()
         regs = self._registers()
         saved = {sid: r.clone() for sid, r in regs.items()}
@@ -367,7 +376,8 @@ class Batch(_BatchBase):
             raise _lib.CanaryError(_lib.ERR_UNSUPPORTED, "batched form at N=%d" % s.N)
         if Q is None:
             Q = s.Q if s.Q is not None else getattr(s, "Qd", None)
-        if Q is None or Q.size != s.Np * s.nstate * s.nelem:
+        nq = None if Q is None else (Q.numel() if isinstance(Q, torch.Tensor) else Q.size)
+        if nq != s.Np * s.nstate * s.nelem:
             raise ValueError("Q must hold (Np, nstate, nelem) values")
         v = _Solve()
         v.setup, v.dt, v.N, v.nelem = s, float(dt), s.N, s.nelem

@@ -188,7 +188,7 @@ class Server:
                     admitted = True
                 if admitted:
                     for dim, b in self.batches.items():
-                        if b.solves and b._table is None:
+                        if b.solves and b.dirty:
                             self._finalize(dim, b)
                 if not active:
                     if futures:
@@ -223,7 +223,7 @@ class Server:
                     done += 1
                 if fin:
                     for dim, b in self.batches.items():
-                        if b.solves and b._table is None:
+                        if b.solves and b.dirty:
                             self._finalize(dim, b)
             for b in self.batches.values():
                 b.check()

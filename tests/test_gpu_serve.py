@@ -26,7 +26,7 @@ def fp32_alone(s, dt, nsteps):
     pf = Perturb32(s, s.Q0d, s.Qd)
     pf.run(dt, nsteps)
     pf.check()
-    return host(pf.full_state(), s.Q.shape)
+    return host(pf.full_state(), (s.Np, s.nstate, s.nelem))
 
 
 def fp64_alone(s, dt, nsteps):
