@@ -12,8 +12,9 @@ A request names an equation set / problem, a mesh size, N and an end time
     resident solves stays within a budget and a slot is free,
   * keeps every admitted state resident on the GPU in one batch per
     dimension (Batch32: FP32 perturbation form; or Batch: plain FP64), so
-    the requests on the GPU advance together, one captured graph per step
-    and batch, the 2D and 3D batches on their own streams,
+    the requests on the GPU advance together, the 2D and 3D batches on their
+    own streams (launched stage by stage: replaying captured step graphs
+    inside this loop is switched off, see Server),
   * retires a request when it reaches its end time (its state is copied to
     the host) and admits waiting ones in the freed room.
 
@@ -107,7 +108,7 @@ def make_setup(req):
 
 class Server:
     def __init__(self, precision="fp32", hbm_budget=64 << 30, max_solves=256, step_chunk=8,
-                 prep_workers=8, graph=True, epw=None, keep_setups=False, device="cuda"):
+                 prep_workers=8, graph=False, epw=None, keep_setups=False, device="cuda"):
         if precision not in ("fp32", "fp64"):
             raise ValueError("precision is 'fp32' (perturbation form) or 'fp64'")
         self.precision = precision
@@ -115,7 +116,15 @@ class Server:
         self.max_solves = int(max_solves)
         self.step_chunk = int(step_chunk)
         self.prep_workers = int(prep_workers)
-        self.graph = graph
+        if graph:
+            # Replaying step graphs inside the serving loop (batches whose
+            # graphs are re-captured as requests join and leave, worker
+            # threads preparing meshes meanwhile) ended twice in a host
+            # segmentation fault inside hipGraphLaunch; its cause is not
+            # established, so this path stays off.  Batches outside the loop
+            # (Batch.run) replay graphs normally.
+            raise NotImplementedError("graph replay in the serving loop is switched off")
+        self.graph = False
         self.keep_setups = keep_setups
         self.device = torch.device(device)
         self.epw = dict(epw or {})   # (dim, N) -> elements per workgroup

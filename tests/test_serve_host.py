@@ -1,8 +1,9 @@
 """Host side of the served form: the seeded request stream and the latency
 summary (no GPU)."""
 import numpy as np
+import pytest
 
-from canary_amd.serve import Request, poisson_stream, summary
+from canary_amd.serve import Request, Server, poisson_stream, summary
 
 
 def test_stream_is_seeded_and_in_range():
@@ -39,3 +40,8 @@ def test_summary_percentiles():
     assert abs(s["p50_latency_s"] - np.percentile([(i + 1) * 0.01 for i in range(100)], 50)) < 1e-12
     assert abs(s["p99_latency_s"] - np.percentile([(i + 1) * 0.01 for i in range(100)], 99)) < 1e-12
     assert abs(s["aggregate_gdof_stage_s"] - 100 * 5 * 1000 * 10 / 200.0 / 1e9) < 1e-18
+
+
+def test_graph_replay_in_serving_loop_is_off():
+    with pytest.raises(NotImplementedError):
+        Server("fp32", graph=True)
