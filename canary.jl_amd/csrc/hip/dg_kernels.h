@@ -126,6 +126,23 @@ __device__ __forceinline__ void volume_sweeps(Real* r, const Real* sD, const Rea
   }
 }
 
+// r[s] += sum_m D[m, i_dir] F[..m..][s] along direction DIR only (same per-state
+// accumulation order as volume_sweeps when called for DIR = 0, 1, 2 in turn).
+template <int DIM, int NQ, int NSR, class Real, int DIR>
+__device__ __forceinline__ void volume_sweep_dir(Real* r, const Real* sD, const Real* sF, int n) {
+  constexpr int Np = ipow(NQ, DIM);
+  constexpr int STRIDE = DIR == 0 ? 1 : (DIR == 1 ? NQ : NQ * NQ);
+  const int idx = DIR == 0 ? n % NQ : (DIR == 1 ? (n / NQ) % NQ : n / (NQ * NQ));
+  const int base = n - idx * STRIDE;
+#pragma unroll
+  for (int s = 0; s < NSR; ++s) {
+    Real a = r[s];
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) a = a + sD[m + NQ * idx] * sF[s * Np + base + m * STRIDE];
+    r[s] = a;
+  }
+}
+
 // ------------------------------------------------------------ face terms
 // A face item is one face node `fi` (= f*Nfp + fn) of element e.  Its work
 // is split in three steps so that a kernel can issue the loads early and
@@ -414,7 +431,7 @@ struct FusedCfg {
   static constexpr int NVOL = round64(EPW * S::Np);
   static constexpr int NFACEIT = round64(EPW * S::NFN);
   static constexpr int NT = WIDE && NFACEIT > NVOL ? NFACEIT : NVOL;
-  static constexpr int FLUX = Phys::DIM * S::Np;           // per state
+  static constexpr int FLUX = S::Np;                       // per state, one direction
   static constexpr int POST = S::NFN + S::Np;               // sC + sR per state
   static constexpr int BUF = (FLUX > POST ? FLUX : POST) * Phys::NSR;
   // LDS carve-up (bytes, each piece 16-byte aligned)
@@ -476,6 +493,7 @@ __device__ __forceinline__ void stage_body(
   if (t < NQ * NQ) sD[t] = (Real)D[t];
 
   Real r[NSR], MJI = 0;
+  Real Fv[DIM][NSR];
   if (valid) {
     Real q[NS];
     const Real* qe = Qin + (size_t)NS * Np * (size_t)e;
@@ -494,24 +512,28 @@ __device__ __forceinline__ void stage_body(
     for (int s = 0; s < NS; ++s) sQ[slot * NS * Np + s * Np + n] = q[s];
     const Real P = Phys::HAS_P ? Phys::pressure(q, p) : Real(0);
     if (SP) sP[slot * Np + n] = P;
-    Real F[DIM][NSR];
-    Phys::volume_flux(q, P, met, MJ, p, F);
-#pragma unroll
-    for (int d = 0; d < DIM; ++d)
-#pragma unroll
-      for (int s = 0; s < NSR; ++s) sB[slot * BUF + (d * NSR + s) * Np + n] = F[d][s];
+    Phys::volume_flux(q, P, met, MJ, p, Fv);
     Phys::source(q, MJ, p, r);
   }
   // 2. P-side gathers (stay in flight over the sweeps)
 #pragma unroll
   for (int k = 0; k < KP; ++k)
     if (fval[k]) face_gather<Phys, NQ>(fit[k], Qin);
-  lds_barrier();
-  // 3. sweeps
-  if (valid) {
-    const Real* b = sB + slot * BUF;
-    volume_sweeps<DIM, NQ, NSR>(r, sD, b, b + (DIM > 1 ? NSR * Np : 0),
-                                b + (DIM > 2 ? 2 * NSR * Np : 0), n);
+  // 3. sweeps, one direction at a time through one flux buffer
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    if (d > 0) lds_barrier();  // the previous direction's fluxes are consumed
+    if (valid) {
+#pragma unroll
+      for (int s = 0; s < NSR; ++s) sB[slot * BUF + s * Np + n] = Fv[d][s];
+    }
+    lds_barrier();
+    if (valid) {
+      const Real* b = sB + slot * BUF;
+      if (d == 0) volume_sweep_dir<DIM, NQ, NSR, Real, 0>(r, sD, b, n);
+      if (d == 1) volume_sweep_dir<DIM, NQ, NSR, Real, (DIM > 1 ? 1 : 0)>(r, sD, b, n);
+      if (d == 2) volume_sweep_dir<DIM, NQ, NSR, Real, (DIM > 2 ? 2 : 0)>(r, sD, b, n);
+    }
   }
   lds_barrier();  // fluxes consumed; sB is reused below
   if (valid) {
