@@ -97,6 +97,7 @@ def main():
         raise SystemExit("libcanary_dg.so is not built; run python canary.jl_amd/build.py")
     import canary_amd.kernels as K
     import canary_amd.problems as P
+    from canary_amd._lib import FP64, FP64_FMA
 
     t0 = time.time()
     s = P.bubble3d(Ne=args.ne, N=args.N, gravity=9.81, dtheta=2.0, jitter=1e-3, seed=1)
@@ -115,11 +116,11 @@ def main():
     dt = P.cfl(s, s.Q) / s.N ** np.sqrt(2)
     stream = torch.cuda.current_stream()
 
-    def run_fused(q, q2, rhs, nsteps):
+    def run_fused(q, q2, rhs, nsteps, prec=FP64_FMA):
         for _ in range(nsteps):
             for st in range(5):
                 K.stage_fused(3, s.N, rhs, q, q2, vgeo, sgeo, D, s.nelem, vM, vP, bnd,
-                              A[(st + 1) % 5], B[st], dt, phys=s.phys)
+                              A[(st + 1) % 5], B[st], dt, phys=s.phys, prec=prec)
                 q, q2 = q2, q
         return q, q2
 
@@ -157,7 +158,7 @@ def main():
     copy_gbs = 2 * (1 << 30) * 10 / t_copy / 1e9
     del csrc, cdst
 
-    # fused path (the measured value)
+    # fused path in contracted FP64 (the measured value)
     q, q2 = Q0.clone(), torch.empty_like(Q0)
     rhs = torch.zeros_like(Q0)
     q, q2 = run_fused(q, q2, rhs, args.warmup)
@@ -180,6 +181,24 @@ def main():
     achieved = bytes_launch / launch_s / 1e9
 
     extra = {}
+    # the same stages in the reference evaluation order (bit-identical to the
+    # three-kernel path), from the same initial state: its rate, and how far
+    # the contracted state is from it after warmup + steps LSRK54 steps
+    qe, qe2, re = Q0.clone(), torch.empty_like(Q0), torch.zeros_like(Q0)
+    qe, qe2 = run_fused(qe, qe2, re, args.warmup, FP64)
+    hold_e = {}
+
+    def go_exact():
+        hold_e["q"] = run_fused(qe, qe2, re, args.steps, FP64)
+
+    t_exact, _ = timed(go_exact)
+    K.check_errors()
+    qx = hold_e["q"][0]
+    extra["fused_exact_order_gdof_stage_s"] = round(dof * stages / t_exact / 1e9, 3)
+    extra["fused_exact_order_ms_per_stage"] = round(t_exact / stages * 1e3, 4)
+    extra["fma_vs_exact_rel_l2_after_%d_steps" % (args.warmup + args.steps)] = float(
+        "%.3e" % (torch.linalg.vector_norm(qf - qx) / torch.linalg.vector_norm(qx)).item())
+    del qe, qe2, re, hold_e, qx
     if not args.no_plain:
         qp, rp = Q0.clone(), torch.zeros_like(Q0)
         run_plain(qp, rp, 1)
@@ -206,7 +225,8 @@ def main():
 
     traffic = measured_traffic("euler3d_N%d_%d" % (s.N, args.ne))
     line = {
-        "metric": "GDOF-stage/s 3D Euler N=4 FP64 (LSRK54 stages, fused single-pass kernel)",
+        "metric": "GDOF-stage/s 3D Euler N=4 FP64 (LSRK54 stages, fused single-pass kernel, "
+                  "multiply-adds fused)",
         "value": round(value, 3),
         "unit": "GDOF-stage/s",
         "n_gpus": 1,
@@ -226,7 +246,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "stage_kernel<Euler3D,5,1> (fused volume+face+update)",
+                     "kernel": "stage_kernel<Euler3DFma,5,1> (fused volume+face+update)",
                      "bytes_per_element_stage": algorithmic_bytes(3, s.N),
                      "bytes_per_element_stage_without_vmapM": algorithmic_bytes(3, s.N)
                      - 6 * (s.N + 1) ** 2 * 8,
@@ -248,6 +268,7 @@ def precision_form(ne, N, steps_csv, timed_steps=10):
     import torch
     import canary_amd.kernels as K
     import canary_amd.problems as P
+    from canary_amd._lib import FP64, FP64_FMA
     from canary_amd.perturb import Perturb32
 
     t0 = time.time()
@@ -267,12 +288,21 @@ def precision_form(ne, N, steps_csv, timed_steps=10):
     stream = torch.cuda.current_stream()
     st = {"q": Qinit.clone(), "q2": torch.empty_like(Qinit), "rhs": torch.zeros_like(Qinit)}
 
-    def run64(nsteps):
+    def run64(nsteps, prec=FP64):
         for _ in range(nsteps):
             for k in range(5):
                 K.stage_fused(3, N, st["rhs"], st["q"], st["q2"], vgeo, sgeo, D, s.nelem, vM, vP,
-                              bnd, A[(k + 1) % 5], B[k], dt, phys=s.phys)
+                              bnd, A[(k + 1) % 5], B[k], dt, phys=s.phys, prec=prec)
                 st["q"], st["q2"] = st["q2"], st["q"]
+
+    def time_fma():
+        # the contracted FP64 stage on a scratch copy (the error reference
+        # state st stays on the exact-order path)
+        saved = {k: v.clone() for k, v in st.items()}
+        t = timed(lambda: run64(timed_steps, FP64_FMA)) / (5 * timed_steps)
+        for k in st:
+            st[k].copy_(saved[k])
+        return t
 
     pf = Perturb32(s, Q0, s.Q)
 
@@ -292,6 +322,7 @@ def precision_form(ne, N, steps_csv, timed_steps=10):
     for m in marks:
         n = m - done
         if t64 is None and n >= timed_steps:
+            t64f = time_fma()
             t64 = timed(lambda: run64(timed_steps)) / (5 * timed_steps)
             t32 = timed(lambda: pf.run(dt, timed_steps)) / (5 * timed_steps)
             n -= timed_steps
@@ -307,6 +338,7 @@ def precision_form(ne, N, steps_csv, timed_steps=10):
     if t64 is None:
         run64(1)
         pf.run(dt, 1)
+        t64f = time_fma()
         t64 = timed(lambda: run64(timed_steps)) / (5 * timed_steps)
         t32 = timed(lambda: pf.run(dt, timed_steps)) / (5 * timed_steps)
     dof = s.nelem * s.Np * 5
@@ -314,7 +346,9 @@ def precision_form(ne, N, steps_csv, timed_steps=10):
     return {"workload": "3D bubble %d^3 hexes N=%d, FP32 perturbation about an FP64 hydrostatic "
                         "background vs FP64 fused stage" % (ne, N),
             "fp64_ms_per_stage": round(t64 * 1e3, 4), "fp32_ms_per_stage": round(t32 * 1e3, 4),
+            "fp64_fma_ms_per_stage": round(t64f * 1e3, 4),
             "speedup_fp32_vs_fp64": round(t64 / t32, 3),
+            "speedup_fp32_vs_fp64_fma": round(t64f / t32, 3),
             "fp32_gdof_stage_s": round(dof / t32 / 1e9, 3),
             "fp64_gdof_stage_s": round(dof / t64 / 1e9, 3),
             "fp32_bytes_per_element_stage": b32,

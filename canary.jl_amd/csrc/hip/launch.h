@@ -34,64 +34,76 @@ inline int launch_status() {
   return err == hipSuccess ? 0 : 3 /*CANARY_ERR_LAUNCH*/;
 }
 
-template <class Phys, int NQ>
+template <class Phys, int NQ, int EPW>
+void launch_stage(int64_t nb, const KArgs& a) {
+  // measured on 3D N=4 (32^3): one prefetched face item per thread, the
+  // nodal pressure cached in LDS, workgroup sized by the volume nodes
+  constexpr bool WIDE = false;
+  constexpr int PF = 1;
+  constexpr bool PM_LDS = true;
+  constexpr int NT = FusedCfg<Phys, NQ, EPW, WIDE>::NT;
+  const int64_t chunk = (nb + 7) / 8;
+  stage_kernel<Phys, NQ, EPW, WIDE, PF, PM_LDS><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
+      (double*)a.rhs, (const double*)a.Q, (double*)a.Qout, (const double*)a.vgeo,
+      (const double*)a.sgeo, a.D, a.nreal, a.ntotal, a.vmapM, a.vmapP, a.bndy, a.rka, a.rkb,
+      a.dt, chunk, a.phys);
+}
+
+// STAGE_ONLY: the fused stage alone is instantiated (the contracted-FP64
+// units); the other operations report CANARY_ERR_UNSUPPORTED.
+template <class Phys, int NQ, bool STAGE_ONLY>
 int launch(int op, const KArgs& a) {
   using S = Shape<Phys, NQ>;
   constexpr int EPW = default_epw<S::Np>();
   if (a.nreal <= 0) return 0;
   const int64_t nb = (a.nreal + EPW - 1) / EPW;
-  double* rhs = (double*)a.rhs;
-  const double* Q = (const double*)a.Q;
-  switch (op) {
-    case OP_VOLUME: {
-      constexpr int NT = VolumeCfg<Phys, NQ, EPW>::NT;
-      volume_kernel<Phys, NQ, EPW><<<dim3((unsigned)nb), dim3(NT), 0, a.stream>>>(
-          rhs, Q, (const double*)a.vgeo, a.D, a.nreal, a.phys);
-      break;
+  if constexpr (STAGE_ONLY) {
+    if (op != OP_STAGE) return 2 /*CANARY_ERR_UNSUPPORTED*/;
+    launch_stage<Phys, NQ, EPW>(nb, a);
+  } else {
+    double* rhs = (double*)a.rhs;
+    const double* Q = (const double*)a.Q;
+    switch (op) {
+      case OP_VOLUME: {
+        constexpr int NT = VolumeCfg<Phys, NQ, EPW>::NT;
+        volume_kernel<Phys, NQ, EPW><<<dim3((unsigned)nb), dim3(NT), 0, a.stream>>>(
+            rhs, Q, (const double*)a.vgeo, a.D, a.nreal, a.phys);
+        break;
+      }
+      case OP_FACE: {
+        constexpr int NT = FaceCfg<Phys, NQ, EPW>::NT;
+        face_kernel<Phys, NQ, EPW><<<dim3((unsigned)nb), dim3(NT), 0, a.stream>>>(
+            rhs, Q, (const double*)a.sgeo, a.nreal, a.ntotal, a.vmapM, a.vmapP, a.bndy, a.phys);
+        break;
+      }
+      case OP_UPDATE: {
+        const int64_t total = a.nreal * S::Np;
+        int64_t blocks = (total + 255) / 256;
+        if (blocks > 65536) blocks = 65536;
+        update_kernel<Phys, NQ><<<dim3((unsigned)blocks), dim3(256), 0, a.stream>>>(
+            rhs, (double*)a.Qout, (const double*)a.vgeo, a.nreal, a.rka, a.rkb, a.dt, a.advection);
+        break;
+      }
+      case OP_STAGE:
+        launch_stage<Phys, NQ, EPW>(nb, a);
+        break;
+      default:
+        return 1 /*CANARY_ERR_ARG*/;
     }
-    case OP_FACE: {
-      constexpr int NT = FaceCfg<Phys, NQ, EPW>::NT;
-      face_kernel<Phys, NQ, EPW><<<dim3((unsigned)nb), dim3(NT), 0, a.stream>>>(
-          rhs, Q, (const double*)a.sgeo, a.nreal, a.ntotal, a.vmapM, a.vmapP, a.bndy, a.phys);
-      break;
-    }
-    case OP_UPDATE: {
-      const int64_t total = a.nreal * S::Np;
-      int64_t blocks = (total + 255) / 256;
-      if (blocks > 65536) blocks = 65536;
-      update_kernel<Phys, NQ><<<dim3((unsigned)blocks), dim3(256), 0, a.stream>>>(
-          rhs, (double*)a.Qout, (const double*)a.vgeo, a.nreal, a.rka, a.rkb, a.dt, a.advection);
-      break;
-    }
-    case OP_STAGE: {
-      // measured on 3D N=4 (32^3): one prefetched face item per thread, the
-      // nodal pressure cached in LDS, workgroup sized by the volume nodes
-      constexpr bool WIDE = false;
-      constexpr int PF = 1;
-      constexpr bool PM_LDS = true;
-      constexpr int NT = FusedCfg<Phys, NQ, EPW, WIDE>::NT;
-      const int64_t chunk = (nb + 7) / 8;
-      stage_kernel<Phys, NQ, EPW, WIDE, PF, PM_LDS><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
-          rhs, Q, (double*)a.Qout, (const double*)a.vgeo, (const double*)a.sgeo, a.D, a.nreal,
-          a.ntotal, a.vmapM, a.vmapP, a.bndy, a.rka, a.rkb, a.dt, chunk, a.phys);
-      break;
-    }
-    default:
-      return 1 /*CANARY_ERR_ARG*/;
   }
   return launch_status();
 }
 
-template <class Phys>
+template <class Phys, bool STAGE_ONLY = false>
 int dispatch(int op, int N, const KArgs& a) {
   switch (N) {
-    case 1: return launch<Phys, 2>(op, a);
-    case 2: return launch<Phys, 3>(op, a);
-    case 3: return launch<Phys, 4>(op, a);
-    case 4: return launch<Phys, 5>(op, a);
-    case 5: return launch<Phys, 6>(op, a);
-    case 6: return launch<Phys, 7>(op, a);
-    case 7: return launch<Phys, 8>(op, a);
+    case 1: return launch<Phys, 2, STAGE_ONLY>(op, a);
+    case 2: return launch<Phys, 3, STAGE_ONLY>(op, a);
+    case 3: return launch<Phys, 4, STAGE_ONLY>(op, a);
+    case 4: return launch<Phys, 5, STAGE_ONLY>(op, a);
+    case 5: return launch<Phys, 6, STAGE_ONLY>(op, a);
+    case 6: return launch<Phys, 7, STAGE_ONLY>(op, a);
+    case 7: return launch<Phys, 8, STAGE_ONLY>(op, a);
   }
   return 2 /*CANARY_ERR_UNSUPPORTED*/;
 }
@@ -112,6 +124,16 @@ inline int read_status(int64_t* info, int clear) {
 }
 
 }  // namespace canary
+
+// The contracted-FP64 unit of an equation set: the fused stage only, built
+// with -ffp-contract=fast from a physics policy whose pressure avoids pow.
+#define CANARY_DEFINE_FMA_STAGE(NAME, PHYS)                                       \
+  extern "C" int canary_k_##NAME##_fma(int op, int N, const canary::KArgs* a) {  \
+    return canary::dispatch<PHYS, true>(op, N, *a);                               \
+  }                                                                               \
+  extern "C" int canary_k_##NAME##_fma_status(int64_t* info, int clear) {        \
+    return canary::read_status(info, clear);                                      \
+  }
 
 #define CANARY_DEFINE_EQUATION(NAME, PHYS)                                        \
   extern "C" int canary_k_##NAME(int op, int N, const canary::KArgs* a) {        \

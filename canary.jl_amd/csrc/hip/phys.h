@@ -1,8 +1,9 @@
 // Physics policies of the DG kernels: fluxes, sources and the Rusanov face
 // flux of the set2c compressible Euler equations (2D/3D) and of the
 // shallow-water equations, in FP64 with the reference's evaluation order.
-// The translation units that include this header are compiled with
-// -ffp-contract=off, so a*b+c stays two correctly rounded operations.
+// The exact-order units (euler3d.hip, euler2d.hip, swe2d.hip, batched.hip)
+// are compiled with -ffp-contract=off, so a*b+c stays two correctly rounded
+// operations; the *_fma.hip units use the contracted policies below.
 //
 // Reference formulas (same association, left to right):
 //   3D Euler volume  examples/3d_kernels/euler3d_set2c.jl:175-215
@@ -150,6 +151,28 @@ struct Euler2D {
     out[U] = sMJ * ((nx * (fUMx + fUPx) + ny * (fUMy + fUPy) + -(l * (UP - UM))) / 2);
     out[V] = sMJ * ((nx * (fVMx + fVPx) + ny * (fVMy + fVPy) + -(l * (VP - VM))) / 2);
     out[E] = sMJ * ((nx * (fEMx + fEPx) + ny * (fEMy + fEPy) + -(l * (EP - EM))) / 2);
+  }
+};
+
+// ------------------------------------------- Euler with contracted FP64 (FMA)
+// The same equations for the kernels built with -ffp-contract=fast (the
+// CANARY_FP64_FMA stage): products are fused into the adds that follow them
+// and the pressure is exp(kappa * log x) instead of pow(x, kappa).  Both cut
+// FP64 instructions; the results leave the reference's bit pattern but stay
+// within a few ulp per operation (log and exp are each within an ulp, and
+// for the x = R rho theta / p0 of an atmosphere, |kappa log x| < 1, so the
+// product adds no more than an ulp of its own).
+__device__ __forceinline__ double pow_explog(double x, double k) { return exp(k * log(x)); }
+
+struct Euler3DFma : Euler3D {
+  __device__ static double pressure(const double* q, const PhysParams& p) {
+    return p.p0 * pow_explog(p.R * q[E] / p.p0, p.kappa);
+  }
+};
+
+struct Euler2DFma : Euler2D {
+  __device__ static double pressure(const double* q, const PhysParams& p) {
+    return p.p0 * pow_explog(p.R * q[E] / p.p0, p.kappa);
   }
 };
 

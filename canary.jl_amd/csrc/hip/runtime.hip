@@ -10,9 +10,15 @@ extern "C" {
 int canary_k_euler3d(int op, int N, const canary::KArgs* a);
 int canary_k_euler2d(int op, int N, const canary::KArgs* a);
 int canary_k_swe2d(int op, int N, const canary::KArgs* a);
+int canary_k_euler3d_fma(int op, int N, const canary::KArgs* a);
+int canary_k_euler2d_fma(int op, int N, const canary::KArgs* a);
+int canary_k_swe2d_fma(int op, int N, const canary::KArgs* a);
 int canary_k_euler3d_status(int64_t* info, int clear);
 int canary_k_euler2d_status(int64_t* info, int clear);
 int canary_k_swe2d_status(int64_t* info, int clear);
+int canary_k_euler3d_fma_status(int64_t* info, int clear);
+int canary_k_euler2d_fma_status(int64_t* info, int clear);
+int canary_k_swe2d_fma_status(int64_t* info, int clear);
 int canary_k_perturb32_status(int64_t* info, int clear);
 int canary_k_batch_status(int64_t* info, int clear);
 int canary_k_geometry_status(int64_t* info, int clear);
@@ -23,10 +29,14 @@ namespace {
 
 using KFn = int (*)(int, int, const canary::KArgs*);
 
-KFn select(int eqn, int dim) {
-  if (eqn == CANARY_EULER && dim == 3) return canary_k_euler3d;
-  if (eqn == CANARY_EULER && dim == 2) return canary_k_euler2d;
-  if (eqn == CANARY_SWE && dim == 2) return canary_k_swe2d;
+// CANARY_FP64 selects the exact-order units, CANARY_FP64_FMA the contracted
+// ones (which hold the fused stage only).
+KFn select(int eqn, int dim, int prec) {
+  const bool fma = prec == CANARY_FP64_FMA;
+  if (prec != CANARY_FP64 && !fma) return nullptr;
+  if (eqn == CANARY_EULER && dim == 3) return fma ? canary_k_euler3d_fma : canary_k_euler3d;
+  if (eqn == CANARY_EULER && dim == 2) return fma ? canary_k_euler2d_fma : canary_k_euler2d;
+  if (eqn == CANARY_SWE && dim == 2) return fma ? canary_k_swe2d_fma : canary_k_swe2d;
   return nullptr;
 }
 
@@ -49,11 +59,13 @@ canary::PhysParams to_params(const canary_phys* ph) {
   return p;
 }
 
-int check_common(int eqn, int dim, int N, int prec, int64_t nreal, KFn* fn) {
-  *fn = select(eqn, dim);
+// stage_only: CANARY_FP64_FMA is accepted (canary_stage_fused only).
+int check_common(int eqn, int dim, int N, int prec, int64_t nreal, KFn* fn,
+                 bool stage_only = false) {
+  if (prec == CANARY_FP64_FMA && !stage_only) return CANARY_ERR_UNSUPPORTED;
+  *fn = select(eqn, dim, prec);
   if (!*fn) return CANARY_ERR_UNSUPPORTED;
   if (N < 1 || N > canary::kMaxN) return CANARY_ERR_UNSUPPORTED;
-  if (prec != CANARY_FP64) return CANARY_ERR_UNSUPPORTED;
   if (nreal < 0) return CANARY_ERR_ARG;
   return 0;
 }
@@ -135,7 +147,7 @@ extern "C" int canary_stage_fused(int eqn, int dim, int N, int prec, void* rhs, 
                                   const int64_t* elemtobndy, double rka, double rkb, double dt,
                                   const canary_phys* phys, void* stream) {
   KFn fn;
-  int rc = check_common(eqn, dim, N, prec, nreal, &fn);
+  int rc = check_common(eqn, dim, N, prec, nreal, &fn, true);
   if (rc) return rc;
   if (nreal == 0) return 0;
   if (nelem_total < nreal) return CANARY_ERR_ARG;
@@ -166,10 +178,11 @@ extern "C" int canary_check_errors(void* stream, int64_t* info) {
   if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return CANARY_ERR_LAUNCH;
   int64_t tmp[4];
   int first = 0;
-  int (*readers[7])(int64_t*, int) = {canary_k_euler3d_status,   canary_k_euler2d_status,
-                                      canary_k_swe2d_status,     canary_k_perturb32_status,
-                                      canary_k_batch_status,     canary_k_geometry_status,
-                                      canary_k_aux_status};
+  int (*readers[10])(int64_t*, int) = {
+      canary_k_euler3d_status,     canary_k_euler2d_status,     canary_k_swe2d_status,
+      canary_k_euler3d_fma_status, canary_k_euler2d_fma_status, canary_k_swe2d_fma_status,
+      canary_k_perturb32_status,   canary_k_batch_status,       canary_k_geometry_status,
+      canary_k_aux_status};
   for (auto rd : readers) {
     int rc = rd(tmp, 1);
     if (rc && !first) {

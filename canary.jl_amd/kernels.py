@@ -15,7 +15,7 @@ import ctypes as C
 import torch
 
 from . import _lib
-from ._lib import EULER, SWE, FP64, check, lib
+from ._lib import EULER, SWE, FP64, FP64_FMA, check, lib
 
 _NS = {(EULER, 3): 5, (EULER, 2): 4, (SWE, 2): 4}
 _NV = {3: 14, 2: 8}
@@ -106,8 +106,15 @@ def updatesolution(dim, N, rhs, Q, vgeo, elems, rka, rkb, dt, eqn=EULER, advecti
 
 
 def stage_fused(dim, N, rhs, Qin, Qout, vgeo, sgeo, D, elems, vmapM, vmapP, elemtobndy,
-                rka, rkb, dt, eqn=EULER, phys=None, nelem_total=None, stream=None):
-    """One LSRK stage (volume + faces + update) in one pass; Qin -> Qout."""
+                rka, rkb, dt, eqn=EULER, phys=None, nelem_total=None, stream=None,
+                prec=FP64):
+    """One LSRK stage (volume + faces + update) in one pass; Qin -> Qout.
+
+    prec: FP64 (the reference evaluation order, bit-identical to volumerhs +
+    facerhs + updatesolution) or FP64_FMA (multiply-adds fused, pressure via
+    exp/log; within 1e-12 relative L2 per stage, faster)."""
+    if prec not in (FP64, FP64_FMA):
+        raise ValueError("stage_fused: prec must be FP64 or FP64_FMA")
     n = _nreal(elems)
     Np, Nfp, nf = (N + 1) ** dim, (N + 1) ** (dim - 1), 2 * dim
     ns = _NS[(eqn, dim)]
@@ -123,7 +130,7 @@ def stage_fused(dim, N, rhs, Qin, Qout, vgeo, sgeo, D, elems, vmapM, vmapP, elem
     _need(elemtobndy, "elemtobndy", nf * n, torch.int64)
     if Qin.data_ptr() == Qout.data_ptr():
         raise ValueError("stage_fused needs distinct Qin and Qout")
-    rc = lib().canary_stage_fused(eqn, dim, N, FP64, _ptr(rhs), _ptr(Qin), _ptr(Qout),
+    rc = lib().canary_stage_fused(eqn, dim, N, prec, _ptr(rhs), _ptr(Qin), _ptr(Qout),
                                   _ptr(vgeo), _ptr(sgeo), _ptr(D), n, ntot, _ptr(vmapM),
                                   _ptr(vmapP), _ptr(elemtobndy), float(rka), float(rkb),
                                   float(dt), _phys(phys), _stream(stream))

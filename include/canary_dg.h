@@ -47,6 +47,11 @@ extern "C" {
 #define CANARY_SWE 1     /* (H u, H v, h, b), b read only                    */
 #define CANARY_FP64 0
 #define CANARY_FP32 1
+/* FP64 with a*b+c contracted into fused multiply-adds and the Euler pressure
+ * evaluated as exp(kappa log x): fewer FP64 instructions, not the reference's
+ * bit pattern (within 1e-12 relative L2 per stage of it).  Accepted by
+ * canary_stage_fused only; CANARY_FP64 is the exact reference order. */
+#define CANARY_FP64_FMA 2
 
 /* Physical constants passed at run time (reference defaults in
  * canary_phys_default: examples/3d_kernels/euler3d_set2c.jl:58-63 and
@@ -110,7 +115,9 @@ int canary_updatesolution(int eqn, int dim, int N, int prec, void* rhs, void* Q,
  *   Qin and the updated state is written to Qout (Qin != Qout; the caller
  *   swaps them each stage), so neighbouring elements never read a Q that
  *   is being overwritten.  Ghost elements (nreal..nelem_total-1) of Qin are
- *   read, never written.  rhs is updated in place. */
+ *   read, never written.  rhs is updated in place.
+ *   prec: CANARY_FP64 (bit-identical to the three calls above) or
+ *   CANARY_FP64_FMA (contracted; the faster form). */
 int canary_stage_fused(int eqn, int dim, int N, int prec, void* rhs, const void* Qin,
                        void* Qout, const void* vgeo, const void* sgeo,
                        const double* D, int64_t nreal, int64_t nelem_total,

@@ -49,6 +49,16 @@ def test_device_entry_points_reject_bad_arguments_before_launch():
     assert lib.canary_facerhs(0, 3, 4, 0, None, None, None, 4, 2, None, None, None, ctypes.byref(p), None) == L.ERR_ARG
     # zero elements is a no-op
     assert lib.canary_updatesolution(0, 2, 3, 0, None, None, None, 0, 0.0, 0.0, 0.0, 0, None) == 0
+    # the contracted FP64 form exists for the fused stage only
+    assert lib.canary_volumerhs(0, 3, 4, L.FP64_FMA, None, None, None, None, 1, ctypes.byref(p), None) == L.ERR_UNSUPPORTED
+    assert lib.canary_updatesolution(0, 2, 3, L.FP64_FMA, None, None, None, 1, 0.0, 0.0, 0.0, 0, None) == L.ERR_UNSUPPORTED
+    for eqn, dim in ((0, 3), (0, 2), (1, 2)):
+        assert lib.canary_stage_fused(eqn, dim, 4, L.FP64_FMA, None, None, None, None, None, None, 0, 0,
+                                      None, None, None, 0.0, 0.0, 0.0, ctypes.byref(p), None) == 0
+        assert lib.canary_stage_fused(eqn, dim, 4, L.FP64_FMA, None, None, None, None, None, None, 1, 1,
+                                      None, None, None, 0.0, 0.0, 0.0, ctypes.byref(p), None) == L.ERR_ARG
+    assert lib.canary_stage_fused(0, 3, 8, L.FP64_FMA, None, None, None, None, None, None, 1, 1,
+                                  None, None, None, 0.0, 0.0, 0.0, ctypes.byref(p), None) == L.ERR_UNSUPPORTED
 
 
 def test_phys_defaults_are_the_reference_rationals():

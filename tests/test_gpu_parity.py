@@ -6,7 +6,10 @@ Bar: the exact-order FP64 kernels differ from the oracle only through pow()
     and for the shallow-water equations (no pow), results are bit-identical;
   * on generic states the relative L2 difference per RHS is <= 1e-12 and,
     after 100 LSRK54 steps, <= 1e-10 (the tolerances the task states);
-  * the fused single-pass stage is bit-identical to volume + face + update.
+  * the fused single-pass stage is bit-identical to volume + face + update;
+  * the contracted fused stage (FP64_FMA: multiply-adds fused, pressure as
+    exp(kappa log x)) is within 1e-12 relative L2 of the oracle per RHS and
+    1e-10 after 100 LSRK54 steps.
 """
 import numpy as np
 import pytest
@@ -14,7 +17,7 @@ import torch
 
 import canary_amd.kernels as K
 import canary_amd.problems as P
-from canary_amd._lib import CanaryError, ERR_BC, ERR_VMAP
+from canary_amd._lib import CanaryError, ERR_BC, ERR_VMAP, FP64, FP64_FMA
 from oracle import oracle as O
 
 from gpu_util import DeviceSetup, dev, host
@@ -141,17 +144,46 @@ def test_fused_stage_equals_three_kernels(cuda, eqn, dim, N):
     assert torch.equal(q1, qout) and torch.equal(r1, r2)
 
 
+@pytest.mark.parametrize("eqn,dim,N", CASES)
+def test_fma_stage_parity(cuda, eqn, dim, N):
+    """Contracted stage, rka = 1 and rhs_in = 0: rhs_out is the RHS L(Q) and
+    Q_out - Q = rkb dt L(Q) / MJ; both within 1e-12 of the oracle."""
+    print("fma case eqn=%d dim=%d N=%d" % (eqn, dim, N), flush=True)
+    periodic = (True,) * dim if N == 3 else None
+    s = case_setup(eqn, dim, N, periodic)
+    ds = DeviceSetup(s)
+    for powfree in ((False, True) if eqn == 0 else (False,)):
+        Q = np.asfortranarray(state(s, seed=17 + N, powfree=powfree))
+        want = O.rhs_eval(s, Q, s.zeros_state())
+        qin, qout, r = dev(Q), dev(np.zeros_like(Q)), dev(s.zeros_state())
+        K.stage_fused(dim, N, r, qin, qout, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM, ds.vmapP,
+                      ds.bndy, 1.0, 0.5, 0.01, eqn=eqn, phys=s.phys, prec=FP64_FMA)
+        K.check_errors()
+        got = host(r, Q.shape)
+        assert rel_l2(got, want) <= 1e-12
+        Qo, ro = Q.copy(order="F"), want.copy(order="F")
+        O.updatesolution(s, ro, Qo, 1.0, 0.5, 0.01)
+        got_q = host(qout, Q.shape)
+        assert rel_l2(got_q, Qo) <= 1e-12
+        # the increment alone: rounding Q + dQ costs up to ulp(Q) / |dQ|, and
+        # |dQ| / |Q| is about 1e-3 here, hence the looser bound
+        assert rel_l2(got_q - Q, Qo - Q) <= 1e-11
+        if eqn == 1:
+            assert np.array_equal(got_q[:, 3, :], Q[:, 3, :])  # b is never evolved
+
+
 def run_gpu_lsrk(s, ds, Q, dt, nsteps, fused):
     A, B, _ = P.rk_coefs()
     q = dev(Q)
     rhs = dev(s.zeros_state())
     q2 = torch.empty_like(q) if fused else None
+    prec = FP64_FMA if fused == "fma" else FP64
     for _ in range(nsteps):
         for st in range(5):
             if fused:
                 K.stage_fused(s.dim, s.N, rhs, q, q2, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM,
                               ds.vmapP, ds.bndy, A[(st + 1) % 5], B[st], dt, eqn=s.eqn,
-                              phys=s.phys)
+                              phys=s.phys, prec=prec)
                 q, q2 = q2, q
             else:
                 K.volumerhs(s.dim, s.N, rhs, q, ds.vgeo, ds.D, s.nelem, eqn=s.eqn, phys=s.phys)
@@ -164,7 +196,7 @@ def run_gpu_lsrk(s, ds, Q, dt, nsteps, fused):
 
 
 @pytest.mark.parametrize("which", ["bubble3d", "bubble2d", "swe"])
-@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("fused", [False, True, "fma"])
 def test_lsrk_100_steps(cuda, which, fused):
     print("lsrk100 %s fused=%s" % (which, fused), flush=True)
     if which == "bubble3d":
@@ -185,7 +217,7 @@ def test_lsrk_100_steps(cuda, which, fused):
     err = rel_l2(Qg, Qo)
     print("rel L2 after 100 steps: %.3e" % err, flush=True)
     assert err <= 1e-10
-    if which == "swe":
+    if which == "swe" and fused != "fma":
         assert max_ulp(Qg, Qo) == 0  # no pow in the shallow-water path
 
 
