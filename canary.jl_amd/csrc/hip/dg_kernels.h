@@ -156,6 +156,7 @@ struct FaceItem {
   Real sMJ;
   Real qP[Phys::NS];
   int64_t eP;
+  int64_t idM, idP, bcode;  // map and bc entries as loaded (face_load)
   int vidM;  // -1: no contribution (an error was recorded)
   int vidP;
   int bc;
@@ -169,6 +170,63 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// face_meta in two halves: face_load issues the loads (no use of their
+// values, so a kernel can issue more loads behind them), face_resolve checks
+// the maps and bc and splits the node ids.
+template <class Phys, int NQ>
+__device__ __forceinline__ void face_load(FaceItem<Phys>& it,
+                                          const typename Phys::Real* __restrict__ sgeo,
+                                          const int64_t* __restrict__ vmapM,
+                                          const int64_t* __restrict__ vmapP,
+                                          const int64_t* __restrict__ bndy, int64_t e, int fi) {
+  using S = Shape<Phys, NQ>;
+  constexpr int DIM = Phys::DIM, NSG = Phys::NSGEO;
+  const int f = fi / S::Nfp;
+  const size_t fo = (size_t)fi + (size_t)S::NFN * (size_t)e;
+  const typename Phys::Real* sg = sgeo + (size_t)NSG * fo;
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) it.nrm[c] = sg[c];
+  it.sMJ = sg[DIM];
+  it.idM = vmapM[fo];
+  it.idP = vmapP[fo];
+  it.bcode = bndy[f + S::NFACE * e];
+}
+
+// The maps are split and selected without branching on them (a load whose
+// value is first used under a branch is moved under it by the compiler, and
+// then waiting for it waits for every load issued before); errors are
+// recorded in the reference's order of checks: vmapM, then vmapP of an
+// interior face, then the bc code.
+template <class Phys, int NQ>
+__device__ __forceinline__ void face_resolve(FaceItem<Phys>& it, int64_t e, int fi,
+                                             int64_t ntotal) {
+  using S = Shape<Phys, NQ>;
+  constexpr int Np = S::Np;
+  const int f = fi / S::Nfp;
+  const int64_t idM = it.idM, idP = it.idP, bc = it.bcode;
+  const int64_t nodes = ntotal * (int64_t)Np;
+  const bool okM = idM >= 1 && idM <= nodes;
+  const bool okP = idP >= 1 && idP <= nodes;
+  const bool interior = bc == 0;
+  const bool good = okM && (bc == 1 || (interior && okP));
+  int64_t eP, eM;
+  int vP, vM;
+  split_id<Np>(okP ? idP : 1, eP, vP);
+  split_id<Np>(okM ? idM : 1, eM, vM);
+  it.bc = (int)bc;
+  it.eP = good && interior ? eP : 0;
+  it.vidP = good && interior ? vP : 0;
+  it.vidM = good ? vM : -1;  // -1: no contribution
+  if (!good) {
+    if (!okM)
+      record_error(5 /*CANARY_ERR_VMAP*/, f + 1, e + 1, idM);
+    else if (interior)
+      record_error(5 /*CANARY_ERR_VMAP*/, f + 1, e + 1, idP);
+    else
+      record_error(4 /*CANARY_ERR_BC*/, f + 1, e + 1, bc);
+  }
+}
+
 template <class Phys, int NQ>
 __device__ __forceinline__ void face_meta(FaceItem<Phys>& it,
                                           const typename Phys::Real* __restrict__ sgeo,
@@ -176,38 +234,8 @@ __device__ __forceinline__ void face_meta(FaceItem<Phys>& it,
                                           const int64_t* __restrict__ vmapP,
                                           const int64_t* __restrict__ bndy, int64_t e, int fi,
                                           int64_t ntotal) {
-  using S = Shape<Phys, NQ>;
-  constexpr int Np = S::Np, DIM = Phys::DIM, NSG = Phys::NSGEO;
-  const int f = fi / S::Nfp;
-  const size_t fo = (size_t)fi + (size_t)S::NFN * (size_t)e;
-  const typename Phys::Real* sg = sgeo + (size_t)NSG * fo;
-#pragma unroll
-  for (int c = 0; c < DIM; ++c) it.nrm[c] = sg[c];
-  it.sMJ = sg[DIM];
-  const int64_t idM = vmapM[fo];
-  const int64_t idP = vmapP[fo];
-  const int64_t bc = bndy[f + S::NFACE * e];
-  const int64_t nodes = ntotal * (int64_t)Np;
-  it.vidM = -1;
-  it.bc = (int)bc;
-  it.eP = 0;
-  it.vidP = 0;
-  if (idM < 1 || idM > nodes) {
-    record_error(5 /*CANARY_ERR_VMAP*/, f + 1, e + 1, idM);
-    return;
-  }
-  if (bc == 0) {
-    if (idP < 1 || idP > nodes) {
-      record_error(5 /*CANARY_ERR_VMAP*/, f + 1, e + 1, idP);
-      return;
-    }
-    split_id<Np>(idP, it.eP, it.vidP);
-  } else if (bc != 1) {
-    record_error(4 /*CANARY_ERR_BC*/, f + 1, e + 1, bc);
-    return;
-  }
-  int64_t eMdummy;
-  split_id<Np>(idM, eMdummy, it.vidM);
+  face_load<Phys, NQ>(it, sgeo, vmapM, vmapP, bndy, e, fi);
+  face_resolve<Phys, NQ>(it, e, fi, ntotal);
 }
 
 template <class Phys, int NQ>
@@ -219,6 +247,25 @@ __device__ __forceinline__ void face_gather(FaceItem<Phys>& it,
 #pragma unroll
     for (int s = 0; s < NS; ++s) it.qP[s] = qe[it.vidP + Np * s];
   }
+}
+
+// face_gather with a load on every lane: items without a P-side state
+// (invalid, wall, bad map) read node 0 of element e_safe, which is in bounds,
+// and leave qP to face_finish.
+template <class Phys, int NQ>
+__device__ __forceinline__ void face_gather_any(FaceItem<Phys>& it,
+                                                const typename Phys::Real* __restrict__ Q,
+                                                bool valid, int64_t e_safe) {
+  constexpr int Np = Shape<Phys, NQ>::Np, NS = Phys::NS;
+  const bool ok = valid && it.vidM >= 0 && it.bc == 0;
+  const typename Phys::Real* qe = Q + (size_t)NS * Np * (size_t)(ok ? it.eP : e_safe);
+  const int v = ok ? it.vidP : 0;
+  typename Phys::Real g[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) g[s] = qe[v + Np * s];
+  if (ok)
+#pragma unroll
+    for (int s = 0; s < NS; ++s) it.qP[s] = g[s];
 }
 
 // M-side state through qm_at(s, node) and pm_at(node, qM); out[NSR].
@@ -414,9 +461,10 @@ __global__ __launch_bounds__(256) void update_kernel(double* __restrict__ rhs,
 //
 // Schedule (per workgroup):
 //   1. each thread issues the loads of its first face item (maps, normals,
-//      bc) together with its volume node's Q, metric terms and rhs;
-//   2. volume fluxes -> LDS; the P-side gather of the face item is issued
-//      as soon as its map has arrived;
+//      bc), of D and of its volume node's Q, metric terms and rhs, before
+//      it uses any of them;
+//   2. the P-side gather of the face item is issued as soon as its map has
+//      arrived; volume fluxes -> LDS;
 //   3. the xi/eta/zeta sweeps run from LDS while the gathers are in flight
 //      (barriers fence LDS only, so loads stay outstanding across them);
 //   4. face fluxes -> LDS, scatter onto the staged rhs (DIM rounds);
@@ -480,7 +528,12 @@ __device__ __forceinline__ void stage_body(
   const int64_t e = e0 + slot;
   const bool valid = slot < EPW && e < nreal;
 
-  // 1. this thread's face items (t, t + NT, ...): metadata now
+  // 1. issue every load of the block before any of them is used: the face
+  //    items' normals, maps and bc (t, t + NT, ...), D, the node's Q, metric
+  //    terms and rhs -- one memory round trip instead of one per kind.  The
+  //    loads are unconditional (lanes without work read in-bounds element
+  //    e0, node 0, and ignore it): a load under a branch makes every later
+  //    wait count for the path without it, i.e. wait for everything.
   FaceItem<Phys> fit[KP];
   bool fval[KP];
 #pragma unroll
@@ -488,26 +541,42 @@ __device__ __forceinline__ void stage_body(
     const int item = t + k * NT;
     const int sl = item / NFN;
     fval[k] = item < EPW * NFN && e0 + sl < nreal;
-    if (fval[k]) face_meta<Phys, NQ>(fit[k], sgeo, vmapM, vmapP, bndy, e0 + sl, item - sl * NFN, ntotal);
+    face_load<Phys, NQ>(fit[k], sgeo, vmapM, vmapP, bndy, fval[k] ? e0 + sl : e0,
+                        fval[k] ? item - sl * NFN : 0);
   }
-  if (t < NQ * NQ) sD[t] = (Real)D[t];
-
-  Real r[NSR], MJI = 0;
+  const double dval = D[t < NQ * NQ ? t : 0];
+  // keep the map loads ahead of the node loads: waiting for the maps then
+  // leaves the node loads in flight (the wait counter is in issue order)
+  __builtin_amdgcn_sched_barrier(0);
+  const int64_t el = valid ? e : e0;
+  const int nl = valid ? n : 0;
+  Real q[NS], met[DIM * DIM], r[NSR];
+  const Real* qe = Qin + (size_t)NS * Np * (size_t)el;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) q[s] = qe[nl + Np * s];
+  const Real* vg = vgeo + (size_t)NV * Np * (size_t)el;
+#pragma unroll
+  for (int c = 0; c < DIM * DIM; ++c) met[c] = vg[nl + Np * c];
+  const Real MJ = vg[nl + Np * Phys::iMJ];
+  const Real MJI = vg[nl + Np * Phys::iMJI];
+  const Real* re = rhs + (size_t)NS * Np * (size_t)el;
+#pragma unroll
+  for (int s = 0; s < NSR; ++s) r[s] = re[nl + Np * s];
+  // 2. maps checked as soon as they arrive and the P-side gathers issued
+  //    (unconditionally, as above: lanes without an item check the item
+  //    they loaded, e0's first face node, which some lane owns anyway);
+  //    the gathers stay in flight over the volume work and the sweeps
+#pragma unroll
+  for (int k = 0; k < KP; ++k) {
+    const int item = t + k * NT;
+    const int sl = item / NFN;
+    face_resolve<Phys, NQ>(fit[k], fval[k] ? e0 + sl : e0, fval[k] ? item - sl * NFN : 0,
+                           ntotal);
+    face_gather_any<Phys, NQ>(fit[k], Qin, fval[k], e0);
+  }
+  if (t < NQ * NQ) sD[t] = (Real)dval;
   Real Fv[DIM][NSR];
   if (valid) {
-    Real q[NS];
-    const Real* qe = Qin + (size_t)NS * Np * (size_t)e;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) q[s] = qe[n + Np * s];
-    const Real* vg = vgeo + (size_t)NV * Np * (size_t)e;
-    Real met[DIM * DIM];
-#pragma unroll
-    for (int c = 0; c < DIM * DIM; ++c) met[c] = vg[n + Np * c];
-    const Real MJ = vg[n + Np * Phys::iMJ];
-    MJI = vg[n + Np * Phys::iMJI];
-    const Real* re = rhs + (size_t)NS * Np * (size_t)e;
-#pragma unroll
-    for (int s = 0; s < NSR; ++s) r[s] = re[n + Np * s];
 #pragma unroll
     for (int s = 0; s < NS; ++s) sQ[slot * NS * Np + s * Np + n] = q[s];
     const Real P = Phys::HAS_P ? Phys::pressure(q, p) : Real(0);
@@ -515,10 +584,6 @@ __device__ __forceinline__ void stage_body(
     Phys::volume_flux(q, P, met, MJ, p, Fv);
     Phys::source(q, MJ, p, r);
   }
-  // 2. P-side gathers (stay in flight over the sweeps)
-#pragma unroll
-  for (int k = 0; k < KP; ++k)
-    if (fval[k]) face_gather<Phys, NQ>(fit[k], Qin);
   // 3. sweeps, one direction at a time through one flux buffer
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
