@@ -29,8 +29,75 @@ __device__ __forceinline__ double jmax(double a, double b) {
   return b > a ? b : a;
 }
 
+// ------------------------------------------------------- arithmetic policies
+// ExactOps: IEEE division and pow, the reference's operations (the exact-order
+// kernels, compiled with -ffp-contract=off).
+struct ExactOps {
+  __device__ static double recip(double x) { return 1 / x; }
+  __device__ static double powk(double x, double k) { return pow(x, k); }
+};
+
+// FastOps: the contracted kernels (CANARY_FP64_FMA, -ffp-contract=fast).
+//   recip: the hardware reciprocal estimate refined by two Newton steps
+//     r <- r + r (1 - x r), each step squaring the relative error, so the
+//     estimate's error falls below the last bit (within an ulp of 1/x).
+//   powk: x^k = exp(k log x) with both functions as short series where the
+//     arguments of an atmosphere put them (x = R rho theta / p0 near 1):
+//     log x = 2 atanh(s) = 2 (s + s^3/3 + s^5/5 + ...), s = (x - 1)/(x + 1),
+//     for 1/sqrt2 < x < sqrt2 (|s| < 0.1716: ten terms leave < 2^-55);
+//     e^y as its Taylor series to y^14/14! for |y| <= 1/2 (the next term is
+//     < 2^-55 of e^y).  Outside those ranges the device log / exp are used.
+//     Evaluated by Horner's rule with fused multiply-adds; a few ulp in all.
+struct FastOps {
+  __device__ static double recip(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+  }
+  __device__ static double powk(double x, double k) {
+    double lx;
+    if (x > 0.70710678118654752 && x < 1.4142135623730950) {
+      const double s = (x - 1) * recip(x + 1);
+      const double z = s * s;
+      double a = 1.0 / 19;
+      a = fma(a, z, 1.0 / 17);
+      a = fma(a, z, 1.0 / 15);
+      a = fma(a, z, 1.0 / 13);
+      a = fma(a, z, 1.0 / 11);
+      a = fma(a, z, 1.0 / 9);
+      a = fma(a, z, 1.0 / 7);
+      a = fma(a, z, 1.0 / 5);
+      a = fma(a, z, 1.0 / 3);
+      a = fma(a, z, 1.0);
+      lx = (2 * s) * a;
+    } else {
+      lx = log(x);
+    }
+    const double y = k * lx;
+    if (!(fabs(y) <= 0.5)) return exp(y);
+    double b = 1.0 / 87178291200.0;  // 1/14!
+    b = fma(b, y, 1.0 / 6227020800.0);
+    b = fma(b, y, 1.0 / 479001600.0);
+    b = fma(b, y, 1.0 / 39916800.0);
+    b = fma(b, y, 1.0 / 3628800.0);
+    b = fma(b, y, 1.0 / 362880.0);
+    b = fma(b, y, 1.0 / 40320.0);
+    b = fma(b, y, 1.0 / 5040.0);
+    b = fma(b, y, 1.0 / 720.0);
+    b = fma(b, y, 1.0 / 120.0);
+    b = fma(b, y, 1.0 / 24.0);
+    b = fma(b, y, 1.0 / 6.0);
+    b = fma(b, y, 0.5);
+    b = fma(b, y, 1.0);
+    return fma(b, y, 1.0);
+  }
+};
+
 // ------------------------------------------------------------ 3D Euler set2c
-struct Euler3D {
+template <class Ops>
+struct Euler3DT {
   using Real = double;
   static constexpr int DIM = 3, NS = 5, NSR = 5, NVGEO = 14, NSGEO = 5;
   static constexpr int iMJ = 9, iMJI = 10;
@@ -38,13 +105,13 @@ struct Euler3D {
   enum { U = 0, V = 1, W = 2, RHO = 3, E = 4 };
 
   __device__ static double pressure(const double* q, const PhysParams& p) {
-    return p.p0 * pow(p.R * q[E] / p.p0, p.kappa);
+    return p.p0 * Ops::powk(p.R * q[E] / p.p0, p.kappa);
   }
   // met[d + DIM*c] = d(xi_d)/d(x_c) in the vgeo field order.
   __device__ static void volume_flux(const double* q, double P, const double* met, double MJ,
                                      const PhysParams&, double F[DIM][NSR]) {
     const double Uq = q[U], Vq = q[V], Wq = q[W], rho = q[RHO], Eq = q[E];
-    const double ri = 1 / rho;
+    const double ri = Ops::recip(rho);
     const double frx = Uq, fUx = ri * Uq * Uq + P, fVx = ri * Vq * Uq, fWx = ri * Wq * Uq, fEx = Eq * ri * Uq;
     const double fry = Vq, fUy = ri * Uq * Vq, fVy = ri * Vq * Vq + P, fWy = ri * Wq * Vq, fEy = Eq * ri * Vq;
     const double frz = Wq, fUz = ri * Uq * Wq, fVz = ri * Vq * Wq, fWz = ri * Wq * Wq + P, fEz = Eq * ri * Wq;
@@ -77,11 +144,11 @@ struct Euler3D {
     const double nx = n[0], ny = n[1], nz = n[2];
     const double UM = qM[U], VM = qM[V], WM = qM[W], rM = qM[RHO], EM = qM[E];
     const double UP = qP[U], VP = qP[V], WP = qP[W], rP = qP[RHO], EP = qP[E];
-    const double rMi = 1 / rM;
+    const double rMi = Ops::recip(rM);
     const double frMx = UM, fUMx = rMi * UM * UM + PM, fVMx = rMi * VM * UM, fWMx = rMi * WM * UM, fEMx = rMi * UM * EM;
     const double frMy = VM, fUMy = rMi * UM * VM, fVMy = rMi * VM * VM + PM, fWMy = rMi * WM * VM, fEMy = rMi * VM * EM;
     const double frMz = WM, fUMz = rMi * UM * WM, fVMz = rMi * VM * WM, fWMz = rMi * WM * WM + PM, fEMz = rMi * WM * EM;
-    const double rPi = 1 / rP;
+    const double rPi = Ops::recip(rP);
     const double frPx = UP, fUPx = rPi * UP * UP + PP, fVPx = rPi * VP * UP, fWPx = rPi * WP * UP, fEPx = rPi * UP * EP;
     const double frPy = VP, fUPy = rPi * UP * VP, fVPy = rPi * VP * VP + PP, fWPy = rPi * WP * VP, fEPy = rPi * VP * EP;
     const double frPz = WP, fUPz = rPi * UP * WP, fVPz = rPi * VP * WP, fWPz = rPi * WP * WP + PP, fEPz = rPi * WP * EP;
@@ -96,8 +163,12 @@ struct Euler3D {
   }
 };
 
+struct Euler3D : Euler3DT<ExactOps> {};
+struct Euler3DFma : Euler3DT<FastOps> {};
+
 // ------------------------------------------------------------ 2D Euler set2c
-struct Euler2D {
+template <class Ops>
+struct Euler2DT {
   using Real = double;
   static constexpr int DIM = 2, NS = 4, NSR = 4, NVGEO = 8, NSGEO = 4;
   static constexpr int iMJ = 4, iMJI = 5;
@@ -105,12 +176,12 @@ struct Euler2D {
   enum { U = 0, V = 1, RHO = 2, E = 3 };
 
   __device__ static double pressure(const double* q, const PhysParams& p) {
-    return p.p0 * pow(p.R * q[E] / p.p0, p.kappa);
+    return p.p0 * Ops::powk(p.R * q[E] / p.p0, p.kappa);
   }
   __device__ static void volume_flux(const double* q, double P, const double* met, double MJ,
                                      const PhysParams&, double F[DIM][NSR]) {
     const double Uq = q[U], Vq = q[V], rho = q[RHO], Eq = q[E];
-    const double ri = 1 / rho;
+    const double ri = Ops::recip(rho);
     const double frx = Uq, fUx = ri * Uq * Uq + P, fVx = ri * Vq * Uq, fEx = ri * Uq * Eq;
     const double fry = Vq, fUy = ri * Uq * Vq, fVy = ri * Vq * Vq + P, fEy = ri * Vq * Eq;
 #pragma unroll
@@ -138,10 +209,10 @@ struct Euler2D {
     const double nx = n[0], ny = n[1];
     const double UM = qM[U], VM = qM[V], rM = qM[RHO], EM = qM[E];
     const double UP = qP[U], VP = qP[V], rP = qP[RHO], EP = qP[E];
-    const double rMi = 1 / rM;
+    const double rMi = Ops::recip(rM);
     const double frMx = UM, fUMx = rMi * UM * UM + PM, fVMx = rMi * VM * UM, fEMx = rMi * UM * EM;
     const double frMy = VM, fUMy = rMi * UM * VM, fVMy = rMi * VM * VM + PM, fEMy = rMi * VM * EM;
-    const double rPi = 1 / rP;
+    const double rPi = Ops::recip(rP);
     const double frPx = UP, fUPx = rPi * UP * UP + PP, fVPx = rPi * VP * UP, fEPx = rPi * UP * EP;
     const double frPy = VP, fUPy = rPi * UP * VP, fVPy = rPi * VP * VP + PP, fEPy = rPi * VP * EP;
     const double lM = rMi * fabs(nx * UM + ny * VM) + sqrt(rMi * p.gamma * PM);
@@ -154,27 +225,8 @@ struct Euler2D {
   }
 };
 
-// ------------------------------------------- Euler with contracted FP64 (FMA)
-// The same equations for the kernels built with -ffp-contract=fast (the
-// CANARY_FP64_FMA stage): products are fused into the adds that follow them
-// and the pressure is exp(kappa * log x) instead of pow(x, kappa).  Both cut
-// FP64 instructions; the results leave the reference's bit pattern but stay
-// within a few ulp per operation (log and exp are each within an ulp, and
-// for the x = R rho theta / p0 of an atmosphere, |kappa log x| < 1, so the
-// product adds no more than an ulp of its own).
-__device__ __forceinline__ double pow_explog(double x, double k) { return exp(k * log(x)); }
-
-struct Euler3DFma : Euler3D {
-  __device__ static double pressure(const double* q, const PhysParams& p) {
-    return p.p0 * pow_explog(p.R * q[E] / p.p0, p.kappa);
-  }
-};
-
-struct Euler2DFma : Euler2D {
-  __device__ static double pressure(const double* q, const PhysParams& p) {
-    return p.p0 * pow_explog(p.R * q[E] / p.p0, p.kappa);
-  }
-};
+struct Euler2D : Euler2DT<ExactOps> {};
+struct Euler2DFma : Euler2DT<FastOps> {};
 
 // ------------------------------------------------------------ shallow water
 // Q = (H u, H v, h, b); b (bathymetry / still depth) has no tendency.
