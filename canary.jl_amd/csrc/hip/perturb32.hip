@@ -197,41 +197,52 @@ __device__ __forceinline__ void body(Lds32<DIM, NQ, EPW>& L, float* __restrict__
   const int64_t e = e0 + slot;
   const bool valid = slot < EPW && e < nreal;
 
-  // first face item: maps and normals now, P-side gather after the volume loads
+  // Every load of the block is issued before any is used (face item maps and
+  // normals, D, the node's perturbation, metric terms, background and rhs),
+  // unconditionally: lanes without work read element e0, node 0, in bounds,
+  // and ignore it (a load under a branch makes later waits count for the
+  // path without it, i.e. wait for every load).
   const int fsl = t / NFN, ffi = t - fsl * NFN;
   const bool fvalid = t < EPW * NFN && e0 + fsl < nreal;
-  float fn4[4] = {0.f, 0.f, 0.f, 0.f};
-  int32_t pnode = 0;
-  int fbc = 1;
+  const size_t fo = fvalid ? (size_t)ffi + (size_t)NFN * (size_t)(e0 + fsl) : (size_t)NFN * (size_t)e0;
+  const float4 s4 = *reinterpret_cast<const float4*>(sg + 4 * fo);
+  const int32_t pnode_l = nbr[fo];
+  const int fbc_l = bnd[(fvalid ? ffi / Nfp : 0) + NF * (fvalid ? e0 + fsl : e0)];
+  const double dval = D[t < NQ * NQ ? t : 0];
+  const int64_t el = valid ? e : e0;
+  const int nl = valid ? n : 0;
+  float r[NS], q[NS], met[DIM * DIM];
+  const float* qe = qin + (size_t)NS * Np * el;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) q[s] = qe[nl + Np * s];
+  const float* v = vg + (size_t)G::NV * Np * el;
+#pragma unroll
+  for (int c = 0; c < DIM * DIM; ++c) met[c] = v[nl + Np * c];
+  const float MJ = v[nl + Np * DIM * DIM];
+  const double MJI = (double)v[nl + Np * (DIM * DIM + 1)];
+  const double* b = bg + (size_t)3 * Np * el;
+  const double b0 = b[nl], b1 = b[nl + Np], b2 = b[nl + 2 * Np];
+  const float* re = rhs + (size_t)NS * Np * el;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) r[s] = re[nl + Np * s];
+  // the P-side gather of the first face item, as soon as its map is here
+  const float fn4[4] = {s4.x, s4.y, s4.z, s4.w};
+  const int fbc = fvalid ? fbc_l : 1;
+  const bool gat = fvalid && fbc == 0;
+  const int32_t pnode = gat ? pnode_l : (int32_t)(e0 * Np);
   float qP[NS];
+  {
+    const float* qg = qin + (size_t)NS * Np * (size_t)(pnode / Np);
+    const int pv = pnode % Np;
 #pragma unroll
-  for (int s = 0; s < NS; ++s) qP[s] = 0.f;
-  if (fvalid) {
-    const size_t fo = (size_t)ffi + (size_t)NFN * (size_t)(e0 + fsl);
-    const float4 s4 = *reinterpret_cast<const float4*>(sg + 4 * fo);
-    fn4[0] = s4.x; fn4[1] = s4.y; fn4[2] = s4.z; fn4[3] = s4.w;
-    pnode = nbr[fo];
-    fbc = bnd[ffi / Nfp + NF * (e0 + fsl)];
+    for (int s = 0; s < NS; ++s) {
+      const float g = qg[pv + Np * s];
+      qP[s] = gat ? g : 0.f;
+    }
   }
-  if (t < NQ * NQ) L.D[t] = (float)D[t];
-
-  float r[NS], q[NS];
-  double MJI = 0.0;
+  if (t < NQ * NQ) L.D[t] = (float)dval;
   if (valid) {
-    const float* qe = qin + (size_t)NS * Np * e;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) q[s] = qe[n + Np * s];
-    const float* v = vg + (size_t)G::NV * Np * e;
-    float met[DIM * DIM];
-#pragma unroll
-    for (int c = 0; c < DIM * DIM; ++c) met[c] = v[n + Np * c];
-    const float MJ = v[n + Np * DIM * DIM];
-    MJI = (double)v[n + Np * (DIM * DIM + 1)];
-    const double* b = bg + (size_t)3 * Np * e;
-    const float rho0 = (float)b[n], rt0 = (float)b[n + Np], P0 = (float)b[n + 2 * Np];
-    const float* re = rhs + (size_t)NS * Np * e;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) r[s] = re[n + Np * s];
+    const float rho0 = (float)b0, rt0 = (float)b1, P0 = (float)b2;
     const float Pp = pprime(P0, rt0, q[E], p.kappa);
 #pragma unroll
     for (int s = 0; s < NS; ++s) L.Q[slot][s * Np + n] = q[s];
@@ -262,12 +273,6 @@ __device__ __forceinline__ void body(Lds32<DIM, NQ, EPW>& L, float* __restrict__
       }
     }
     r[DIM - 1] -= MJ * q[RHO] * p.g;
-  }
-  if (fvalid && fbc == 0) {
-    const float* qe = qin + (size_t)NS * Np * (size_t)(pnode / Np);
-    const int pv = pnode % Np;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) qP[s] = qe[pv + Np * s];
   }
   lds_barrier();
   if (valid) {
@@ -360,14 +365,14 @@ __device__ __forceinline__ void body(Lds32<DIM, NQ, EPW>& L, float* __restrict__
     }
     lds_barrier();
   }
-  if (valid) {
+  if (valid) {  // non-temporal stores: not read again in this stage
     float* qo = qout + (size_t)NS * Np * e;
-    float* re = rhs + (size_t)NS * Np * e;
+    float* ro = rhs + (size_t)NS * Np * e;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       const float rs = L.B[slot][NFN * NS + s * Np + n];
-      qo[n + Np * s] = (float)((double)q[s] + (bdt * (double)rs) * MJI);
-      re[n + Np * s] = rka * rs;
+      __builtin_nontemporal_store((float)((double)q[s] + (bdt * (double)rs) * MJI), &qo[n + Np * s]);
+      __builtin_nontemporal_store(rka * rs, &ro[n + Np * s]);
     }
   }
 }
