@@ -2,8 +2,10 @@
 32^3 hexahedra, N = 4 (4.1M nodes x 5 states), FP64, one MI355X.
 
 A step is one LSRK54 step (5 stages).  Each stage is one launch of the fused
-single-pass kernel (volume + faces + update, Q ping-pong); the three-kernel
-drop-in path (volumerhs, facerhs, updatesolution) is timed beside it.
+single-pass kernel (volume + faces + update, Q ping-pong) in contracted FP64
+(multiply-adds fused); the same kernel in the reference evaluation order and
+the three-kernel drop-in path (volumerhs, facerhs, updatesolution) are timed
+beside it.
 
 Metric: GDOF-stage/s = nelem * Np * nstate * stages / seconds / 1e9, with
 all arrays resident in HBM before the timed region.
@@ -394,14 +396,14 @@ def _gpu_timer():
 def batched_form(nsolves, steps, seed=42):
     """Ensemble of independent periodic shallow-water solves (Ne^2 quads, Ne
     uniform in 8..64, N in {3,4,5,7}, seeded bumps) advanced together:
-    FP64 and FP32 batches (autotuned elements per workgroup, one graph per
-    step) against the same solves run one after another with the
-    single-solve fused stage."""
+    FP64 (reference evaluation order), contracted FP64 and FP32 batches
+    (autotuned elements per workgroup, one graph per step) against the same
+    solves run one after another with the single-solve fused stage."""
     import numpy as np
     import torch
     import canary_amd.kernels as K
     import canary_amd.problems as P
-    from canary_amd._lib import SWE, FP32, FP64
+    from canary_amd._lib import SWE, FP32, FP64, FP64_FMA
     from canary_amd.batch import Batch
 
     t0 = time.time()
@@ -416,7 +418,7 @@ def batched_form(nsolves, steps, seed=42):
            "solves": nsolves, "dof": dof, "elements": sum(s.nelem for s, _ in ens),
            "orders": sorted({s.N for s, _ in ens}), "steps": steps}
     states = {}
-    for name, prec in (("fp64", FP64), ("fp32", FP32)):
+    for name, prec in (("fp64", FP64), ("fp64_fma", FP64_FMA), ("fp32", FP32)):
         b = Batch(SWE, 2, prec)
         ids = [b.add(s, dt) for s, dt in ens]
         b.finalize(autotune=True)
@@ -436,6 +438,9 @@ def batched_form(nsolves, steps, seed=42):
     errs = [np.linalg.norm(a[:, :3] - c[:, :3]) / np.linalg.norm(c[:, :3])
             for a, c in zip(states["fp32"], states["fp64"])]
     out["fp32_rel_l2_vs_fp64_max"] = float("%.3e" % max(errs))
+    errs = [np.linalg.norm(a[:, :3] - c[:, :3]) / np.linalg.norm(c[:, :3])
+            for a, c in zip(states["fp64_fma"], states["fp64"])]
+    out["fp64_fma_rel_l2_vs_fp64_max"] = float("%.3e" % max(errs))
     # the same solves one after another, single-solve fused stage, FP64
     A, B, _ = P.rk_coefs()
     dev = torch.device("cuda:0")
@@ -463,6 +468,8 @@ def batched_form(nsolves, steps, seed=42):
                               "ms_per_step": round(t / steps * 1e3, 4)}
     out["speedup_batched_fp64_vs_sequential"] = round(
         out["fp64"]["gdof_stage_s"] / out["sequential_fp64"]["gdof_stage_s"], 3)
+    out["speedup_batched_fp64_fma_vs_sequential"] = round(
+        out["fp64_fma"]["gdof_stage_s"] / out["sequential_fp64"]["gdof_stage_s"], 3)
     return out
 
 

@@ -23,7 +23,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import EULER, SWE, FP64, FP32, check, lib
+from ._lib import EULER, SWE, FP64, FP32, FP64_FMA, check, lib
 from .perturb import Perturb32
 from .problems import device_arrays, rk_coefs
 from .streams import OwnedStream
@@ -346,11 +346,11 @@ class Batch(_BatchBase):
     def __init__(self, eqn, dim, prec=FP64, device="cuda"):
         if (eqn, dim) not in _NS:
             raise ValueError("unknown (eqn, dim) = (%r, %r)" % (eqn, dim))
-        if prec not in (FP64, FP32):
-            raise ValueError("prec must be FP64 or FP32")
+        if prec not in (FP64, FP64_FMA, FP32):
+            raise ValueError("prec must be FP64, FP64_FMA or FP32")
         super().__init__(device)
         self.eqn, self.dim, self.prec = eqn, dim, prec
-        self.rdtype = torch.float64 if prec == FP64 else torch.float32
+        self.rdtype = torch.float32 if prec == FP32 else torch.float64
 
     def _variants(self, N):
         return variants(self.eqn, self.dim, self.prec, N)
@@ -360,7 +360,7 @@ class Batch(_BatchBase):
                                         items_ptr, count, parity, rka, rkb, stream)
 
     def _word(self):
-        return 8 if self.prec == FP64 else 4
+        return 4 if self.prec == FP32 else 8
 
     def _bytes_per_element_stage(self, v):
         return bytes_per_element_stage(self.eqn, self.dim, v.N, self._word())

@@ -1,0 +1,168 @@
+// Batched form, shared by the exact-order unit (batched.hip) and the
+// contracted-FP64 unit (batched_fma.hip): many independent solves (mixed N,
+// element counts, meshes, dt and constants) advanced through a device work
+// table of element blocks.  Each table entry names a solve and the first
+// element of a block; the kernel for one (N, elements-per-workgroup) class
+// runs the fused stage body of dg_kernels.h on every entry of its slice of
+// the table.  One kernel per class keeps each variant's register allocation
+// its own (a single kernel switching over every N is sized by its worst
+// variant and drops to one wave per SIMD); the host launches the classes
+// back to back, or on forked streams inside one graph.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../../include/canary_dg.h"
+#include "dg_kernels.h"
+
+namespace canary {
+namespace batch {
+
+constexpr int MAXT = 256;  // threads per workgroup ceiling
+
+// Candidate `WHICH` of elements per workgroup for (DIM, NQ): 0 = the most
+// elements that fit MAXT threads, 1 = about half of that, 2 = the
+// single-solve choice (first lane fill >= 90%).  0 = no such candidate.
+template <int DIM, int NQ, int WHICH>
+constexpr int epw_candidate() {
+  constexpr int Np = ipow(NQ, DIM);
+  constexpr int emax = MAXT / Np;
+  if (WHICH == 0) return emax;
+  if (WHICH == 1) return emax >= 2 ? (emax + 1) / 2 : 0;
+  if (WHICH == 2) {
+    constexpr int d = default_epw<Np>();
+    return d * Np <= MAXT ? d : 0;
+  }
+  return 0;
+}
+
+__device__ __forceinline__ PhysParams params_of(const canary_phys& c) {
+  PhysParams p;
+  p.gamma = c.gamma;
+  p.p0 = c.p0;
+  p.R = c.R_gas;
+  p.cp = c.c_p;
+  p.cv = c.c_v;
+  p.g = c.gravity;
+  p.kappa = c.c_p / c.c_v;
+  p.swe_g = c.swe_gravity;
+  p.dnl = c.delta_nl;
+  return p;
+}
+
+template <class Phys, int NQ, int EPW>
+struct BCfg {
+  using C = FusedCfg<Phys, NQ, EPW, false>;
+  static constexpr int NT = C::NT;
+};
+
+// One workgroup per work-table entry; workgroup b takes entry
+// (b & 7) * chunk + (b >> 3) (grid = 8 * chunk), so XCD b & 7 gets a
+// contiguous run of the table.  (A grid-stride loop over entries keeps the
+// per-thread face and LDS addressing live across iterations and costs
+// about 100 VGPRs in 3D; one entry per workgroup does not.)
+template <class Phys, int NQ, int EPW>
+__global__ __launch_bounds__((BCfg<Phys, NQ, EPW>::NT)) void batched_kernel(
+    const canary_batch_solve* __restrict__ solves, const canary_batch_item* __restrict__ items,
+    int64_t nitems, int64_t chunk, int parity, double rka, double rkb) {
+  using C = FusedCfg<Phys, NQ, EPW, false>;
+  using Real = typename Phys::Real;
+  __shared__ __attribute__((aligned(16))) char lds[C::LDS];
+  const int64_t w = xcd_block(blockIdx.x, chunk);
+  if (w >= nitems) return;
+  const canary_batch_item it = items[w];
+  const canary_batch_solve& sv = solves[it.solve];
+  if (it.N != NQ - 1 || it.epw != EPW || sv.N != NQ - 1) {
+    if (threadIdx.x == 0) record_error(1 /*CANARY_ERR_ARG*/, it.solve, it.e0 + 1, it.N);
+    return;
+  }
+  if (it.e0 < 0 || it.e0 >= sv.nreal) {
+    if (threadIdx.x == 0) record_error(1 /*CANARY_ERR_ARG*/, it.solve, it.e0 + 1, sv.nreal);
+    return;
+  }
+  const PhysParams p = params_of(sv.phys);
+  stage_body<Phys, NQ, EPW, C::NT, 1, true>(
+      lds, (Real*)sv.rhs, (const Real*)sv.Q[parity], (Real*)sv.Q[1 - parity],
+      (const Real*)sv.vgeo, (const Real*)sv.sgeo, sv.D, it.e0, sv.nreal, sv.nelem_total,
+      sv.vmapM, sv.vmapP, sv.elemtobndy, rka, rkb, sv.dt, p);
+}
+
+template <class Phys, int NQ, int WHICH>
+int run_which(int* epw, int* threads, const canary_batch_solve* solves,
+              const canary_batch_item* items, int64_t nitems, int parity, double rka, double rkb,
+              hipStream_t st) {
+  constexpr int E = epw_candidate<Phys::DIM, NQ, WHICH>();
+  if constexpr (E < 1) {
+    return CANARY_ERR_UNSUPPORTED;
+  } else {
+    constexpr int NT = BCfg<Phys, NQ, E>::NT;
+    if (epw) {  // query
+      *epw = E;
+      *threads = NT;
+      return 0;
+    }
+    const int64_t chunk = (nitems + 7) / 8;
+    if (chunk > ((int64_t)1 << 27)) return CANARY_ERR_ARG;
+    batched_kernel<Phys, NQ, E><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, st>>>(
+        solves, items, nitems, chunk, parity, rka, rkb);
+    return hipGetLastError() == hipSuccess ? 0 : CANARY_ERR_LAUNCH;
+  }
+}
+
+// which >= 0: query candidate `which`; which < 0: launch the candidate whose
+// elements per workgroup equal `want`.
+template <class Phys, int NQ>
+int run_nq(int which, int want, int* epw, int* threads, const canary_batch_solve* solves,
+           const canary_batch_item* items, int64_t nitems, int parity, double rka, double rkb,
+           hipStream_t st) {
+  if (which >= 0) {
+    if (which == 0) return run_which<Phys, NQ, 0>(epw, threads, 0, 0, 0, 0, 0, 0, 0);
+    if (which == 1) return run_which<Phys, NQ, 1>(epw, threads, 0, 0, 0, 0, 0, 0, 0);
+    if (which == 2) return run_which<Phys, NQ, 2>(epw, threads, 0, 0, 0, 0, 0, 0, 0);
+    return CANARY_ERR_UNSUPPORTED;
+  }
+  constexpr int E0 = epw_candidate<Phys::DIM, NQ, 0>();
+  constexpr int E1 = epw_candidate<Phys::DIM, NQ, 1>();
+  constexpr int E2 = epw_candidate<Phys::DIM, NQ, 2>();
+  if (want == E0 && E0 > 0)
+    return run_which<Phys, NQ, 0>(nullptr, nullptr, solves, items, nitems, parity, rka, rkb, st);
+  if (want == E1 && E1 > 0)
+    return run_which<Phys, NQ, 1>(nullptr, nullptr, solves, items, nitems, parity, rka, rkb, st);
+  if (want == E2 && E2 > 0)
+    return run_which<Phys, NQ, 2>(nullptr, nullptr, solves, items, nitems, parity, rka, rkb, st);
+  return CANARY_ERR_UNSUPPORTED;
+}
+
+template <class Phys>
+int run(int N, int which, int want, int* epw, int* threads, const canary_batch_solve* solves,
+        const canary_batch_item* items, int64_t nitems, int parity, double rka, double rkb,
+        hipStream_t st) {
+#define CANARY_BATCH_N(NN)                                                                    \
+  case NN:                                                                                    \
+    return run_nq<Phys, NN + 1>(which, want, epw, threads, solves, items, nitems, parity, rka, \
+                                rkb, st);
+  if constexpr (Phys::DIM == 3) {
+    switch (N) {
+      CANARY_BATCH_N(1)
+      CANARY_BATCH_N(2)
+      CANARY_BATCH_N(3)
+      CANARY_BATCH_N(4)
+      CANARY_BATCH_N(5)
+    }
+  } else {
+    switch (N) {
+      CANARY_BATCH_N(1)
+      CANARY_BATCH_N(2)
+      CANARY_BATCH_N(3)
+      CANARY_BATCH_N(4)
+      CANARY_BATCH_N(5)
+      CANARY_BATCH_N(6)
+      CANARY_BATCH_N(7)
+    }
+  }
+#undef CANARY_BATCH_N
+  return CANARY_ERR_UNSUPPORTED;
+}
+
+}  // namespace batch
+}  // namespace canary

@@ -21,6 +21,7 @@ int canary_k_euler2d_fma_status(int64_t* info, int clear);
 int canary_k_swe2d_fma_status(int64_t* info, int clear);
 int canary_k_perturb32_status(int64_t* info, int clear);
 int canary_k_batch_status(int64_t* info, int clear);
+int canary_k_batch_fma_status(int64_t* info, int clear);
 int canary_k_geometry_status(int64_t* info, int clear);
 int canary_k_aux_status(int64_t* info, int clear);
 }
@@ -178,11 +179,11 @@ extern "C" int canary_check_errors(void* stream, int64_t* info) {
   if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return CANARY_ERR_LAUNCH;
   int64_t tmp[4];
   int first = 0;
-  int (*readers[10])(int64_t*, int) = {
+  int (*readers[11])(int64_t*, int) = {
       canary_k_euler3d_status,     canary_k_euler2d_status,     canary_k_swe2d_status,
       canary_k_euler3d_fma_status, canary_k_euler2d_fma_status, canary_k_swe2d_fma_status,
-      canary_k_perturb32_status,   canary_k_batch_status,       canary_k_geometry_status,
-      canary_k_aux_status};
+      canary_k_perturb32_status,   canary_k_batch_status,       canary_k_batch_fma_status,
+      canary_k_geometry_status,    canary_k_aux_status};
   for (auto rd : readers) {
     int rc = rd(tmp, 1);
     if (rc && !first) {

@@ -156,9 +156,10 @@ int canary_perturb32_reconstruct(int dim, int N, int64_t nelem, const void* qp, 
  * N, element count, mesh, dt and constants, advanced through a device work
  * table of element blocks.  No reference counterpart (the reference runs one
  * solve per process, euler3d_set2c.jl:1049).  Arrays of a solve are the
- * drop-in ones (FP64), or FP32 copies of Q, rhs, vgeo and sgeo for
- * prec = CANARY_FP32 (shallow water only).  Both tables live in device
- * memory. */
+ * drop-in ones (FP64: prec = CANARY_FP64 in the reference evaluation order
+ * or CANARY_FP64_FMA contracted, as for canary_stage_fused), or FP32 copies
+ * of Q, rhs, vgeo and sgeo for prec = CANARY_FP32 (shallow water only).
+ * Both tables live in device memory. */
 typedef struct canary_batch_solve {
   void* Q[2];                  /* ping-pong states; stage reads Q[parity], writes Q[1-parity] */
   void* rhs;                   /* LSRK register                                  */

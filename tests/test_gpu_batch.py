@@ -6,7 +6,9 @@ stage_fused runs (themselves bit-compared with the CPU oracle in
 test_gpu_parity.py), for any mix of N, meshes, dt and constants, for every
 elements-per-workgroup variant, eagerly and through captured graphs, and
 when solves join and leave a running batch.  The FP32 shallow-water variant
-is held to a relative L2 error against FP64 instead.
+is held to a relative L2 error against FP64 instead, and the contracted FP64
+variant (FP64_FMA) to the single-solve contracted stage and the exact-order
+batch.
 """
 import numpy as np
 import pytest
@@ -14,7 +16,7 @@ import torch
 
 import canary_amd.kernels as K
 import canary_amd.problems as P
-from canary_amd._lib import CanaryError, EULER, SWE, FP32, FP64, ERR_ARG
+from canary_amd._lib import CanaryError, EULER, SWE, FP32, FP64, FP64_FMA, ERR_ARG
 from canary_amd.batch import Batch, ITEM_DTYPE, variants
 from oracle import oracle as O
 
@@ -58,7 +60,7 @@ def euler3d_mix():
     return out
 
 
-def single_run(s, dt, nsteps):
+def single_run(s, dt, nsteps, prec=FP64):
     """nsteps LSRK54 steps of one solve through the single-solve fused stage."""
     A, B, _ = P.rk_coefs()
     ds = DeviceSetup(s)
@@ -66,7 +68,8 @@ def single_run(s, dt, nsteps):
     for _ in range(nsteps):
         for st in range(5):
             K.stage_fused(s.dim, s.N, rhs, q, q2, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM,
-                          ds.vmapP, ds.bndy, A[(st + 1) % 5], B[st], dt, eqn=s.eqn, phys=s.phys)
+                          ds.vmapP, ds.bndy, A[(st + 1) % 5], B[st], dt, eqn=s.eqn, phys=s.phys,
+                          prec=prec)
             q, q2 = q2, q
     K.check_errors()
     out = host(q, s.Q.shape)
@@ -92,6 +95,25 @@ def test_batch_equals_single_solves(cuda, eqn, dim, mixf):
     for (s, dt), i in zip(mix, ids):
         want = single_run(s, dt, 3)
         assert max_ulp(b.state(i), want) == 0, "solve N=%d" % s.N
+
+
+@pytest.mark.parametrize("eqn,dim,mixf", [(SWE, 2, swe_mix), (EULER, 2, euler2d_mix),
+                                          (EULER, 3, euler3d_mix)])
+def test_fma_batch_tracks_single_and_exact(cuda, eqn, dim, mixf):
+    mix = mixf()
+    print("fma batch eqn=%d dim=%d N=%s" % (eqn, dim, [s.N for s, _ in mix]), flush=True)
+    b, ids = make_batch(eqn, dim, mix, prec=FP64_FMA)
+    b.run(3, graph=False)
+    b.check()
+    ex, eids = make_batch(eqn, dim, mix)
+    ex.run(3, graph=False)
+    for (s, dt), i, j in zip(mix, ids, eids):
+        got = b.state(i)
+        want = single_run(s, dt, 3, prec=FP64_FMA)
+        print("N=%d ulp vs single contracted %d, rel L2 vs exact %.2e"
+              % (s.N, max_ulp(got, want), rel_l2(got, ex.state(j))), flush=True)
+        assert rel_l2(got, want) <= 1e-13
+        assert rel_l2(got, ex.state(j)) <= 1e-12
 
 
 def test_batch_swe_matches_oracle(cuda):
