@@ -209,6 +209,22 @@ def main():
         st_plain = 5 * max(1, args.steps // 2)
         extra["plain_three_kernel_gdof_stage_s"] = round(dof * st_plain / t_plain / 1e9, 3)
         extra["plain_ms_per_stage"] = round(t_plain / st_plain * 1e3, 4)
+        # volumerhs, then facerhs + updatesolution fused (Q ping-pong)
+        q2p = torch.empty_like(qp)
+
+        def run_two(nsteps):
+            a, b = qp, q2p
+            for _ in range(nsteps):
+                for st in range(5):
+                    K.volumerhs(3, s.N, rp, a, vgeo, D, s.nelem, phys=s.phys)
+                    K.facerhs_update(3, s.N, rp, a, b, vgeo, sgeo, s.nelem, vM, vP, bnd,
+                                     A[(st + 1) % 5], B[st], dt, phys=s.phys)
+                    a, b = b, a
+        run_two(1)
+        t_two, _ = timed(lambda: run_two(max(1, args.steps // 2)))
+        K.check_errors()
+        extra["volume_then_face_update_gdof_stage_s"] = round(dof * st_plain / t_two / 1e9, 3)
+        del qp, q2p, rp
 
     cpu = None
     if not args.no_cpu_baseline:

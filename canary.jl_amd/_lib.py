@@ -84,6 +84,8 @@ _SIGS = {
     "canary_updatesolution": (INT, [INT, INT, INT, INT, P, P, P, I64, D, D, D, INT, P]),
     "canary_stage_fused": (INT, [INT, INT, INT, INT, P, P, P, P, P, P, I64, I64, P, P, P,
                                  D, D, D, C.POINTER(Phys), P]),
+    "canary_facerhs_update": (INT, [INT, INT, INT, INT, P, P, P, P, P, I64, I64, P, P, P,
+                                    D, D, D, C.POINTER(Phys), P]),
     "canary_check_errors": (INT, [P, P]),
     "canary_perturb32_prepare": (INT, [INT, INT, I64, I64, P, P, P, P, P, P, P, C.POINTER(Phys),
                                        P, P, P, P, P, P, P]),

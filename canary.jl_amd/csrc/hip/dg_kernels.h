@@ -497,7 +497,10 @@ struct FusedCfg {
 // FusedCfg::LDS bytes.  PF: face items per thread whose loads are issued
 // before the volume work (the rest are handled afterwards); PM_LDS: keep
 // the nodal pressure in LDS for the M side of the faces (else recompute it).
-template <class Phys, int NQ, int EPW, int NT, int PF, bool PM_LDS>
+// VOLUME = false leaves the volume term out (face terms + update only: the
+// opt-in facerhs_update, after a separate volumerhs); vgeo is then read for
+// MJI alone and D is not read.
+template <class Phys, int NQ, int EPW, int NT, int PF, bool PM_LDS, bool VOLUME = true>
 __device__ __forceinline__ void stage_body(
     char* __restrict__ lds, typename Phys::Real* __restrict__ rhs,
     const typename Phys::Real* __restrict__ Qin, typename Phys::Real* __restrict__ Qout,
@@ -544,7 +547,8 @@ __device__ __forceinline__ void stage_body(
     face_load<Phys, NQ>(fit[k], sgeo, vmapM, vmapP, bndy, fval[k] ? e0 + sl : e0,
                         fval[k] ? item - sl * NFN : 0);
   }
-  const double dval = D[t < NQ * NQ ? t : 0];
+  double dval = 0.0;
+  if constexpr (VOLUME) dval = D[t < NQ * NQ ? t : 0];
   // keep the map loads ahead of the node loads: waiting for the maps then
   // leaves the node loads in flight (the wait counter is in issue order)
   __builtin_amdgcn_sched_barrier(0);
@@ -555,9 +559,12 @@ __device__ __forceinline__ void stage_body(
 #pragma unroll
   for (int s = 0; s < NS; ++s) q[s] = qe[nl + Np * s];
   const Real* vg = vgeo + (size_t)NV * Np * (size_t)el;
+  Real MJ = 0;
+  if constexpr (VOLUME) {
 #pragma unroll
-  for (int c = 0; c < DIM * DIM; ++c) met[c] = vg[nl + Np * c];
-  const Real MJ = vg[nl + Np * Phys::iMJ];
+    for (int c = 0; c < DIM * DIM; ++c) met[c] = vg[nl + Np * c];
+    MJ = vg[nl + Np * Phys::iMJ];
+  }
   const Real MJI = vg[nl + Np * Phys::iMJI];
   const Real* re = rhs + (size_t)NS * Np * (size_t)el;
 #pragma unroll
@@ -574,19 +581,21 @@ __device__ __forceinline__ void stage_body(
                            ntotal);
     face_gather_any<Phys, NQ>(fit[k], Qin, fval[k], e0);
   }
-  if (t < NQ * NQ) sD[t] = (Real)dval;
+  if (VOLUME && t < NQ * NQ) sD[t] = (Real)dval;
   Real Fv[DIM][NSR];
   if (valid) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) sQ[slot * NS * Np + s * Np + n] = q[s];
     const Real P = Phys::HAS_P ? Phys::pressure(q, p) : Real(0);
     if (SP) sP[slot * Np + n] = P;
-    Phys::volume_flux(q, P, met, MJ, p, Fv);
-    Phys::source(q, MJ, p, r);
+    if constexpr (VOLUME) {
+      Phys::volume_flux(q, P, met, MJ, p, Fv);
+      Phys::source(q, MJ, p, r);
+    }
   }
   // 3. sweeps, one direction at a time through one flux buffer
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) {
+  for (int d = 0; d < (VOLUME ? DIM : 0); ++d) {
     if (d > 0) lds_barrier();  // the previous direction's fluxes are consumed
     if (valid) {
 #pragma unroll
@@ -685,6 +694,25 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, WIDE>::NT)) void stage_ker
   if (e0 >= nreal) return;  // whole workgroup leaves together
   stage_body<Phys, NQ, EPW, C::NT, PF, PM_LDS>(lds, rhs, Qin, Qout, vgeo, sgeo, D, e0, nreal,
                                                ntotal, vmapM, vmapP, bndy, rka, rkb, dt, p);
+}
+
+// facerhs! + updatesolution! of one stage in one pass (the volume term was
+// added to rhs by volume_kernel before): Qin -> Qout, rhs in place.
+template <class Phys, int NQ, int EPW>
+__global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, false>::NT)) void face_update_kernel(
+    double* __restrict__ rhs, const double* __restrict__ Qin, double* __restrict__ Qout,
+    const double* __restrict__ vgeo, const double* __restrict__ sgeo, int64_t nreal,
+    int64_t ntotal, const int64_t* __restrict__ vmapM, const int64_t* __restrict__ vmapP,
+    const int64_t* __restrict__ bndy, double rka, double rkb, double dt, int64_t chunk,
+    PhysParams p) {
+  using C = FusedCfg<Phys, NQ, EPW, false>;
+  __shared__ __attribute__((aligned(16))) char lds[C::LDS];
+  const int64_t blk = xcd_block(blockIdx.x, chunk);
+  const int64_t e0 = blk * EPW;
+  if (e0 >= nreal) return;  // whole workgroup leaves together
+  stage_body<Phys, NQ, EPW, C::NT, 1, true, false>(lds, rhs, Qin, Qout, vgeo, sgeo, nullptr, e0,
+                                                   nreal, ntotal, vmapM, vmapP, bndy, rka, rkb,
+                                                   dt, p);
 }
 
 }  // namespace canary

@@ -8,7 +8,7 @@
 
 namespace canary {
 
-enum Op { OP_VOLUME = 0, OP_FACE = 1, OP_UPDATE = 2, OP_STAGE = 3 };
+enum Op { OP_VOLUME = 0, OP_FACE = 1, OP_UPDATE = 2, OP_STAGE = 3, OP_FACE_UPDATE = 4 };
 
 struct KArgs {
   void* rhs;
@@ -49,8 +49,19 @@ void launch_stage(int64_t nb, const KArgs& a) {
       a.dt, chunk, a.phys);
 }
 
-// STAGE_ONLY: the fused stage alone is instantiated (the contracted-FP64
-// units); the other operations report CANARY_ERR_UNSUPPORTED.
+template <class Phys, int NQ, int EPW>
+void launch_face_update(int64_t nb, const KArgs& a) {
+  constexpr int NT = FusedCfg<Phys, NQ, EPW, false>::NT;
+  const int64_t chunk = (nb + 7) / 8;
+  face_update_kernel<Phys, NQ, EPW><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
+      (double*)a.rhs, (const double*)a.Q, (double*)a.Qout, (const double*)a.vgeo,
+      (const double*)a.sgeo, a.nreal, a.ntotal, a.vmapM, a.vmapP, a.bndy, a.rka, a.rkb, a.dt,
+      chunk, a.phys);
+}
+
+// STAGE_ONLY: the fused kernels alone (stage, face + update) are
+// instantiated (the contracted-FP64 units); the other operations report
+// CANARY_ERR_UNSUPPORTED.
 template <class Phys, int NQ, bool STAGE_ONLY>
 int launch(int op, const KArgs& a) {
   using S = Shape<Phys, NQ>;
@@ -58,8 +69,12 @@ int launch(int op, const KArgs& a) {
   if (a.nreal <= 0) return 0;
   const int64_t nb = (a.nreal + EPW - 1) / EPW;
   if constexpr (STAGE_ONLY) {
-    if (op != OP_STAGE) return 2 /*CANARY_ERR_UNSUPPORTED*/;
-    launch_stage<Phys, NQ, EPW>(nb, a);
+    if (op == OP_STAGE)
+      launch_stage<Phys, NQ, EPW>(nb, a);
+    else if (op == OP_FACE_UPDATE)
+      launch_face_update<Phys, NQ, EPW>(nb, a);
+    else
+      return 2 /*CANARY_ERR_UNSUPPORTED*/;
   } else {
     double* rhs = (double*)a.rhs;
     const double* Q = (const double*)a.Q;
@@ -86,6 +101,9 @@ int launch(int op, const KArgs& a) {
       }
       case OP_STAGE:
         launch_stage<Phys, NQ, EPW>(nb, a);
+        break;
+      case OP_FACE_UPDATE:
+        launch_face_update<Phys, NQ, EPW>(nb, a);
         break;
       default:
         return 1 /*CANARY_ERR_ARG*/;

@@ -60,7 +60,7 @@ canary::PhysParams to_params(const canary_phys* ph) {
   return p;
 }
 
-// stage_only: CANARY_FP64_FMA is accepted (canary_stage_fused only).
+// stage_only: CANARY_FP64_FMA is accepted (the fused entries only).
 int check_common(int eqn, int dim, int N, int prec, int64_t nreal, KFn* fn,
                  bool stage_only = false) {
   if (prec == CANARY_FP64_FMA && !stage_only) return CANARY_ERR_UNSUPPORTED;
@@ -173,6 +173,39 @@ extern "C" int canary_stage_fused(int eqn, int dim, int N, int prec, void* rhs, 
   a.phys = to_params(phys);
   a.stream = (hipStream_t)stream;
   return fn(canary::OP_STAGE, N, &a);
+}
+
+extern "C" int canary_facerhs_update(int eqn, int dim, int N, int prec, void* rhs,
+                                     const void* Qin, void* Qout, const void* vgeo,
+                                     const void* sgeo, int64_t nreal, int64_t nelem_total,
+                                     const int64_t* vmapM, const int64_t* vmapP,
+                                     const int64_t* elemtobndy, double rka, double rkb, double dt,
+                                     const canary_phys* phys, void* stream) {
+  KFn fn;
+  int rc = check_common(eqn, dim, N, prec, nreal, &fn, true);
+  if (rc) return rc;
+  if (nreal == 0) return 0;
+  if (nelem_total < nreal) return CANARY_ERR_ARG;
+  if (!rhs || !Qin || !Qout || !vgeo || !sgeo || !vmapM || !vmapP || !elemtobndy)
+    return CANARY_ERR_ARG;
+  if (Qin == (const void*)Qout) return CANARY_ERR_ARG;
+  canary::KArgs a = {};
+  a.rhs = rhs;
+  a.Q = Qin;
+  a.Qout = Qout;
+  a.vgeo = vgeo;
+  a.sgeo = sgeo;
+  a.nreal = nreal;
+  a.ntotal = nelem_total;
+  a.vmapM = vmapM;
+  a.vmapP = vmapP;
+  a.bndy = elemtobndy;
+  a.rka = rka;
+  a.rkb = rkb;
+  a.dt = dt;
+  a.phys = to_params(phys);
+  a.stream = (hipStream_t)stream;
+  return fn(canary::OP_FACE_UPDATE, N, &a);
 }
 
 extern "C" int canary_check_errors(void* stream, int64_t* info) {

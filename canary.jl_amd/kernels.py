@@ -137,6 +137,34 @@ def stage_fused(dim, N, rhs, Qin, Qout, vgeo, sgeo, D, elems, vmapM, vmapP, elem
     check(rc, "stage_fused")
 
 
+def facerhs_update(dim, N, rhs, Qin, Qout, vgeo, sgeo, elems, vmapM, vmapP, elemtobndy,
+                   rka, rkb, dt, eqn=EULER, phys=None, nelem_total=None, stream=None,
+                   prec=FP64):
+    """facerhs! then updatesolution! of one stage in one pass (after a
+    separate volumerhs); Qin -> Qout, rhs in place.  prec as for stage_fused."""
+    if prec not in (FP64, FP64_FMA):
+        raise ValueError("facerhs_update: prec must be FP64 or FP64_FMA")
+    n = _nreal(elems)
+    Np, Nfp, nf = (N + 1) ** dim, (N + 1) ** (dim - 1), 2 * dim
+    ns = _NS[(eqn, dim)]
+    ntot = n if nelem_total is None else int(nelem_total)
+    _need(rhs, "rhs", Np * ns * n)
+    _need(Qin, "Qin", Np * ns * ntot)
+    _need(Qout, "Qout", Np * ns * n)
+    _need(vgeo, "vgeo", Np * _NV[dim] * n)
+    _need(sgeo, "sgeo", _NSG[dim] * Nfp * nf * n)
+    _need(vmapM, "vmapM", Nfp * nf * n, torch.int64)
+    _need(vmapP, "vmapP", Nfp * nf * n, torch.int64)
+    _need(elemtobndy, "elemtobndy", nf * n, torch.int64)
+    if Qin.data_ptr() == Qout.data_ptr():
+        raise ValueError("facerhs_update needs distinct Qin and Qout")
+    rc = lib().canary_facerhs_update(eqn, dim, N, prec, _ptr(rhs), _ptr(Qin), _ptr(Qout),
+                                     _ptr(vgeo), _ptr(sgeo), n, ntot, _ptr(vmapM), _ptr(vmapP),
+                                     _ptr(elemtobndy), float(rka), float(rkb), float(dt),
+                                     _phys(phys), _stream(stream))
+    check(rc, "facerhs_update")
+
+
 def check_errors(stream=None):
     """Synchronise and raise if a kernel recorded a bad bc / vmap index."""
     info = (C.c_int64 * 4)()

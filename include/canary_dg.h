@@ -125,6 +125,19 @@ int canary_stage_fused(int eqn, int dim, int N, int prec, void* rhs, const void*
                        const int64_t* elemtobndy, double rka, double rkb, double dt,
                        const canary_phys* phys, void* stream);
 
+/* facerhs! ; updatesolution! of one stage in a single pass, after a
+ * separate canary_volumerhs (the opt-in face + update fusion):
+ *   equivalent to the facerhs! and updatesolution! calls of one iteration of
+ *   the stage loop (euler3d_set2c.jl:849-854), with Q ping-pong as in
+ *   canary_stage_fused (Qin != Qout; ghosts of Qin read, never written).
+ *   prec: CANARY_FP64 (bit-identical to canary_facerhs + canary_updatesolution)
+ *   or CANARY_FP64_FMA (contracted).  No SWE `advection` mode. */
+int canary_facerhs_update(int eqn, int dim, int N, int prec, void* rhs, const void* Qin,
+                          void* Qout, const void* vgeo, const void* sgeo, int64_t nreal,
+                          int64_t nelem_total, const int64_t* vmapM, const int64_t* vmapP,
+                          const int64_t* elemtobndy, double rka, double rkb, double dt,
+                          const canary_phys* phys, void* stream);
+
 /* ---- precision form (Euler set2c, dim 2 or 3, N = 1..7) -------------------
  * FP32 perturbation q' = Q - Q0 about an FP64 background Q0 at rest
  * (hydrostatic, or uniform without gravity); no reference counterpart: the

@@ -59,6 +59,13 @@ def test_device_entry_points_reject_bad_arguments_before_launch():
                                       None, None, None, 0.0, 0.0, 0.0, ctypes.byref(p), None) == L.ERR_ARG
     assert lib.canary_stage_fused(0, 3, 8, L.FP64_FMA, None, None, None, None, None, None, 1, 1,
                                   None, None, None, 0.0, 0.0, 0.0, ctypes.byref(p), None) == L.ERR_UNSUPPORTED
+    for prec in (L.FP64, L.FP64_FMA):
+        assert lib.canary_facerhs_update(0, 3, 4, prec, None, None, None, None, None, 0, 0, None,
+                                         None, None, 0.0, 0.0, 0.0, ctypes.byref(p), None) == 0
+        assert lib.canary_facerhs_update(1, 2, 4, prec, None, None, None, None, None, 3, 3, None,
+                                         None, None, 0.0, 0.0, 0.0, ctypes.byref(p), None) == L.ERR_ARG
+    assert lib.canary_facerhs_update(0, 3, 4, L.FP32, None, None, None, None, None, 3, 3, None,
+                                     None, None, 0.0, 0.0, 0.0, ctypes.byref(p), None) == L.ERR_UNSUPPORTED
 
 
 def test_phys_defaults_are_the_reference_rationals():

@@ -172,6 +172,34 @@ def test_fma_stage_parity(cuda, eqn, dim, N):
             assert np.array_equal(got_q[:, 3, :], Q[:, 3, :])  # b is never evolved
 
 
+@pytest.mark.parametrize("eqn,dim,N", [(0, 3, 4), (0, 3, 2), (0, 2, 5), (1, 2, 4), (1, 2, 7)])
+def test_facerhs_update_equals_face_then_update(cuda, eqn, dim, N):
+    """The face + update fusion after volumerhs: bit-identical to facerhs then
+    updatesolution; the contracted one within 1e-12."""
+    s = case_setup(eqn, dim, N, periodic=(True,) + (False,) * (dim - 1))
+    ds = DeviceSetup(s)
+    Q = np.asfortranarray(state(s, seed=8))
+    rhs0 = np.asfortranarray(np.random.default_rng(6).standard_normal(Q.shape) * 10)
+    if eqn == 1:
+        rhs0[:, 3, :] = 0.0
+    A, B, _ = P.rk_coefs()
+    q1, r1 = dev(Q), dev(rhs0)
+    K.volumerhs(dim, N, r1, q1, ds.vgeo, ds.D, s.nelem, eqn=eqn, phys=s.phys)
+    r2 = r1.clone()
+    K.facerhs(dim, N, r1, q1, ds.sgeo, s.nelem, ds.vmapM, ds.vmapP, ds.bndy, eqn=eqn, phys=s.phys)
+    K.updatesolution(dim, N, r1, q1, ds.vgeo, s.nelem, A[2], B[1], 0.02, eqn=eqn)
+    for prec in (FP64, FP64_FMA):
+        qin, qout, r = dev(Q), dev(np.zeros_like(Q)), r2.clone()
+        K.facerhs_update(dim, N, r, qin, qout, ds.vgeo, ds.sgeo, s.nelem, ds.vmapM, ds.vmapP,
+                         ds.bndy, A[2], B[1], 0.02, eqn=eqn, phys=s.phys, prec=prec)
+        K.check_errors()
+        if prec == FP64:
+            assert torch.equal(q1, qout) and torch.equal(r1, r)
+        else:
+            assert rel_l2(host(r, Q.shape), host(r1, Q.shape)) <= 1e-12
+            assert rel_l2(host(qout, Q.shape), host(q1, Q.shape)) <= 1e-12
+
+
 def run_gpu_lsrk(s, ds, Q, dt, nsteps, fused):
     A, B, _ = P.rk_coefs()
     q = dev(Q)
