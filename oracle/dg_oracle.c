@@ -22,6 +22,20 @@
  * (:251) is not reproduced; it does not change any value.
  * `nthreads` > 1 splits the element loops with OpenMP (each element only
  * writes its own rhs/Q entries, so the result is identical).
+ *
+ * How each function is pinned (the reference's own tests hold no vectors
+ * for these kernels; Julia cannot run here):
+ *   volumerhs / facerhs (all three equation sets): equal to an independent
+ *     dense strong-form assembly in NumPy on tiny meshes, free-stream
+ *     preservation, discrete conservation, buoyancy-only residual at rest,
+ *     wall mirror symmetry, bad-bc error (tests/test_oracle.py);
+ *   updatesolution: the formula on random data, the SWE velocity freeze;
+ *   RK tables and stage loop: the stability polynomial of LSRK54 on
+ *     y' = z y (1, 1, 1/2, 1/6, 1/24, 1/200) and h-convergence of the
+ *     isentropic vortex at order >= N + 1/2 (tests/test_convergence.py);
+ *   cfl, L2energysquared: closed forms on uniform states.
+ * Parity unpinned at the last ulp: libm pow (glibc here, Julia's libm in
+ * the reference) in the pressure; the 1e-12 / 1e-10 tolerances cover it.
  */
 #include <math.h>
 #include <stdint.h>
