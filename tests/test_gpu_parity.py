@@ -263,6 +263,38 @@ def test_invalid_bc_reported(cuda):
     K.check_errors()  # cleared
 
 
+@pytest.mark.parametrize("which", ["stage", "face_update", "batch"])
+def test_contracted_units_report_errors(cuda, which):
+    """A bad bc code met by a kernel of the contracted units (their own device
+    status words) reaches canary_check_errors like the exact-order ones."""
+    from canary_amd.batch import Batch
+    s = case_setup(0, 3, 2)
+    ds = DeviceSetup(s)
+    bad = s.elemtobndy.copy(order="F")
+    bad[3, 1] = 5
+    Q = dev(smooth_euler_state(s))
+    rhs = dev(s.zeros_state())
+    qout = torch.empty_like(Q)
+    if which == "stage":
+        K.stage_fused(3, 2, rhs, Q, qout, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM, ds.vmapP,
+                      dev(bad), 0.0, 0.1, 0.01, phys=s.phys, prec=FP64_FMA)
+    elif which == "face_update":
+        K.facerhs_update(3, 2, rhs, Q, qout, ds.vgeo, ds.sgeo, s.nelem, ds.vmapM, ds.vmapP,
+                         dev(bad), 0.0, 0.1, 0.01, phys=s.phys, prec=FP64_FMA)
+    else:
+        s.elemtobndy = bad
+        s.Q = np.asfortranarray(smooth_euler_state(s))
+        b = Batch(0, 3, FP64_FMA)
+        b.add(s, 0.01)
+        b.finalize()
+        b.run(1, graph=False)
+    torch.cuda.synchronize()  # the batch runs on a stream of its own
+    with pytest.raises(CanaryError) as ei:
+        K.check_errors()
+    assert ei.value.code == ERR_BC and ei.value.info[1:] == (4, 2, 5)
+    K.check_errors()  # cleared
+
+
 def test_vmap_out_of_range_reported(cuda):
     s = case_setup(0, 2, 3)
     ds = DeviceSetup(s)
