@@ -201,6 +201,27 @@ def main():
     extra["fma_vs_exact_rel_l2_after_%d_steps" % (args.warmup + args.steps)] = float(
         "%.3e" % (torch.linalg.vector_norm(qf - qx) / torch.linalg.vector_norm(qx)).item())
     del qe, qe2, re, hold_e, qx
+    # the library's native LSRK54 loop (canary_lsrk54), two steps per captured
+    # hipGraph, on a stream of its own: the same kernels without a host loop
+    from canary_amd.streams import OwnedStream
+    from canary_amd._lib import PATH_FUSED
+    own = OwnedStream(dev)
+    qn, qn2, rn = Q0.clone(), torch.empty_like(Q0), torch.zeros_like(Q0)
+    torch.cuda.synchronize()
+
+    def native(nsteps):
+        with torch.cuda.stream(own.stream):
+            return K.lsrk54(3, s.N, qn, rn, vgeo, sgeo, D, s.nelem, vM, vP, bnd, dt, nsteps,
+                            phys=s.phys, path=PATH_FUSED, prec=FP64_FMA, Qalt=qn2, graph=True)
+    native(2)
+    n0, n1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    n0.record(own.stream)
+    native(args.steps)
+    n1.record(own.stream)
+    torch.cuda.synchronize()
+    K.check_errors()
+    extra["native_loop_graph_gdof_stage_s"] = round(dof * stages / (n0.elapsed_time(n1) / 1e3) / 1e9, 3)
+    del qn, qn2, rn, own
     if not args.no_plain:
         qp, rp = Q0.clone(), torch.zeros_like(Q0)
         run_plain(qp, rp, 1)

@@ -16,6 +16,7 @@ ERR_NOCONVERGENCE, ERR_LOBATTO_N1, ERR_ORDERING, ERR_ORIENTATION = 6, 7, 8, 9
 ERR_NOMEM, ERR_CAPACITY = 10, 11
 EULER, SWE = 0, 1
 FP64, FP32, FP64_FMA = 0, 1, 2
+PATH_KERNELS, PATH_FUSED, PATH_FACE_UPDATE = 0, 1, 2
 REDUCE_WORK = 1024  # CANARY_REDUCE_WORK
 ENDPT_NEITHER, ENDPT_LEFT, ENDPT_RIGHT, ENDPT_BOTH = 0, 1, 2, 3
 
@@ -86,6 +87,9 @@ _SIGS = {
                                  D, D, D, C.POINTER(Phys), P]),
     "canary_facerhs_update": (INT, [INT, INT, INT, INT, P, P, P, P, P, I64, I64, P, P, P,
                                     D, D, D, C.POINTER(Phys), P]),
+    "canary_lsrk54": (INT, [INT, INT, INT, INT, INT, I64, D, P, P, P, P, P, P, I64, I64, P, P,
+                            P, C.POINTER(Phys), INT, INT, C.POINTER(INT), P]),
+    "canary_lsrk54_coefs": (None, [P, P, P]),
     "canary_check_errors": (INT, [P, P]),
     "canary_perturb32_prepare": (INT, [INT, INT, I64, I64, P, P, P, P, P, P, P, C.POINTER(Phys),
                                        P, P, P, P, P, P, P]),

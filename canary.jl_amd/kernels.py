@@ -165,6 +165,46 @@ def facerhs_update(dim, N, rhs, Qin, Qout, vgeo, sgeo, elems, vmapM, vmapP, elem
     check(rc, "facerhs_update")
 
 
+def lsrk54(dim, N, Q, rhs, vgeo, sgeo, D, elems, vmapM, vmapP, elemtobndy, dt, nsteps,
+           eqn=EULER, phys=None, nelem_total=None, path=_lib.PATH_FUSED, prec=FP64, Qalt=None,
+           advection=False, graph=False, stream=None):
+    """nsteps LSRK54 steps of one solve in the library's native time loop
+    (lowstorageRK).  path: PATH_KERNELS (three kernels per stage, Q in place),
+    PATH_FUSED or PATH_FACE_UPDATE (Q and Qalt ping-pong).  Returns the tensor
+    holding the final state (Q or Qalt)."""
+    n = _nreal(elems)
+    Np, Nfp, nf = (N + 1) ** dim, (N + 1) ** (dim - 1), 2 * dim
+    ns = _NS[(eqn, dim)]
+    ntot = n if nelem_total is None else int(nelem_total)
+    _need(Q, "Q", Np * ns * ntot)
+    _need(rhs, "rhs", Np * ns * n)
+    _need(vgeo, "vgeo", Np * _NV[dim] * n)
+    _need(sgeo, "sgeo", _NSG[dim] * Nfp * nf * n)
+    _need(D, "D", (N + 1) ** 2)
+    _need(vmapM, "vmapM", Nfp * nf * n, torch.int64)
+    _need(vmapP, "vmapP", Nfp * nf * n, torch.int64)
+    _need(elemtobndy, "elemtobndy", nf * n, torch.int64)
+    if path != _lib.PATH_KERNELS:
+        if Qalt is None:
+            raise ValueError("lsrk54: the ping-pong paths need Qalt")
+        _need(Qalt, "Qalt", Np * ns * ntot)
+    alt = C.c_int(0)
+    rc = lib().canary_lsrk54(eqn, dim, N, prec, path, int(nsteps), float(dt), _ptr(Q),
+                             _ptr(Qalt) if Qalt is not None else None, _ptr(rhs), _ptr(vgeo),
+                             _ptr(sgeo), _ptr(D), n, ntot, _ptr(vmapM), _ptr(vmapP),
+                             _ptr(elemtobndy), _phys(phys), 1 if advection else 0,
+                             1 if graph else 0, C.byref(alt), _stream(stream))
+    check(rc, "lsrk54")
+    return Qalt if alt.value else Q
+
+
+def lsrk54_coefs():
+    """(RKA, RKB, RKC) of the library's LSRK54 tables."""
+    a, b, c = (C.c_double * 5)(), (C.c_double * 5)(), (C.c_double * 5)()
+    lib().canary_lsrk54_coefs(a, b, c)
+    return tuple(a), tuple(b), tuple(c)
+
+
 def check_errors(stream=None):
     """Synchronise and raise if a kernel recorded a bad bc / vmap index."""
     info = (C.c_int64 * 4)()

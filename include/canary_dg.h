@@ -138,6 +138,31 @@ int canary_facerhs_update(int eqn, int dim, int N, int prec, void* rhs, const vo
                           const int64_t* elemtobndy, double rka, double rkb, double dt,
                           const canary_phys* phys, void* stream);
 
+/* ---- native time loop ---------------------------------------------------
+ * lowstorageRK (euler3d_set2c.jl:754-891, euler2d_set2c.jl:639-775,
+ * shallow_water.jl:615-747): nsteps LSRK54 steps of one solve through the
+ * entry points above, stage loop as the reference's (:819-855).
+ *   path CANARY_PATH_KERNELS: volumerhs, facerhs, updatesolution per stage,
+ *        Q updated in place (Qalt unused), prec CANARY_FP64 only; `advection`
+ *        (shallow water) as in updatesolution;
+ *   path CANARY_PATH_FUSED: canary_stage_fused, Q and Qalt ping-pong;
+ *   path CANARY_PATH_FACE_UPDATE: canary_volumerhs + canary_facerhs_update,
+ *        Q and Qalt ping-pong.
+ *   *result_in_alt: 1 if the final state is in Qalt, else 0 (in Q).
+ *   graph != 0 and a non-null stream: two steps are captured once into a
+ *   hipGraph and replayed (the call then returns after the steps ran). */
+#define CANARY_PATH_KERNELS 0
+#define CANARY_PATH_FUSED 1
+#define CANARY_PATH_FACE_UPDATE 2
+int canary_lsrk54(int eqn, int dim, int N, int prec, int path, int64_t nsteps, double dt,
+                  void* Q, void* Qalt, void* rhs, const void* vgeo, const void* sgeo,
+                  const double* D, int64_t nreal, int64_t nelem_total, const int64_t* vmapM,
+                  const int64_t* vmapP, const int64_t* elemtobndy, const canary_phys* phys,
+                  int advection, int graph, int* result_in_alt, void* stream);
+/* RKA, RKB, RKC of LSRK54 as the reference's rationals (euler3d_set2c.jl:772-788);
+ * any pointer may be NULL. */
+void canary_lsrk54_coefs(double* rka, double* rkb, double* rkc);
+
 /* ---- precision form (Euler set2c, dim 2 or 3, N = 1..7) -------------------
  * FP32 perturbation q' = Q - Q0 about an FP64 background Q0 at rest
  * (hydrostatic, or uniform without gravity); no reference counterpart: the

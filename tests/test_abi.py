@@ -68,6 +68,25 @@ def test_device_entry_points_reject_bad_arguments_before_launch():
                                      None, None, 0.0, 0.0, 0.0, ctypes.byref(p), None) == L.ERR_UNSUPPORTED
 
 
+def test_lsrk54_tables_match_the_host_ones():
+    import canary_amd.kernels as K
+    import canary_amd.problems as P
+    assert K.lsrk54_coefs() == P.rk_coefs()
+    lib = L.lib()
+    alt = ctypes.c_int(7)
+    p = L.default_phys()
+    # argument checks before any launch
+    assert lib.canary_lsrk54(0, 3, 4, 0, 1, 5, 0.1, None, None, None, None, None, None, 1, 1,
+                             None, None, None, ctypes.byref(p), 0, 0, ctypes.byref(alt),
+                             None) == L.ERR_ARG  # fused path needs Qalt
+    assert lib.canary_lsrk54(0, 3, 4, L.FP64_FMA, 0, 5, 0.1, None, None, None, None, None, None,
+                             1, 1, None, None, None, ctypes.byref(p), 0, 0, ctypes.byref(alt),
+                             None) == L.ERR_UNSUPPORTED  # three kernels are exact order only
+    assert lib.canary_lsrk54(0, 3, 4, 0, 0, 0, 0.1, None, None, None, None, None, None, 1, 1,
+                             None, None, None, ctypes.byref(p), 0, 0, ctypes.byref(alt),
+                             None) == 0 and alt.value == 0  # no steps
+
+
 def test_phys_defaults_are_the_reference_rationals():
     p = L.default_phys()
     assert (p.gamma, p.p0, p.R_gas, p.c_p, p.c_v, p.gravity) == (14 / 10, 100000.0, 28717 / 100,

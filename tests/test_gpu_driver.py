@@ -1,0 +1,102 @@
+"""The library's native LSRK54 loop (canary_lsrk54) on the GPU: the same
+final state, bit for bit, as the Python stage loop over the same entry
+points, for each path, eagerly and with its captured step graph; against the
+oracle's lowstorageRK within the task's tolerances."""
+import numpy as np
+import pytest
+import torch
+
+import canary_amd.kernels as K
+import canary_amd.problems as P
+from canary_amd._lib import FP64, FP64_FMA, PATH_FACE_UPDATE, PATH_FUSED, PATH_KERNELS
+from canary_amd.streams import OwnedStream
+from oracle import oracle as O
+
+from gpu_util import DeviceSetup, dev, host
+from states import max_ulp, rel_l2, small_setup, smooth_euler_state, smooth_swe_state
+
+pytestmark = pytest.mark.gpu
+
+
+def python_loop(s, ds, Q, dt, nsteps, path, prec):
+    A, B, _ = P.rk_coefs()
+    q, q2, rhs = dev(Q), dev(np.zeros_like(Q)), dev(np.zeros_like(Q))
+    for _ in range(nsteps):
+        for st in range(5):
+            a, b = A[(st + 1) % 5], B[st]
+            if path == PATH_KERNELS:
+                K.volumerhs(s.dim, s.N, rhs, q, ds.vgeo, ds.D, s.nelem, eqn=s.eqn, phys=s.phys)
+                K.facerhs(s.dim, s.N, rhs, q, ds.sgeo, s.nelem, ds.vmapM, ds.vmapP, ds.bndy,
+                          eqn=s.eqn, phys=s.phys)
+                K.updatesolution(s.dim, s.N, rhs, q, ds.vgeo, s.nelem, a, b, dt, eqn=s.eqn)
+                continue
+            if path == PATH_FUSED:
+                K.stage_fused(s.dim, s.N, rhs, q, q2, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM,
+                              ds.vmapP, ds.bndy, a, b, dt, eqn=s.eqn, phys=s.phys, prec=prec)
+            else:
+                K.volumerhs(s.dim, s.N, rhs, q, ds.vgeo, ds.D, s.nelem, eqn=s.eqn, phys=s.phys)
+                K.facerhs_update(s.dim, s.N, rhs, q, q2, ds.vgeo, ds.sgeo, s.nelem, ds.vmapM,
+                                 ds.vmapP, ds.bndy, a, b, dt, eqn=s.eqn, phys=s.phys, prec=prec)
+            q, q2 = q2, q
+    K.check_errors()
+    return host(q, Q.shape), host(rhs, Q.shape)
+
+
+def native_loop(s, ds, Q, dt, nsteps, path, prec, graph):
+    q, q2, rhs = dev(Q), dev(np.zeros_like(Q)), dev(np.zeros_like(Q))
+    st = OwnedStream(torch.device("cuda:0"))
+    cur = torch.cuda.current_stream()
+    st.stream.wait_stream(cur)
+    with torch.cuda.stream(st.stream):
+        out = K.lsrk54(s.dim, s.N, q, rhs, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM, ds.vmapP,
+                       ds.bndy, dt, nsteps, eqn=s.eqn, phys=s.phys, path=path, prec=prec,
+                       Qalt=q2, graph=graph)
+    cur.wait_stream(st.stream)
+    torch.cuda.synchronize()
+    K.check_errors()
+    return host(out, Q.shape), host(rhs, Q.shape)
+
+
+CASES = [(0, 3, 3), (0, 2, 4), (1, 2, 5)]
+
+
+@pytest.mark.parametrize("eqn,dim,N", CASES)
+@pytest.mark.parametrize("path,prec", [(PATH_KERNELS, FP64), (PATH_FUSED, FP64),
+                                       (PATH_FUSED, FP64_FMA), (PATH_FACE_UPDATE, FP64)])
+@pytest.mark.parametrize("graph", [False, True])
+def test_native_loop_equals_python_loop(cuda, eqn, dim, N, path, prec, graph):
+    print("driver eqn=%d dim=%d N=%d path=%d prec=%d graph=%s" % (eqn, dim, N, path, prec, graph),
+          flush=True)
+    s = small_setup(eqn, dim, N, periodic=(True,) * dim)
+    ds = DeviceSetup(s)
+    Q = np.asfortranarray(smooth_swe_state(s, seed=2) if eqn == 1 else smooth_euler_state(s, seed=2))
+    dt = 1e-4 if eqn == 1 else 0.01
+    nsteps = 5  # odd: the graph covers two steps, the last one runs eagerly
+    want_q, want_r = python_loop(s, ds, Q, dt, nsteps, path, prec)
+    got_q, got_r = native_loop(s, ds, Q, dt, nsteps, path, prec, graph)
+    assert max_ulp(got_q, want_q) == 0 and max_ulp(got_r, want_r) == 0
+
+
+def test_native_loop_matches_oracle_lowstoragerk(cuda):
+    s = P.bubble3d(Ne=4, N=4, dtheta=2.0, jitter=1e-3, seed=3)
+    ds = DeviceSetup(s)
+    dt, nsteps = 0.02, 40
+    Qo, ro = np.asfortranarray(s.Q.copy()), s.zeros_state()
+    assert O.lsrk(s, Qo, ro, dt, nsteps, nthreads=8) == 0
+    for prec in (FP64, FP64_FMA):
+        got, _ = native_loop(s, ds, s.Q, dt, nsteps, PATH_FUSED, prec, True)
+        e = rel_l2(got, Qo)
+        print("native loop (prec %d) vs oracle after %d steps: %.3e" % (prec, nsteps, e), flush=True)
+        assert e <= 1e-10
+
+
+def test_native_loop_swe_advection(cuda):
+    s = P.swe2d(Ne=5, N=3, bumps=((0.4, 0.5, 0.05, 0.1),))
+    ds = DeviceSetup(s)
+    Qo, ro = np.asfortranarray(s.Q.copy()), s.zeros_state()
+    assert O.lsrk(s, Qo, ro, 1e-4, 7, advection=True) == 0
+    q, rhs = dev(s.Q), dev(s.zeros_state())
+    out = K.lsrk54(2, 3, q, rhs, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM, ds.vmapP, ds.bndy,
+                   1e-4, 7, eqn=1, phys=s.phys, path=PATH_KERNELS, advection=True)
+    K.check_errors()
+    assert max_ulp(host(out, s.Q.shape), Qo) == 0
