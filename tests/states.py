@@ -77,3 +77,30 @@ def max_ulp(a, b):
     a = np.where(a < 0, np.int64(-0x8000000000000000) - a, a)
     b = np.where(b < 0, np.int64(-0x8000000000000000) - b, b)
     return int(np.max(np.abs(a - b))) if a.size else 0
+
+
+def warped_setup(eqn, dim, N, Ne=3, amp=0.15, seed=0, periodic=None, gravity=9.81):
+    """small_setup with every interior vertex moved by up to amp of the
+    element size (vertices on the domain boundary stay, so periodic faces
+    still match): trilinear, non-affine elements, geometry recomputed."""
+    import canary_amd.mesh as M
+    s = small_setup(eqn, dim, N, Ne=Ne, periodic=periodic, gravity=gravity)
+    etc = np.array(s.mesh["elemtocoord"], order="F", copy=True)
+    lo = etc.min(axis=(1, 2))
+    hi = etc.max(axis=(1, 2))
+    h = np.ptp(etc[:, :, 0], axis=1).min()
+    rng = np.random.default_rng(seed)
+    moved = {}
+    for e in range(etc.shape[2]):
+        for v in range(etc.shape[1]):
+            x = etc[:, v, e]
+            if np.any(np.isclose(x, lo)) or np.any(np.isclose(x, hi)):
+                continue
+            k = tuple(np.round(x, 9))
+            if k not in moved:
+                moved[k] = rng.uniform(-amp, amp, size=x.size) * h
+            etc[:, v, e] = x + moved[k]
+    assert moved, "no interior vertex to move"
+    s.mesh["elemtocoord"] = etc
+    s.vgeo, s.sgeo = M.computegeometry(dim, N, etc, s.vmapM)
+    return s

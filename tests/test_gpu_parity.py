@@ -22,7 +22,7 @@ from oracle import oracle as O
 
 from gpu_util import DeviceSetup, dev, host
 from states import (max_ulp, pow_free_euler_state, rel_l2, small_setup, smooth_euler_state,
-                    smooth_swe_state)
+                    smooth_swe_state, warped_setup)
 
 pytestmark = pytest.mark.gpu
 
@@ -69,6 +69,32 @@ def test_rhs_parity(cuda, eqn, dim, N):
             assert max_ulp(got, want) == 0, "expected bit-identical results"
         else:
             assert rel_l2(got, want) <= 1e-12
+
+
+@pytest.mark.parametrize("eqn,dim,N", [(0, 3, 4), (0, 3, 3), (0, 2, 5), (1, 2, 4)])
+def test_rhs_parity_warped(cuda, eqn, dim, N):
+    """Non-affine elements (every metric term and normal component nonzero):
+    the same bars as on bricks, for the three kernels and both fused forms."""
+    s = warped_setup(eqn, dim, N, Ne=3, periodic=(True,) * dim, seed=N)
+    ds = DeviceSetup(s)
+    for powfree in ((False, True) if eqn == 0 else (False,)):
+        Q = np.asfortranarray(state(s, seed=21, powfree=powfree))
+        want = O.rhs_eval(s, Q, s.zeros_state())
+        got = gpu_rhs(s, ds, Q, s.zeros_state())
+        if powfree or eqn == 1:
+            assert max_ulp(got, want) == 0
+        else:
+            assert rel_l2(got, want) <= 1e-12
+        for prec in (FP64, FP64_FMA):
+            qin, qout, r = dev(Q), dev(np.zeros_like(Q)), dev(s.zeros_state())
+            K.stage_fused(dim, N, r, qin, qout, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM,
+                          ds.vmapP, ds.bndy, 1.0, 0.5, 0.01, eqn=eqn, phys=s.phys, prec=prec)
+            K.check_errors()
+            g = host(r, Q.shape)
+            if prec == FP64 and (powfree or eqn == 1):
+                assert max_ulp(g, want) == 0
+            else:
+                assert rel_l2(g, want) <= 1e-12
 
 
 @pytest.mark.parametrize("eqn,dim,N", [(0, 3, 4), (0, 2, 4), (1, 2, 4)])

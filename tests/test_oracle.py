@@ -15,7 +15,7 @@ import canary_amd.problems as P
 from oracle import oracle as O
 
 from dg_numpy import rhs_strong
-from states import rel_l2, small_setup, smooth_euler_state, smooth_swe_state
+from states import rel_l2, small_setup, smooth_euler_state, smooth_swe_state, warped_setup
 
 
 def oracle_rhs(s, Q):
@@ -53,6 +53,25 @@ def test_free_stream_preservation(dim):
     r = oracle_rhs(s, Q)
     scale = np.max(np.abs(s.vgeo[:, 9 if dim == 3 else 4, :])) * 1e5 * 2 / 500.0
     assert np.max(np.abs(r)) < 1e-11 * scale
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_free_stream_preservation_warped(dim):
+    """On non-affine (trilinear) elements every metric term is nonzero, so
+    a mixed-up metric index or normal component breaks the cancellation."""
+    s = warped_setup(0, dim, 4, Ne=3, periodic=(True,) * dim, gravity=0.0)
+    Q = s.zeros_state()
+    Q[:, dim, :] = 1.2
+    Q[:, :dim, :] = 1.2 * np.array([5.0, -3.0, 2.0][:dim])[None, :, None]
+    Q[:, dim + 1, :] = 1.2 * 300.0
+    r = oracle_rhs(s, Q)
+    scale = np.max(np.abs(s.vgeo[:, 9 if dim == 3 else 4, :])) * 1e5 * 2 / 500.0
+    assert np.max(np.abs(r)) < 1e-10 * scale
+    # the same state with two metric fields swapped does not cancel
+    vg = np.array(s.vgeo, order="F", copy=True)
+    vg[:, [1, 2]] = s.vgeo[:, [2, 1]]
+    s.vgeo = vg
+    assert np.max(np.abs(oracle_rhs(s, Q))) > 1e-6 * scale
 
 
 @pytest.mark.parametrize("dim", [2, 3])
