@@ -591,8 +591,9 @@ def measured_traffic(tag):
 def cpu_baseline(s, sample_elems):
     """The oracle (a C restatement of the reference's Julia CPU kernels) on the
     host: one LSRK stage (volumerhs!, facerhs!, updatesolution!) over the first
-    `sample_elems` elements of the same mesh, single thread, timed like the
-    reference (best of 3 after a warm-up)."""
+    `sample_elems` elements of the same mesh, single thread and on up to 16
+    threads, median of 5 after a warm-up; the host's CPU model and the
+    threads it offers this process are reported beside it."""
     import numpy as np
     from oracle import oracle as O
     n = min(sample_elems, s.nelem)
@@ -612,7 +613,18 @@ def cpu_baseline(s, sample_elems):
     ncpu = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 8)
     stage_time(ncpu)
     tn = sorted(stage_time(ncpu) for _ in range(5))[2]
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     return {"value": round(n * s.Np * 5 / t1 / 1e9, 5), "unit": "GDOF-stage/s", "cores": 1,
+            "cpu_model": model, "threads_available": avail,
             "kind": "port",
             "sample": "LSRK stages (volumerhs!, facerhs!, updatesolution!) over %d of %d "
                       "elements, median of 5 after a warm-up, single thread; C restatement of "
