@@ -18,16 +18,18 @@
 namespace canary {
 namespace batch {
 
-constexpr int MAXT = 256;  // threads per workgroup ceiling
+constexpr int MAXT = 256;  // threads per workgroup ceiling for several elements
 
 // Candidate `WHICH` of elements per workgroup for (DIM, NQ): 0 = the most
-// elements that fit MAXT threads, 1 = about half of that, 2 = the
-// single-solve choice (first lane fill >= 90%).  0 = no such candidate.
+// elements that fit MAXT threads (one element, with as many threads as it
+// has nodes, when it alone exceeds MAXT: 3D N = 6, 7), 1 = about half of
+// that, 2 = the single-solve choice (first lane fill >= 90%).  0 = no such
+// candidate.
 template <int DIM, int NQ, int WHICH>
 constexpr int epw_candidate() {
   constexpr int Np = ipow(NQ, DIM);
   constexpr int emax = MAXT / Np;
-  if (WHICH == 0) return emax;
+  if (WHICH == 0) return emax > 0 ? emax : 1;  // more than MAXT nodes: one element alone
   if (WHICH == 1) return emax >= 2 ? (emax + 1) / 2 : 0;
   if (WHICH == 2) {
     constexpr int d = default_epw<Np>();
@@ -141,24 +143,14 @@ int run(int N, int which, int want, int* epw, int* threads, const canary_batch_s
   case NN:                                                                                    \
     return run_nq<Phys, NN + 1>(which, want, epw, threads, solves, items, nitems, parity, rka, \
                                 rkb, st);
-  if constexpr (Phys::DIM == 3) {
-    switch (N) {
-      CANARY_BATCH_N(1)
-      CANARY_BATCH_N(2)
-      CANARY_BATCH_N(3)
-      CANARY_BATCH_N(4)
-      CANARY_BATCH_N(5)
-    }
-  } else {
-    switch (N) {
-      CANARY_BATCH_N(1)
-      CANARY_BATCH_N(2)
-      CANARY_BATCH_N(3)
-      CANARY_BATCH_N(4)
-      CANARY_BATCH_N(5)
-      CANARY_BATCH_N(6)
-      CANARY_BATCH_N(7)
-    }
+  switch (N) {
+    CANARY_BATCH_N(1)
+    CANARY_BATCH_N(2)
+    CANARY_BATCH_N(3)
+    CANARY_BATCH_N(4)
+    CANARY_BATCH_N(5)
+    CANARY_BATCH_N(6)
+    CANARY_BATCH_N(7)
   }
 #undef CANARY_BATCH_N
   return CANARY_ERR_UNSUPPORTED;

@@ -470,7 +470,7 @@ template <int DIM, int NQ, int WHICH>
 constexpr int epw_candidate() {
   constexpr int Np = ipow(NQ, DIM);
   constexpr int emax = MAXT / Np;
-  if (WHICH == 0) return emax;
+  if (WHICH == 0) return emax > 0 ? emax : 1;  // more than MAXT nodes: one element alone
   if (WHICH == 1) return emax >= 2 ? (emax + 1) / 2 : 0;
   if (WHICH == 2) return single_epw<DIM, NQ>() * Np <= MAXT ? single_epw<DIM, NQ>() : 0;
   return 0;
@@ -523,8 +523,7 @@ int batch_nq(int which, int want, int* epw, int* threads, const canary_p32_solve
 }  // namespace p32
 }  // namespace canary
 
-// N -> NQ dispatch, N = 1..7 (the batched variants of 3D N >= 6 do not
-// exist: one element has more than 256 nodes)
+// N -> NQ dispatch, N = 1..7
 #define P32_DISPATCH(DIM_, N, FN, ARGS)                    \
   if ((DIM_) == 3) {                                       \
     switch (N) {                                           \

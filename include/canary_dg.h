@@ -236,7 +236,7 @@ int canary_batch_stage(int eqn, int dim, int prec, int N, int epw,
 
 /* Batched precision form: one solve of the table is the set of arrays of
  * the precision form above (q[parity] -> q[1-parity]); items as for
- * canary_batch_stage.  3D needs N <= 5 (Np <= 256). */
+ * canary_batch_stage. */
 typedef struct canary_p32_solve {
   void* q[2];
   void* rhs;

@@ -53,7 +53,7 @@ def euler2d_mix():
 
 def euler3d_mix():
     out = []
-    for N, Ne, seed in ((2, 3, 1), (3, 2, 2), (4, 2, 3), (5, 2, 4)):
+    for N, Ne, seed in ((2, 3, 1), (3, 2, 2), (4, 2, 3), (5, 2, 4), (6, 2, 5), (7, 1, 6)):
         s = P.bubble3d(Ne=Ne, N=N, jitter=1e-3, seed=seed)
         s.Q = np.asfortranarray(smooth_euler_state(s, seed=seed))
         out.append((s, P.cfl(s, s.Q) / s.N ** np.sqrt(2)))

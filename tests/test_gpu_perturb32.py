@@ -93,7 +93,7 @@ def test_perturb32_vortex2d_tracks_fp64(cuda, N):
 
 def p32_mix():
     out = []
-    for Ne, N, seed in ((3, 3, 1), (2, 4, 2), (4, 2, 3), (2, 5, 4)):
+    for Ne, N, seed in ((3, 3, 1), (2, 4, 2), (4, 2, 3), (2, 5, 4), (2, 6, 5), (1, 7, 6)):
         s = P.bubble3d(Ne=Ne, N=N, dtheta=2.0, jitter=1e-3, seed=seed)
         out.append((s, P.cfl(s, s.Q) / s.N ** np.sqrt(2)))
     return out
