@@ -222,6 +222,11 @@ def main():
     K.check_errors()
     extra["native_loop_graph_gdof_stage_s"] = round(dof * stages / (n0.elapsed_time(n1) / 1e3) / 1e9, 3)
     del qn, qn2, rn, own
+    # one GPU's share of the mesh split over 8 ranks (contiguous Hilbert
+    # chunks, ghosts appended): the stage at that real shape, and the halo
+    # pack (fillsendQ) + unpack (transferrecvQ) a stage would add; the
+    # exchange itself is not run (one GPU)
+    extra["rank_share_of_8"] = rank_share_form(s, K, P, dev, dt, args.steps, FP64_FMA)
     if not args.no_plain:
         qp, rp = Q0.clone(), torch.zeros_like(Q0)
         run_plain(qp, rp, 1)
@@ -297,6 +302,59 @@ def main():
         **extra,
     }
     print(json.dumps(line), flush=True)
+
+
+def rank_share_form(s, K, P, dev, dt, steps, prec, nranks=8, rank=0):
+    import numpy as np
+    import torch
+    sh = P.rank_share(s, nranks, rank)
+    Np, ns = s.Np, s.nstate
+
+    def d(a):
+        return torch.from_numpy(np.ascontiguousarray(np.asarray(a).reshape(-1, order="F"))).to(dev)
+
+    vg, sg, D = d(sh.vgeo), d(sh.sgeo), d(s.D)
+    vM, vP, bd = d(sh.vmapM), d(sh.vmapP), d(sh.elemtobndy)
+    q = d(sh.local_state(s.Q))
+    q2, rhs = torch.empty_like(q), torch.zeros(Np * ns * sh.nreal, dtype=torch.float64, device=dev)
+    se = d(sh.sendelems)
+    sendQ = torch.empty(Np * ns * sh.sendelems.size, dtype=torch.float64, device=dev)
+    recvQ = q.view(-1, Np * ns)[sh.nreal:].clone().reshape(-1)
+    A, B, _ = P.rk_coefs()
+
+    def run(n, halo):
+        a, b = q, q2
+        for _ in range(n):
+            for k in range(5):
+                if halo:
+                    K.fillsendQ(3, s.N, ns, sendQ, a, se, sh.nelem_total)
+                    K.transferrecvQ(3, s.N, ns, a, recvQ, sh.nreal)
+                K.stage_fused(3, s.N, rhs, a, b, vg, sg, D, sh.nreal, vM, vP, bd,
+                              A[(k + 1) % 5], B[k], dt, phys=s.phys, nelem_total=sh.nelem_total,
+                              prec=prec)
+                a, b = b, a
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3
+
+    run(1, False)
+    t = timed(lambda: run(steps, False)) / (5 * steps)
+    th = timed(lambda: run(steps, True)) / (5 * steps)
+    K.check_errors()
+    dof = sh.nreal * Np * ns
+    return {"ranks": nranks, "rank": rank, "real_elements": int(sh.nreal),
+            "ghost_elements": int(sh.nelem_total - sh.nreal),
+            "send_elements": int(sh.sendelems.size),
+            "stage_us": round(t * 1e6, 2), "gdof_stage_s": round(dof / t / 1e9, 3),
+            "stage_with_pack_unpack_us": round(th * 1e6, 2),
+            "halo_bytes_per_stage_whole_elements": int((sh.sendelems.size + sh.nelem_total
+                                                        - sh.nreal) * Np * ns * 8)}
 
 
 def precision_form(ne, N, steps_csv, timed_steps=10):

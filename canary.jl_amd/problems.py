@@ -408,6 +408,67 @@ def swe_ensemble(n=64, seed=42, ne=(8, 64), orders=(3, 4, 5, 7), depth=10.0,
     return out
 
 
+@dataclass
+class RankShare:
+    """One rank's share of a mesh split into contiguous element chunks of the
+    (Hilbert) element order, the reference's partition (src/mesh.jl
+    partition): the real elements first, then the ghost elements their faces
+    reach, as the reference appends them after realelems (src/mesh.jl
+    :944-955).  Maps are renumbered locally; vgeo/sgeo hold the real
+    elements only (ghosts are read through vmapP, never computed)."""
+    nreal: int
+    nelem_total: int
+    real: np.ndarray     # global element index of each real element
+    ghost: np.ndarray    # global element index of each ghost element, ascending
+    vgeo: np.ndarray
+    sgeo: np.ndarray
+    vmapM: np.ndarray
+    vmapP: np.ndarray
+    elemtobndy: np.ndarray
+    sendelems: np.ndarray  # local indices of real elements some other rank reads (1-based)
+
+    def local_state(self, Q):
+        """(Np, nstate, nreal + nghost) state of this share from a global one."""
+        return np.asfortranarray(np.concatenate([Q[:, :, self.real], Q[:, :, self.ghost]], axis=2))
+
+
+def rank_share(s, nranks, rank):
+    """The share of rank `rank` when setup `s` is split into `nranks`
+    contiguous chunks (sizes differing by at most one, the reference's
+    linearpartition)."""
+    ne, Np = s.nelem, s.Np
+    lo = rank * ne // nranks
+    hi = (rank + 1) * ne // nranks
+    real = np.arange(lo, hi)
+    nreal = hi - lo
+    vmapP = np.asarray(s.vmapP)[..., lo:hi]
+    eP = (vmapP - 1) // Np
+    nodeP = (vmapP - 1) % Np
+    outside = (eP < lo) | (eP >= hi)
+    ghost = np.unique(eP[outside])
+    local = np.full(ne, -1, dtype=np.int64)
+    local[real] = np.arange(nreal)
+    local[ghost] = nreal + np.arange(ghost.size)
+    vmapP_l = np.asfortranarray(local[eP] * Np + nodeP + 1)
+    vmapM = np.asarray(s.vmapM)[..., lo:hi]
+    vmapM_l = np.asfortranarray(vmapM - lo * Np)
+    # the real elements some other rank's ghosts are (what a send buffer packs)
+    sends = []
+    for r in range(nranks):
+        if r == rank:
+            continue
+        rlo, rhi = r * ne // nranks, (r + 1) * ne // nranks
+        ePr = (np.asarray(s.vmapP)[..., rlo:rhi] - 1) // Np
+        sends.append(ePr[(ePr >= lo) & (ePr < hi)])
+    sendelems = (np.unique(np.concatenate(sends)) - lo + 1) if sends else np.zeros(0, np.int64)
+    return RankShare(nreal=nreal, nelem_total=nreal + ghost.size, real=real, ghost=ghost,
+                     vgeo=np.asfortranarray(np.asarray(s.vgeo)[..., lo:hi]),
+                     sgeo=np.asfortranarray(np.asarray(s.sgeo)[..., lo:hi]),
+                     vmapM=vmapM_l, vmapP=vmapP_l,
+                     elemtobndy=np.asfortranarray(np.asarray(s.elemtobndy)[..., lo:hi]),
+                     sendelems=sendelems.astype(np.int64))
+
+
 def rk_coefs():
     """LSRK54 (Carpenter & Kennedy 1994) as DFloat(num)/DFloat(den):
     euler3d_set2c.jl:772-788."""

@@ -1,0 +1,65 @@
+"""A mesh split into rank shares on one GPU (the reference's partition with
+ghost elements appended): each share's send buffer packed with fillsendQ,
+the "exchange" emulated by gathering the senders' packed elements into each
+receiver's ghost order, unpacked with transferrecvQ, then one fused stage per
+share.  Every real element must come out bit-identical to the single-mesh
+stage, and ghost slots of Qout are never written."""
+import numpy as np
+import pytest
+import torch
+
+import canary_amd.kernels as K
+import canary_amd.problems as P
+from canary_amd._lib import FP64, FP64_FMA
+
+from gpu_util import dev, host
+from states import max_ulp, small_setup, smooth_euler_state
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dim,N,nranks", [(3, 3, 4), (2, 4, 3), (3, 4, 8)])
+@pytest.mark.parametrize("prec", [FP64, FP64_FMA])
+def test_rank_shares_reproduce_the_single_mesh_stage(cuda, dim, N, nranks, prec):
+    s = small_setup(0, dim, N, Ne=3, periodic=(True,) + (False,) * (dim - 1))
+    Np, ns = s.Np, s.nstate
+    Q = np.asfortranarray(smooth_euler_state(s, seed=4))
+    rhs0 = np.asfortranarray(np.random.default_rng(1).standard_normal(Q.shape))
+    A, B, _ = P.rk_coefs()
+    args = (A[2], B[1], 0.01)
+    # the single mesh
+    q, qo, r = dev(Q), dev(np.zeros_like(Q)), dev(rhs0)
+    K.stage_fused(dim, N, r, q, qo, dev(s.vgeo), dev(s.sgeo), dev(s.D), s.nelem, dev(s.vmapM),
+                  dev(s.vmapP), dev(s.elemtobndy), *args, phys=s.phys, prec=prec)
+    K.check_errors()
+    want_q, want_r = host(qo, Q.shape), host(r, Q.shape)
+    shares = [P.rank_share(s, nranks, k) for k in range(nranks)]
+    # every share packs what the others read of it
+    send = []
+    for sh in shares:
+        ql = dev(sh.local_state(Q))
+        se = dev(sh.sendelems)
+        buf = torch.empty(Np * ns * max(1, sh.sendelems.size), dtype=torch.float64,
+                          device=ql.device)
+        if sh.sendelems.size:
+            K.fillsendQ(dim, N, ns, buf, ql, se, sh.nelem_total)
+        send.append(buf.view(-1, Np * ns))
+    where = {}
+    for k, sh in enumerate(shares):
+        for j, le in enumerate(sh.sendelems):
+            where[int(sh.real[le - 1])] = (k, j)
+    for sh in shares:
+        ql = dev(sh.local_state(Q))
+        ql.view(-1, Np * ns)[sh.nreal:] = float("nan")  # ghosts arrive only through the buffers
+        recv = torch.stack([send[where[int(g)][0]][where[int(g)][1]] for g in sh.ghost])
+        K.transferrecvQ(dim, N, ns, ql, recv.reshape(-1), sh.nreal)
+        qol = torch.full_like(ql, -7.0)
+        rl = dev(np.asfortranarray(rhs0[:, :, sh.real]))
+        K.stage_fused(dim, N, rl, ql, qol, dev(sh.vgeo), dev(sh.sgeo), dev(s.D), sh.nreal,
+                      dev(sh.vmapM), dev(sh.vmapP), dev(sh.elemtobndy), *args, phys=s.phys,
+                      nelem_total=sh.nelem_total, prec=prec)
+        K.check_errors()
+        got = host(qol, (Np, ns, sh.nelem_total))
+        assert max_ulp(got[:, :, :sh.nreal], want_q[:, :, sh.real]) == 0
+        assert np.all(got[:, :, sh.nreal:] == -7.0)
+        assert max_ulp(host(rl, (Np, ns, sh.nreal)), want_r[:, :, sh.real]) == 0

@@ -1,0 +1,39 @@
+"""One rank's share of a mesh (contiguous chunks of the Hilbert element
+order, ghosts appended after the real elements): the renumbered maps point
+at the same nodes as the global ones, ghosts are exactly the elements of
+other ranks that the faces reach, and the send lists are the other ranks'
+ghosts (no GPU)."""
+import numpy as np
+import pytest
+
+import canary_amd.problems as P
+
+from states import small_setup
+
+
+@pytest.mark.parametrize("dim,nranks", [(2, 3), (3, 4), (3, 8)])
+def test_rank_shares_cover_the_mesh(dim, nranks):
+    s = small_setup(0, dim, 2, Ne=3, periodic=(True,) + (False,) * (dim - 1))
+    shares = [P.rank_share(s, nranks, r) for r in range(nranks)]
+    assert np.array_equal(np.concatenate([sh.real for sh in shares]), np.arange(s.nelem))
+    Np = s.Np
+    for r, sh in enumerate(shares):
+        glob = np.concatenate([sh.real, sh.ghost])  # local element -> global element
+        assert sh.nelem_total == glob.size and len(set(glob)) == glob.size
+        assert np.all((sh.ghost < sh.real[0]) | (sh.ghost > sh.real[-1]))
+        # local maps -> global node ids equal the global maps of these elements
+        for loc, ref in ((sh.vmapP, s.vmapP), (sh.vmapM, s.vmapM)):
+            e_l, n_l = (loc - 1) // Np, (loc - 1) % Np
+            assert loc.min() >= 1 and loc.max() <= sh.nelem_total * Np
+            assert np.array_equal(glob[e_l] * Np + n_l + 1, np.asarray(ref)[..., sh.real])
+        # every ghost is reached by some face
+        assert set(sh.ghost) == set(((np.asarray(s.vmapP)[..., sh.real] - 1) // Np).ravel()) - set(sh.real)
+        # what this rank sends is what the others hold as ghosts of it
+        others = set()
+        for q, shq in enumerate(shares):
+            if q != r:
+                others |= set(shq.ghost) & set(sh.real)
+        assert set(sh.real[sh.sendelems - 1]) == others
+    x = np.random.default_rng(0).standard_normal((Np, 4 if dim == 2 else 5, s.nelem))
+    loc = shares[1].local_state(x)
+    assert np.array_equal(loc[:, :, :shares[1].nreal], x[:, :, shares[1].real])
