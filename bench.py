@@ -320,15 +320,21 @@ def rank_share_form(s, K, P, dev, dt, steps, prec, nranks=8, rank=0):
     se = d(sh.sendelems)
     sendQ = torch.empty(Np * ns * sh.sendelems.size, dtype=torch.float64, device=dev)
     recvQ = q.view(-1, Np * ns)[sh.nreal:].clone().reshape(-1)
+    sn, rn = d(sh.send_nodes), d(sh.recv_nodes)
+    sendT = torch.empty(sh.send_nodes.size * ns, dtype=torch.float64, device=dev)
+    recvT = torch.zeros(sh.recv_nodes.size * ns, dtype=torch.float64, device=dev)
     A, B, _ = P.rk_coefs()
 
     def run(n, halo):
         a, b = q, q2
         for _ in range(n):
             for k in range(5):
-                if halo:
+                if halo == "elements":
                     K.fillsendQ(3, s.N, ns, sendQ, a, se, sh.nelem_total)
                     K.transferrecvQ(3, s.N, ns, a, recvQ, sh.nreal)
+                elif halo == "traces":
+                    K.pack_nodes(3, s.N, ns, sendT, a, sn, sh.nelem_total)
+                    K.unpack_nodes(3, s.N, ns, a, recvT, rn, sh.nelem_total)
                 K.stage_fused(3, s.N, rhs, a, b, vg, sg, D, sh.nreal, vM, vP, bd,
                               A[(k + 1) % 5], B[k], dt, phys=s.phys, nelem_total=sh.nelem_total,
                               prec=prec)
@@ -343,9 +349,10 @@ def rank_share_form(s, K, P, dev, dt, steps, prec, nranks=8, rank=0):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / 1e3
 
-    run(1, False)
-    t = timed(lambda: run(steps, False)) / (5 * steps)
-    th = timed(lambda: run(steps, True)) / (5 * steps)
+    run(1, None)
+    t = timed(lambda: run(steps, None)) / (5 * steps)
+    th = timed(lambda: run(steps, "elements")) / (5 * steps)
+    tt = timed(lambda: run(steps, "traces")) / (5 * steps)
     K.check_errors()
     dof = sh.nreal * Np * ns
     return {"ranks": nranks, "rank": rank, "real_elements": int(sh.nreal),
@@ -354,7 +361,10 @@ def rank_share_form(s, K, P, dev, dt, steps, prec, nranks=8, rank=0):
             "stage_us": round(t * 1e6, 2), "gdof_stage_s": round(dof / t / 1e9, 3),
             "stage_with_pack_unpack_us": round(th * 1e6, 2),
             "halo_bytes_per_stage_whole_elements": int((sh.sendelems.size + sh.nelem_total
-                                                        - sh.nreal) * Np * ns * 8)}
+                                                        - sh.nreal) * Np * ns * 8),
+            "stage_with_trace_pack_unpack_us": round(tt * 1e6, 2),
+            "halo_bytes_per_stage_face_traces": int((sh.send_nodes.size + sh.recv_nodes.size)
+                                                    * ns * 8)}
 
 
 def precision_form(ne, N, steps_csv, timed_steps=10):

@@ -90,6 +90,8 @@ _SIGS = {
     "canary_lsrk54": (INT, [INT, INT, INT, INT, INT, I64, D, P, P, P, P, P, P, I64, I64, P, P,
                             P, C.POINTER(Phys), INT, INT, C.POINTER(INT), P]),
     "canary_lsrk54_coefs": (None, [P, P, P]),
+    "canary_pack_nodes": (INT, [INT, INT, INT, P, P, P, I64, I64, INT, P]),
+    "canary_unpack_nodes": (INT, [INT, INT, INT, P, P, P, I64, I64, INT, P]),
     "canary_check_errors": (INT, [P, P]),
     "canary_perturb32_prepare": (INT, [INT, INT, I64, I64, P, P, P, P, P, P, P, C.POINTER(Phys),
                                        P, P, P, P, P, P, P]),

@@ -3,6 +3,8 @@
 //
 //   fillsendQ     sendQ[:, :, e] = Q[:, :, sendelems[e]]   (euler3d_set2c.jl:645-659)
 //   transferrecvQ Q[:, :, nrealelem + e] = recvQ[:, :, e]  (euler3d_set2c.jl:663-676)
+//   pack_nodes / unpack_nodes: the same for a list of single nodes (the face
+//                 traces a neighbour reads through vmapP), buf[k, s] = Q[node_k, s]
 //   l2energysquared  sum_e sum_s sum_i MJ q^2               (euler3d_set2c.jl:738-750)
 //   cfl              min over nodes of the local step       (euler3d_set2c.jl:67-97)
 //   courant          Courant-number step of the Euler sets (no reference counterpart)
@@ -59,6 +61,31 @@ __global__ __launch_bounds__(256) void recv_kernel(W* __restrict__ Q, const W* _
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
        g += (int64_t)gridDim.x * blockDim.x)
     Q[offset + g] = recvQ[g];
+}
+
+// node ids are 1-based over the (Np, nstate, nelem_total) array, as vmapP's;
+// one lane per (node, state) word, consecutive lanes on consecutive nodes
+template <class W, bool PACK>
+__global__ __launch_bounds__(256) void nodes_kernel(W* __restrict__ Q, W* __restrict__ buf,
+                                                    const int64_t* __restrict__ nodes,
+                                                    int64_t nnode, int64_t Np, int64_t nstate,
+                                                    int64_t nelem_total) {
+  const int64_t total = nnode * nstate;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = g / nnode, k = g - s * nnode;
+    const int64_t id = nodes[k];
+    if (id < 1 || id > nelem_total * Np) {
+      if (s == 0) aux_error(CANARY_ERR_VMAP, k + 1, -1, id);
+      continue;
+    }
+    const int64_t e = (id - 1) / Np, n = (id - 1) - e * Np;
+    const size_t q = (size_t)(e * nstate + s) * Np + n;
+    if (PACK)
+      buf[g] = Q[q];
+    else
+      Q[q] = buf[g];
+  }
 }
 
 // LDS tree sum / min of the block's values; result in lane 0
@@ -254,6 +281,44 @@ extern "C" int canary_transferrecvQ(int dim, int N, int nstate, void* Q, const v
   else
     return CANARY_ERR_ARG;
   return status();
+}
+
+namespace {
+template <bool PACK>
+int nodes_entry(int dim, int N, int nstate, void* Q, void* buf, const int64_t* nodes,
+                int64_t nnode, int64_t nelem_total, int word_bytes, void* stream) {
+  if ((dim != 2 && dim != 3) || N < 1 || nstate < 1 || nnode < 0 || nelem_total < 0)
+    return CANARY_ERR_ARG;
+  if (nnode == 0) return 0;
+  if (!Q || !buf || !nodes) return CANARY_ERR_ARG;
+  int64_t Np = 1;
+  for (int d = 0; d < dim; ++d) Np *= N + 1;
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned g = (unsigned)grid_for(nnode * nstate);
+  if (word_bytes == 8)
+    nodes_kernel<double, PACK><<<g, 256, 0, st>>>((double*)Q, (double*)buf, nodes, nnode, Np,
+                                                  nstate, nelem_total);
+  else if (word_bytes == 4)
+    nodes_kernel<float, PACK><<<g, 256, 0, st>>>((float*)Q, (float*)buf, nodes, nnode, Np,
+                                                 nstate, nelem_total);
+  else
+    return CANARY_ERR_ARG;
+  return status();
+}
+}  // namespace
+
+extern "C" int canary_pack_nodes(int dim, int N, int nstate, void* buf, const void* Q,
+                                 const int64_t* nodes, int64_t nnode, int64_t nelem_total,
+                                 int word_bytes, void* stream) {
+  return nodes_entry<true>(dim, N, nstate, const_cast<void*>(Q), buf, nodes, nnode, nelem_total,
+                           word_bytes, stream);
+}
+
+extern "C" int canary_unpack_nodes(int dim, int N, int nstate, void* Q, const void* buf,
+                                   const int64_t* nodes, int64_t nnode, int64_t nelem_total,
+                                   int word_bytes, void* stream) {
+  return nodes_entry<false>(dim, N, nstate, Q, const_cast<void*>(buf), nodes, nnode, nelem_total,
+                            word_bytes, stream);
 }
 
 extern "C" int canary_l2energysquared_device(int Np, int nstate, int nvgeo, int iMJ,

@@ -265,6 +265,28 @@ def transferrecvQ(dim, N, nstate, Q, recvQ, nrealelem, stream=None):
     check(rc, "transferrecvQ")
 
 
+def pack_nodes(dim, N, nstate, buf, Q, nodes, nelem_total, stream=None):
+    """Face-trace halo pack: buf[k + nnode*s] = Q[node k (1-based id), s]."""
+    nnode = nodes.numel()
+    _need(nodes, "nodes", nnode, torch.int64)
+    _need(buf, "buf", nnode * nstate, Q.dtype)
+    _need(Q, "Q", (N + 1) ** dim * nstate * nelem_total, Q.dtype)
+    rc = lib().canary_pack_nodes(dim, N, nstate, _ptr(buf), _ptr(Q), _ptr(nodes), nnode,
+                                 int(nelem_total), _word(Q), _stream(stream))
+    check(rc, "pack_nodes")
+
+
+def unpack_nodes(dim, N, nstate, Q, buf, nodes, nelem_total, stream=None):
+    """Face-trace halo unpack: Q[node k, s] = buf[k + nnode*s]."""
+    nnode = nodes.numel()
+    _need(nodes, "nodes", nnode, torch.int64)
+    _need(buf, "buf", nnode * nstate, Q.dtype)
+    _need(Q, "Q", (N + 1) ** dim * nstate * nelem_total, Q.dtype)
+    rc = lib().canary_unpack_nodes(dim, N, nstate, _ptr(Q), _ptr(buf), _ptr(nodes), nnode,
+                                   int(nelem_total), _word(Q), _stream(stream))
+    check(rc, "unpack_nodes")
+
+
 def l2energysquared(dim, N, Q, vgeo, nelem, nstate=None, stream=None):
     """L2energysquared on the GPU (deterministic tree sum); returns a
     0-dimensional CUDA tensor."""

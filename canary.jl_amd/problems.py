@@ -426,6 +426,8 @@ class RankShare:
     vmapP: np.ndarray
     elemtobndy: np.ndarray
     sendelems: np.ndarray  # local indices of real elements some other rank reads (1-based)
+    send_nodes: np.ndarray  # local node ids (1-based) other ranks read through their vmapP
+    recv_nodes: np.ndarray  # local node ids (1-based) of the ghost nodes this rank's vmapP reads
 
     def local_state(self, Q):
         """(Np, nstate, nreal + nghost) state of this share from a global one."""
@@ -453,20 +455,26 @@ def rank_share(s, nranks, rank):
     vmapM = np.asarray(s.vmapM)[..., lo:hi]
     vmapM_l = np.asfortranarray(vmapM - lo * Np)
     # the real elements some other rank's ghosts are (what a send buffer packs)
-    sends = []
+    sends, snodes = [], []
     for r in range(nranks):
         if r == rank:
             continue
         rlo, rhi = r * ne // nranks, (r + 1) * ne // nranks
-        ePr = (np.asarray(s.vmapP)[..., rlo:rhi] - 1) // Np
-        sends.append(ePr[(ePr >= lo) & (ePr < hi)])
+        vPr = np.asarray(s.vmapP)[..., rlo:rhi]
+        ePr = (vPr - 1) // Np
+        mine = (ePr >= lo) & (ePr < hi)
+        sends.append(ePr[mine])
+        snodes.append(vPr[mine] - lo * Np)
     sendelems = (np.unique(np.concatenate(sends)) - lo + 1) if sends else np.zeros(0, np.int64)
+    send_nodes = np.unique(np.concatenate(snodes)) if snodes else np.zeros(0, np.int64)
+    recv_nodes = np.unique(vmapP_l[vmapP_l > nreal * Np])
     return RankShare(nreal=nreal, nelem_total=nreal + ghost.size, real=real, ghost=ghost,
                      vgeo=np.asfortranarray(np.asarray(s.vgeo)[..., lo:hi]),
                      sgeo=np.asfortranarray(np.asarray(s.sgeo)[..., lo:hi]),
                      vmapM=vmapM_l, vmapP=vmapP_l,
                      elemtobndy=np.asfortranarray(np.asarray(s.elemtobndy)[..., lo:hi]),
-                     sendelems=sendelems.astype(np.int64))
+                     sendelems=sendelems.astype(np.int64), send_nodes=send_nodes.astype(np.int64),
+                     recv_nodes=recv_nodes.astype(np.int64))
 
 
 def rk_coefs():

@@ -409,6 +409,18 @@ int canary_fillsendQ(int dim, int N, int nstate, void* sendQ, const void* Q,
 int canary_transferrecvQ(int dim, int N, int nstate, void* Q, const void* recvQ, int64_t nrecv,
                          int64_t nrealelem, int word_bytes, void* stream);
 
+/* Face-trace halo: the nodes a neighbour reads through vmapP instead of
+ * whole ghost elements (Nfp of Np nodes per shared face; no reference
+ * counterpart, the reduced exchange of the sharded design).  nodes: 1-based
+ * node ids over the (Np, nstate, nelem_total) array, like vmapP;
+ * buf[k + nnode * s] = Q[node k, state s].  A node id out of range is
+ * skipped and recorded as CANARY_ERR_VMAP. */
+int canary_pack_nodes(int dim, int N, int nstate, void* buf, const void* Q, const int64_t* nodes,
+                      int64_t nnode, int64_t nelem_total, int word_bytes, void* stream);
+int canary_unpack_nodes(int dim, int N, int nstate, void* Q, const void* buf,
+                        const int64_t* nodes, int64_t nnode, int64_t nelem_total,
+                        int word_bytes, void* stream);
+
 /* Deterministic device reductions into out[0] (device memory); work holds
  * CANARY_REDUCE_WORK doubles of scratch.  Same values as the host
  * canary_l2energysquared / canary_cfl up to summation order (energy) and

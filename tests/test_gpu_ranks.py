@@ -3,7 +3,9 @@ ghost elements appended): each share's send buffer packed with fillsendQ,
 the "exchange" emulated by gathering the senders' packed elements into each
 receiver's ghost order, unpacked with transferrecvQ, then one fused stage per
 share.  Every real element must come out bit-identical to the single-mesh
-stage, and ghost slots of Qout are never written."""
+stage, and ghost slots of Qout are never written.  The same with the
+face-trace halo (pack_nodes / unpack_nodes): only the ghost nodes vmapP
+reads are sent, the rest of each ghost element stays NaN."""
 import numpy as np
 import pytest
 import torch
@@ -63,3 +65,56 @@ def test_rank_shares_reproduce_the_single_mesh_stage(cuda, dim, N, nranks, prec)
         assert max_ulp(got[:, :, :sh.nreal], want_q[:, :, sh.real]) == 0
         assert np.all(got[:, :, sh.nreal:] == -7.0)
         assert max_ulp(host(rl, (Np, ns, sh.nreal)), want_r[:, :, sh.real]) == 0
+
+
+@pytest.mark.parametrize("dim,N,nranks", [(3, 3, 4), (2, 5, 3), (3, 4, 8)])
+def test_face_trace_halo_reproduces_the_single_mesh_stage(cuda, dim, N, nranks):
+    s = small_setup(0, dim, N, Ne=3, periodic=(True,) + (False,) * (dim - 1))
+    Np, ns = s.Np, s.nstate
+    Q = np.asfortranarray(smooth_euler_state(s, seed=6))
+    rhs0 = np.asfortranarray(np.random.default_rng(2).standard_normal(Q.shape))
+    A, B, _ = P.rk_coefs()
+    args = (A[3], B[2], 0.02)
+    q, qo, r = dev(Q), dev(np.zeros_like(Q)), dev(rhs0)
+    K.stage_fused(dim, N, r, q, qo, dev(s.vgeo), dev(s.sgeo), dev(s.D), s.nelem, dev(s.vmapM),
+                  dev(s.vmapP), dev(s.elemtobndy), *args, phys=s.phys, prec=FP64_FMA)
+    K.check_errors()
+    want_q = host(qo, Q.shape)
+    shares = [P.rank_share(s, nranks, k) for k in range(nranks)]
+    # pack every share's traces; index them by global node id
+    where, bufs = {}, []
+    for k, sh in enumerate(shares):
+        ql = dev(sh.local_state(Q))
+        buf = torch.empty(max(1, sh.send_nodes.size) * ns, dtype=torch.float64, device=ql.device)
+        if sh.send_nodes.size:
+            K.pack_nodes(dim, N, ns, buf, ql, dev(sh.send_nodes), sh.nelem_total)
+        bufs.append(buf.view(ns, -1))
+        for j, nd in enumerate(sh.send_nodes):
+            where[int(nd + sh.real[0] * Np)] = (k, j)
+    for sh in shares:
+        ql = dev(sh.local_state(Q))
+        ql.view(-1, Np * ns)[sh.nreal:] = float("nan")
+        glob_e = np.concatenate([sh.real, sh.ghost])
+        gid = glob_e[(sh.recv_nodes - 1) // Np] * Np + (sh.recv_nodes - 1) % Np + 1
+        recv = torch.stack([bufs[where[int(g)][0]][:, where[int(g)][1]] for g in gid], dim=1)
+        K.unpack_nodes(dim, N, ns, ql, recv.reshape(-1), dev(sh.recv_nodes), sh.nelem_total)
+        qol = torch.full_like(ql, -7.0)
+        rl = dev(np.asfortranarray(rhs0[:, :, sh.real]))
+        K.stage_fused(dim, N, rl, ql, qol, dev(sh.vgeo), dev(sh.sgeo), dev(s.D), sh.nreal,
+                      dev(sh.vmapM), dev(sh.vmapP), dev(sh.elemtobndy), *args, phys=s.phys,
+                      nelem_total=sh.nelem_total, prec=FP64_FMA)
+        K.check_errors()
+        got = host(qol, (Np, ns, sh.nelem_total))
+        assert max_ulp(got[:, :, :sh.nreal], want_q[:, :, sh.real]) == 0
+        assert np.all(got[:, :, sh.nreal:] == -7.0)
+
+
+def test_pack_nodes_out_of_range_reported(cuda):
+    from canary_amd._lib import CanaryError, ERR_VMAP
+    s = small_setup(0, 2, 2, Ne=3)
+    q = dev(s.zeros_state())
+    buf = torch.empty(2 * 4, dtype=torch.float64, device=q.device)
+    K.pack_nodes(2, 2, 4, buf, q, dev(np.array([1, s.nelem * s.Np + 3], dtype=np.int64)), s.nelem)
+    with pytest.raises(CanaryError) as ei:
+        K.check_errors()
+    assert ei.value.code == ERR_VMAP

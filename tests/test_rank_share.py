@@ -34,6 +34,18 @@ def test_rank_shares_cover_the_mesh(dim, nranks):
             if q != r:
                 others |= set(shq.ghost) & set(sh.real)
         assert set(sh.real[sh.sendelems - 1]) == others
+        # face traces: the ghost nodes read are exactly the vmapP entries past
+        # the real elements; what is sent is what the others read of this rank
+        vp = sh.vmapP.ravel()
+        assert np.array_equal(sh.recv_nodes, np.unique(vp[vp > sh.nreal * Np]))
+        lo = sh.real[0] * Np
+        read = set()
+        for q, shq in enumerate(shares):
+            if q != r:
+                g = np.asarray(s.vmapP)[..., shq.real].ravel()
+                read |= set((g[(g > lo) & (g <= lo + sh.nreal * Np)] - lo).tolist())
+        assert set(sh.send_nodes.tolist()) == read
+        assert sh.recv_nodes.size < (sh.nelem_total - sh.nreal) * Np  # traces, not elements
     x = np.random.default_rng(0).standard_normal((Np, 4 if dim == 2 else 5, s.nelem))
     loc = shares[1].local_state(x)
     assert np.array_equal(loc[:, :, :shares[1].nreal], x[:, :, shares[1].real])
