@@ -358,64 +358,6 @@ __global__ __launch_bounds__((VolumeCfg<Phys, NQ, EPW>::NT)) void volume_kernel(
   }
 }
 
-template <class Phys, int NQ, int EPW>
-struct FaceCfg {
-  using S = Shape<Phys, NQ>;
-  static constexpr int NT = round64(EPW * S::NFN) > 1024 ? 1024 : round64(EPW * S::NFN);
-};
-
-template <class Phys, int NQ, int EPW>
-__global__ __launch_bounds__((FaceCfg<Phys, NQ, EPW>::NT)) void face_kernel(
-    double* __restrict__ rhs, const double* __restrict__ Q, const double* __restrict__ sgeo,
-    int64_t nreal, int64_t ntotal, const int64_t* __restrict__ vmapM,
-    const int64_t* __restrict__ vmapP, const int64_t* __restrict__ bndy, PhysParams p) {
-  using S = Shape<Phys, NQ>;
-  constexpr int Np = S::Np, NS = Phys::NS, NSR = Phys::NSR, NFN = S::NFN, Nfp = S::Nfp;
-  constexpr int NT = FaceCfg<Phys, NQ, EPW>::NT;
-  __shared__ double sR[EPW][NSR * Np];
-  __shared__ double sC[EPW][NFN * NSR];
-  __shared__ int sV[EPW][NFN];
-  const int t = threadIdx.x;
-  const int64_t e0 = (int64_t)blockIdx.x * EPW;
-  for (int idx = t; idx < EPW * NSR * Np; idx += NT) {
-    const int slot = idx / (NSR * Np), rem = idx - slot * (NSR * Np);
-    const int64_t e = e0 + slot;
-    if (e < nreal) sR[slot][rem] = rhs[(size_t)NS * Np * (size_t)e + rem];
-  }
-  for (int item = t; item < EPW * NFN; item += NT) {
-    const int slot = item / NFN, fi = item - slot * NFN;
-    const int64_t e = e0 + slot;
-    if (e >= nreal) continue;
-    const double* qe = Q + (size_t)NS * Np * (size_t)e;
-    auto qm_at = [&](int s, int v) { return qe[v + Np * s]; };
-    auto pm_at = [&](int, const double* qm) { return Phys::HAS_P ? Phys::pressure(qm, p) : 0.0; };
-    double c[NSR];
-    const int vid = face_contribution<Phys, NQ>(Q, sgeo, vmapM, vmapP, bndy, e, fi, ntotal, p,
-                                                qm_at, pm_at, c);
-    sV[slot][fi] = vid;
-#pragma unroll
-    for (int s = 0; s < NSR; ++s) sC[slot][fi * NSR + s] = vid >= 0 ? c[s] : __builtin_nan("");
-  }
-  lds_barrier();
-#pragma unroll
-  for (int d = 0; d < S::DIM; ++d) {
-    for (int it = t; it < EPW * 2 * Nfp; it += NT) {
-      const int slot = it / (2 * Nfp), fi = 2 * d * Nfp + (it - slot * 2 * Nfp);
-      const int64_t e = e0 + slot;
-      const int v = sV[slot][fi];
-      if (e < nreal && v >= 0) {
-#pragma unroll
-        for (int s = 0; s < NSR; ++s) sR[slot][s * Np + v] = sR[slot][s * Np + v] - sC[slot][fi * NSR + s];
-      }
-    }
-    lds_barrier();
-  }
-  for (int idx = t; idx < EPW * NSR * Np; idx += NT) {
-    const int slot = idx / (NSR * Np), rem = idx - slot * (NSR * Np);
-    const int64_t e = e0 + slot;
-    if (e < nreal) rhs[(size_t)NS * Np * (size_t)e + rem] = sR[slot][rem];
-  }
-}
 
 template <class Phys, int NQ>
 __global__ __launch_bounds__(256) void update_kernel(double* __restrict__ rhs,
@@ -499,8 +441,10 @@ struct FusedCfg {
 // the nodal pressure in LDS for the M side of the faces (else recompute it).
 // VOLUME = false leaves the volume term out (face terms + update only: the
 // opt-in facerhs_update, after a separate volumerhs); vgeo is then read for
-// MJI alone and D is not read.
-template <class Phys, int NQ, int EPW, int NT, int PF, bool PM_LDS, bool VOLUME = true>
+// MJI alone and D is not read.  UPDATE = false (with VOLUME = false) is
+// facerhs! alone: rhs -= face terms, Q untouched, neither vgeo nor Qout read.
+template <class Phys, int NQ, int EPW, int NT, int PF, bool PM_LDS, bool VOLUME = true,
+          bool UPDATE = true>
 __device__ __forceinline__ void stage_body(
     char* __restrict__ lds, typename Phys::Real* __restrict__ rhs,
     const typename Phys::Real* __restrict__ Qin, typename Phys::Real* __restrict__ Qout,
@@ -558,14 +502,15 @@ __device__ __forceinline__ void stage_body(
   const Real* qe = Qin + (size_t)NS * Np * (size_t)el;
 #pragma unroll
   for (int s = 0; s < NS; ++s) q[s] = qe[nl + Np * s];
-  const Real* vg = vgeo + (size_t)NV * Np * (size_t)el;
+  const Real* vg = (VOLUME || UPDATE) ? vgeo + (size_t)NV * Np * (size_t)el : nullptr;
   Real MJ = 0;
   if constexpr (VOLUME) {
 #pragma unroll
     for (int c = 0; c < DIM * DIM; ++c) met[c] = vg[nl + Np * c];
     MJ = vg[nl + Np * Phys::iMJ];
   }
-  const Real MJI = vg[nl + Np * Phys::iMJI];
+  Real MJI = 0;
+  if constexpr (UPDATE) MJI = vg[nl + Np * Phys::iMJI];
   const Real* re = rhs + (size_t)NS * Np * (size_t)el;
 #pragma unroll
   for (int s = 0; s < NSR; ++s) r[s] = re[nl + Np * s];
@@ -662,7 +607,13 @@ __device__ __forceinline__ void stage_body(
   //    stores: nothing reads these lines again in this stage, and keeping
   //    them out of L2 leaves room for the Q lines the face gathers reuse
   //    (measured on 3D N=4: 600 fewer bytes fetched per element, 3% faster).
-  if (valid) {
+  if (!UPDATE && valid) {  // facerhs! alone: the face terms into rhs
+    Real* re = rhs + (size_t)NS * Np * (size_t)e;
+#pragma unroll
+    for (int s = 0; s < NSR; ++s)
+      __builtin_nontemporal_store(sB[slot * BUF + NFN * NSR + s * Np + n], &re[n + Np * s]);
+  }
+  if (UPDATE && valid) {
     const Real bdt = (Real)(rkb * dt);
     const Real ka = (Real)rka;
     Real* qo = Qout + (size_t)NS * Np * (size_t)e;
@@ -694,6 +645,26 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, WIDE>::NT)) void stage_ker
   if (e0 >= nreal) return;  // whole workgroup leaves together
   stage_body<Phys, NQ, EPW, C::NT, PF, PM_LDS>(lds, rhs, Qin, Qout, vgeo, sgeo, D, e0, nreal,
                                                ntotal, vmapM, vmapP, bndy, rka, rkb, dt, p);
+}
+
+// facerhs! alone (rhs -= sMJ * Rusanov flux of Q): the fused stage body with
+// its volume term and update switched off, so the face terms get the same
+// schedule (every load issued first, nodal Q and pressure staged in LDS for
+// the M side, non-temporal rhs stores) and the same bits.
+template <class Phys, int NQ, int EPW>
+__global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, false>::NT)) void face_kernel(
+    double* __restrict__ rhs, const double* __restrict__ Q, const double* __restrict__ sgeo,
+    int64_t nreal, int64_t ntotal, const int64_t* __restrict__ vmapM,
+    const int64_t* __restrict__ vmapP, const int64_t* __restrict__ bndy, int64_t chunk,
+    PhysParams p) {
+  using C = FusedCfg<Phys, NQ, EPW, false>;
+  __shared__ __attribute__((aligned(16))) char lds[C::LDS];
+  const int64_t blk = xcd_block(blockIdx.x, chunk);
+  const int64_t e0 = blk * EPW;
+  if (e0 >= nreal) return;  // whole workgroup leaves together
+  stage_body<Phys, NQ, EPW, C::NT, 1, true, false, false>(lds, rhs, Q, nullptr, nullptr, sgeo,
+                                                          nullptr, e0, nreal, ntotal, vmapM,
+                                                          vmapP, bndy, 0.0, 0.0, 0.0, p);
 }
 
 // facerhs! + updatesolution! of one stage in one pass (the volume term was

@@ -86,9 +86,11 @@ int launch(int op, const KArgs& a) {
         break;
       }
       case OP_FACE: {
-        constexpr int NT = FaceCfg<Phys, NQ, EPW>::NT;
-        face_kernel<Phys, NQ, EPW><<<dim3((unsigned)nb), dim3(NT), 0, a.stream>>>(
-            rhs, Q, (const double*)a.sgeo, a.nreal, a.ntotal, a.vmapM, a.vmapP, a.bndy, a.phys);
+        constexpr int NT = FusedCfg<Phys, NQ, EPW, false>::NT;
+        const int64_t chunk = (nb + 7) / 8;
+        face_kernel<Phys, NQ, EPW><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
+            rhs, Q, (const double*)a.sgeo, a.nreal, a.ntotal, a.vmapM, a.vmapP, a.bndy, chunk,
+            a.phys);
         break;
       }
       case OP_UPDATE: {
