@@ -53,7 +53,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--ne", type=int, default=32)
     ap.add_argument("--N", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -118,10 +118,10 @@ def main():
     dt = P.cfl(s, s.Q) / s.N ** np.sqrt(2)
     stream = torch.cuda.current_stream()
 
-    def run_fused(q, q2, rhs, nsteps, prec=FP64_FMA):
+    def run_fused(q, q2, rhs, nsteps, prec=FP64_FMA, mapM=vM):
         for _ in range(nsteps):
             for st in range(5):
-                K.stage_fused(3, s.N, rhs, q, q2, vgeo, sgeo, D, s.nelem, vM, vP, bnd,
+                K.stage_fused(3, s.N, rhs, q, q2, vgeo, sgeo, D, s.nelem, mapM, vP, bnd,
                               A[(st + 1) % 5], B[st], dt, phys=s.phys, prec=prec)
                 q, q2 = q2, q
         return q, q2
@@ -201,6 +201,27 @@ def main():
     extra["fma_vs_exact_rel_l2_after_%d_steps" % (args.warmup + args.steps)] = float(
         "%.3e" % (torch.linalg.vector_norm(qf - qx) / torch.linalg.vector_norm(qx)).item())
     del qe, qe2, re, hold_e, qx
+    # the contracted stage with vmapM = NULL (M-side ids from the canonical
+    # face mask, the map not read); same bits as with the map
+    qc, qc2, rc_ = Q0.clone(), torch.empty_like(Q0), torch.zeros_like(Q0)
+    qc, qc2 = run_fused(qc, qc2, rc_, args.warmup, FP64_FMA, None)
+    hold_c = {}
+
+    def go_canon():
+        hold_c["q"] = run_fused(qc, qc2, rc_, args.steps, FP64_FMA, None)
+
+    t_canon, _ = timed(go_canon)
+    K.check_errors()
+    extra["fused_null_vmapM_gdof_stage_s"] = round(dof * stages / t_canon / 1e9, 3)
+    extra["fused_null_vmapM_ms_per_stage"] = round(t_canon / stages * 1e3, 4)
+    extra["fused_null_vmapM_same_bits"] = bool(torch.equal(hold_c["q"][0], qf))
+    # its roofline fraction on the same algorithmic bytes, and on the bytes
+    # without vmapM (the stricter minimum, which it does move)
+    t_c = t_canon / stages
+    extra["fused_null_vmapM_frac"] = round(bytes_launch / t_c / 1e9 / HBM_PEAK_GBS, 4)
+    extra["fused_null_vmapM_frac_without_vmapM_bytes"] = round(
+        (algorithmic_bytes(3, s.N) - 6 * (s.N + 1) ** 2 * 8) * s.nelem / t_c / 1e9 / HBM_PEAK_GBS, 4)
+    del qc, qc2, rc_, hold_c
     # the library's native LSRK54 loop (canary_lsrk54), two steps per captured
     # hipGraph, on a stream of its own: the same kernels without a host loop
     from canary_amd.streams import OwnedStream

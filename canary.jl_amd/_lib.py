@@ -71,6 +71,14 @@ class BatchItem(C.Structure):
     _fields_ = [("solve", C.c_int32), ("N", C.c_int16), ("epw", C.c_int16), ("e0", C.c_int64)]
 
 
+class BatchClass(C.Structure):
+    _fields_ = [("items", C.c_void_p), ("nitems", C.c_int64), ("N", C.c_int32),
+                ("epw", C.c_int32)]
+
+
+BATCH_DROPIN, BATCH_P32 = 0, 1
+
+
 _lib = None
 
 P = C.c_void_p
@@ -92,6 +100,7 @@ _SIGS = {
     "canary_lsrk54_coefs": (None, [P, P, P]),
     "canary_pack_nodes": (INT, [INT, INT, INT, P, P, P, I64, I64, INT, P]),
     "canary_unpack_nodes": (INT, [INT, INT, INT, P, P, P, I64, I64, INT, P]),
+    "canary_batch_run": (INT, [INT, INT, INT, INT, P, P, INT, I64, INT, P]),
     "canary_check_errors": (INT, [P, P]),
     "canary_perturb32_prepare": (INT, [INT, INT, I64, I64, P, P, P, P, P, P, P, C.POINTER(Phys),
                                        P, P, P, P, P, P, P]),

@@ -269,17 +269,34 @@ class _BatchBase:
 
     def run(self, nsteps, graph=True):
         """nsteps LSRK54 steps of every solve; graph=True replays one captured
-        hipGraph per step (one per starting parity)."""
+        hipGraph per step (one per starting parity), graph=False launches the
+        stages from the library's own loop (canary_batch_run) on the current
+        stream."""
         if self.dirty:
             self.finalize()
+        if not graph:
+            self._run_native(nsteps)
+            return
         for _ in range(nsteps):
-            if graph:
-                self._graph().replay()
-                self.parity ^= 1  # five stages flip it an odd number of times
-                for v in self.solves.values():
-                    v.steps += 1
-            else:
-                self.step()
+            self._graph().replay()
+            self.parity ^= 1  # five stages flip it an odd number of times
+            for v in self.solves.values():
+                v.steps += 1
+
+    def _run_native(self, nsteps):
+        if nsteps <= 0 or not self.groups:
+            return
+        base = self._items.data_ptr()
+        cls = (_lib.BatchClass * len(self.groups))()
+        for i, (N, epw, off, cnt) in enumerate(self.groups):
+            cls[i].items, cls[i].nitems, cls[i].N, cls[i].epw = base + 16 * off, cnt, N, epw
+        kind, eqn, dim, prec = self._run_kind()
+        rc = lib().canary_batch_run(kind, eqn, dim, prec, _ptr(self._table), cls, len(self.groups),
+                                    int(nsteps), self.parity, self._st())
+        check(rc, "batch run")
+        self.parity ^= nsteps & 1  # five stages per step flip it an odd number of times
+        for v in self.solves.values():
+            v.steps += nsteps
 
     def check(self):
         info = (C.c_int64 * 4)()
@@ -358,6 +375,9 @@ class Batch(_BatchBase):
     def _launch_raw(self, N, epw, table, items_ptr, count, parity, rka, rkb, stream):
         return lib().canary_batch_stage(self.eqn, self.dim, self.prec, N, epw, _ptr(table),
                                         items_ptr, count, parity, rka, rkb, stream)
+
+    def _run_kind(self):
+        return _lib.BATCH_DROPIN, self.eqn, self.dim, self.prec
 
     def _word(self):
         return 4 if self.prec == FP32 else 8
@@ -442,6 +462,9 @@ class Batch32(_BatchBase):
     def _launch_raw(self, N, epw, table, items_ptr, count, parity, rka, rkb, stream):
         return lib().canary_p32_batch_stage(self.dim, N, epw, _ptr(table), items_ptr, count,
                                             parity, rka, rkb, stream)
+
+    def _run_kind(self):
+        return _lib.BATCH_P32, EULER, self.dim, 0
 
     def _word(self):
         return 4

@@ -170,9 +170,31 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Volume node (0-based) of face node fn of face f: the canonical face mask
+// of src/mesh.jl:978-982 -- dimension f/2 held at its first (even f) or last
+// (odd f) index, the other dimensions in column-major order.  vmapM[:,f,e]
+// is always Np*(e-1) + this + 1 (src/mesh.jl:994).
+template <int DIM, int NQ>
+__device__ __forceinline__ int canon_vid(int f, int fn) {
+  constexpr int N = NQ - 1;
+  const int fe = (f & 1) * N;
+  if (DIM == 2) return (f >> 1) == 0 ? fe + NQ * fn : fn + NQ * fe;
+  const int a = fn % NQ, b = fn / NQ;
+  switch (f >> 1) {
+    case 0: return fe + NQ * a + NQ * NQ * b;
+    case 1: return a + NQ * fe + NQ * NQ * b;
+    default: return a + NQ * b + NQ * NQ * fe;
+  }
+}
+
 // face_meta in two halves: face_load issues the loads (no use of their
 // values, so a kernel can issue more loads behind them), face_resolve checks
 // the maps and bc and splits the node ids.
+//
+// A null vmapM stands for the canonical face mask (the only vmapM that
+// src/mesh.jl builds): the M-side id is computed, and the map load re-reads
+// the vmapP entry instead (a cache hit, no HBM bytes), so both forms run one
+// instruction sequence in one kernel.
 template <class Phys, int NQ>
 __device__ __forceinline__ void face_load(FaceItem<Phys>& it,
                                           const typename Phys::Real* __restrict__ sgeo,
@@ -187,8 +209,10 @@ __device__ __forceinline__ void face_load(FaceItem<Phys>& it,
 #pragma unroll
   for (int c = 0; c < DIM; ++c) it.nrm[c] = sg[c];
   it.sMJ = sg[DIM];
-  it.idM = vmapM[fo];
+  const bool canon = vmapM == nullptr;
+  const int64_t rawM = (canon ? vmapP : vmapM)[fo];
   it.idP = vmapP[fo];
+  it.idM = canon ? (int64_t)S::Np * e + canon_vid<DIM, NQ>(f, fi - f * S::Nfp) + 1 : rawM;
   it.bcode = bndy[f + S::NFACE * e];
 }
 

@@ -86,6 +86,35 @@ int step(Run& r, int& cur) {
 
 }  // namespace
 
+extern "C" int canary_batch_run(int kind, int eqn, int dim, int prec, const void* solves,
+                                const canary_batch_class* classes, int nclasses, int64_t nsteps,
+                                int parity, void* stream) {
+  if (kind != CANARY_BATCH_DROPIN && kind != CANARY_BATCH_P32) return CANARY_ERR_ARG;
+  if (nclasses < 0 || nsteps < 0 || (parity != 0 && parity != 1)) return CANARY_ERR_ARG;
+  if (nsteps == 0 || nclasses == 0) return 0;
+  if (!solves || !classes) return CANARY_ERR_ARG;
+  double A[5], B[5], C[5];
+  lsrk54_tables(A, B, C);
+  int par = parity;
+  for (int64_t k = 0; k < nsteps; ++k)
+    for (int s = 0; s < 5; ++s) {
+      for (int c = 0; c < nclasses; ++c) {
+        const canary_batch_class& cl = classes[c];
+        if (cl.nitems == 0) continue;
+        const int rc =
+            kind == CANARY_BATCH_DROPIN
+                ? canary_batch_stage(eqn, dim, prec, cl.N, cl.epw,
+                                     (const canary_batch_solve*)solves, cl.items, cl.nitems, par,
+                                     A[(s + 1) % 5], B[s], stream)
+                : canary_p32_batch_stage(dim, cl.N, cl.epw, (const canary_p32_solve*)solves,
+                                         cl.items, cl.nitems, par, A[(s + 1) % 5], B[s], stream);
+        if (rc) return rc;
+      }
+      par = 1 - par;
+    }
+  return 0;
+}
+
 extern "C" void canary_lsrk54_coefs(double* rka, double* rkb, double* rkc) {
   double a[5], b[5], c[5];
   lsrk54_tables(a, b, c);

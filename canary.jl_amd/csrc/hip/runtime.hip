@@ -102,7 +102,7 @@ extern "C" int canary_facerhs(int eqn, int dim, int N, int prec, void* rhs, cons
   if (rc) return rc;
   if (nreal == 0) return 0;
   if (nelem_total < nreal) return CANARY_ERR_ARG;
-  if (!rhs || !Q || !sgeo || !vmapM || !vmapP || !elemtobndy) return CANARY_ERR_ARG;
+  if (!rhs || !Q || !sgeo || !vmapP || !elemtobndy) return CANARY_ERR_ARG;
   canary::KArgs a = {};
   a.rhs = rhs;
   a.Q = Q;
@@ -152,7 +152,7 @@ extern "C" int canary_stage_fused(int eqn, int dim, int N, int prec, void* rhs, 
   if (rc) return rc;
   if (nreal == 0) return 0;
   if (nelem_total < nreal) return CANARY_ERR_ARG;
-  if (!rhs || !Qin || !Qout || !vgeo || !sgeo || !D || !vmapM || !vmapP || !elemtobndy)
+  if (!rhs || !Qin || !Qout || !vgeo || !sgeo || !D || !vmapP || !elemtobndy)
     return CANARY_ERR_ARG;
   if (Qin == (const void*)Qout) return CANARY_ERR_ARG;
   canary::KArgs a = {};
@@ -186,7 +186,7 @@ extern "C" int canary_facerhs_update(int eqn, int dim, int N, int prec, void* rh
   if (rc) return rc;
   if (nreal == 0) return 0;
   if (nelem_total < nreal) return CANARY_ERR_ARG;
-  if (!rhs || !Qin || !Qout || !vgeo || !sgeo || !vmapM || !vmapP || !elemtobndy)
+  if (!rhs || !Qin || !Qout || !vgeo || !sgeo || !vmapP || !elemtobndy)
     return CANARY_ERR_ARG;
   if (Qin == (const void*)Qout) return CANARY_ERR_ARG;
   canary::KArgs a = {};

@@ -50,6 +50,14 @@ def _need(t, name, numel, dtype=torch.float64):
         raise ValueError("%s has %d elements, needs at least %d" % (name, t.numel(), numel))
 
 
+def _mapM(vmapM, numel):
+    """vmapM, or None for the canonical face mask (the map is then not read)."""
+    if vmapM is None:
+        return C.c_void_p(None)
+    _need(vmapM, "vmapM", numel, torch.int64)
+    return _ptr(vmapM)
+
+
 def _phys(phys):
     return C.byref(phys) if phys is not None else C.byref(_lib.default_phys())
 
@@ -78,11 +86,11 @@ def facerhs(dim, N, rhs, Q, sgeo, elems, vmapM, vmapP, elemtobndy, eqn=EULER, ph
     _need(rhs, "rhs", Np * ns * n)
     _need(Q, "Q", Np * ns * ntot)
     _need(sgeo, "sgeo", _NSG[dim] * Nfp * nf * n)
-    _need(vmapM, "vmapM", Nfp * nf * n, torch.int64)
+    pM = _mapM(vmapM, Nfp * nf * n)
     _need(vmapP, "vmapP", Nfp * nf * n, torch.int64)
     _need(elemtobndy, "elemtobndy", nf * n, torch.int64)
     rc = lib().canary_facerhs(eqn, dim, N, FP64, _ptr(rhs), _ptr(Q), _ptr(sgeo), n, ntot,
-                              _ptr(vmapM), _ptr(vmapP), _ptr(elemtobndy), _phys(phys),
+                              pM, _ptr(vmapP), _ptr(elemtobndy), _phys(phys),
                               _stream(stream))
     check(rc, "facerhs")
 
@@ -125,13 +133,13 @@ def stage_fused(dim, N, rhs, Qin, Qout, vgeo, sgeo, D, elems, vmapM, vmapP, elem
     _need(vgeo, "vgeo", Np * _NV[dim] * n)
     _need(sgeo, "sgeo", _NSG[dim] * Nfp * nf * n)
     _need(D, "D", (N + 1) ** 2)
-    _need(vmapM, "vmapM", Nfp * nf * n, torch.int64)
+    pM = _mapM(vmapM, Nfp * nf * n)
     _need(vmapP, "vmapP", Nfp * nf * n, torch.int64)
     _need(elemtobndy, "elemtobndy", nf * n, torch.int64)
     if Qin.data_ptr() == Qout.data_ptr():
         raise ValueError("stage_fused needs distinct Qin and Qout")
     rc = lib().canary_stage_fused(eqn, dim, N, prec, _ptr(rhs), _ptr(Qin), _ptr(Qout),
-                                  _ptr(vgeo), _ptr(sgeo), _ptr(D), n, ntot, _ptr(vmapM),
+                                  _ptr(vgeo), _ptr(sgeo), _ptr(D), n, ntot, pM,
                                   _ptr(vmapP), _ptr(elemtobndy), float(rka), float(rkb),
                                   float(dt), _phys(phys), _stream(stream))
     check(rc, "stage_fused")
@@ -153,13 +161,13 @@ def facerhs_update(dim, N, rhs, Qin, Qout, vgeo, sgeo, elems, vmapM, vmapP, elem
     _need(Qout, "Qout", Np * ns * n)
     _need(vgeo, "vgeo", Np * _NV[dim] * n)
     _need(sgeo, "sgeo", _NSG[dim] * Nfp * nf * n)
-    _need(vmapM, "vmapM", Nfp * nf * n, torch.int64)
+    pM = _mapM(vmapM, Nfp * nf * n)
     _need(vmapP, "vmapP", Nfp * nf * n, torch.int64)
     _need(elemtobndy, "elemtobndy", nf * n, torch.int64)
     if Qin.data_ptr() == Qout.data_ptr():
         raise ValueError("facerhs_update needs distinct Qin and Qout")
     rc = lib().canary_facerhs_update(eqn, dim, N, prec, _ptr(rhs), _ptr(Qin), _ptr(Qout),
-                                     _ptr(vgeo), _ptr(sgeo), n, ntot, _ptr(vmapM), _ptr(vmapP),
+                                     _ptr(vgeo), _ptr(sgeo), n, ntot, pM, _ptr(vmapP),
                                      _ptr(elemtobndy), float(rka), float(rkb), float(dt),
                                      _phys(phys), _stream(stream))
     check(rc, "facerhs_update")
@@ -181,7 +189,7 @@ def lsrk54(dim, N, Q, rhs, vgeo, sgeo, D, elems, vmapM, vmapP, elemtobndy, dt, n
     _need(vgeo, "vgeo", Np * _NV[dim] * n)
     _need(sgeo, "sgeo", _NSG[dim] * Nfp * nf * n)
     _need(D, "D", (N + 1) ** 2)
-    _need(vmapM, "vmapM", Nfp * nf * n, torch.int64)
+    pM = _mapM(vmapM, Nfp * nf * n)
     _need(vmapP, "vmapP", Nfp * nf * n, torch.int64)
     _need(elemtobndy, "elemtobndy", nf * n, torch.int64)
     if path != _lib.PATH_KERNELS:
@@ -191,7 +199,7 @@ def lsrk54(dim, N, Q, rhs, vgeo, sgeo, D, elems, vmapM, vmapP, elemtobndy, dt, n
     alt = C.c_int(0)
     rc = lib().canary_lsrk54(eqn, dim, N, prec, path, int(nsteps), float(dt), _ptr(Q),
                              _ptr(Qalt) if Qalt is not None else None, _ptr(rhs), _ptr(vgeo),
-                             _ptr(sgeo), _ptr(D), n, ntot, _ptr(vmapM), _ptr(vmapP),
+                             _ptr(sgeo), _ptr(D), n, ntot, pM, _ptr(vmapP),
                              _ptr(elemtobndy), _phys(phys), 1 if advection else 0,
                              1 if graph else 0, C.byref(alt), _stream(stream))
     check(rc, "lsrk54")

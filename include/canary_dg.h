@@ -14,6 +14,12 @@
  *                                            2D: nx ny sMJ vMJI (4)
  *   vmapM/P     (Nfp, nface, nelem) int64    1-based global node ids
  *   elemtobndy  (nface, nelem) int64         0 = connected, 1 = no-flux wall
+ *
+ * vmapM may be NULL in canary_facerhs, canary_stage_fused,
+ * canary_facerhs_update and canary_lsrk54: the M-side ids are then the
+ * canonical face mask, vmapM[:,f,e] = Np*(e-1) + fmask[:,f], the only vmapM
+ * the reference's `mappings` builds (src/mesh.jl:973-1001, :994), and the
+ * map is not read (one Int64 per face node less to move; same results).
  *   D           (Nq, Nq) column-major        D[j + Nq*k] = l_k'(r_j)
  *
  * Node n of an element is n = i + Nq*j (+ Nq^2*k), 0-based here.
@@ -257,6 +263,26 @@ int canary_p32_batch_variant(int dim, int N, int which, int* epw, int* threads);
 int canary_p32_batch_stage(int dim, int N, int epw, const canary_p32_solve* solves,
                            const canary_batch_item* items, int64_t nitems, int parity,
                            double rka, double rkb, void* stream);
+
+/* nsteps LSRK54 steps of a whole batch from C: for every stage (the
+ * reference's stage loop, euler3d_set2c.jl:819-855), each class's entries in
+ * turn, then the parity flips.  kind CANARY_BATCH_DROPIN: canary_batch_stage
+ * with (eqn, dim, prec) and a canary_batch_solve table; kind
+ * CANARY_BATCH_P32: canary_p32_batch_stage with dim and a canary_p32_solve
+ * table.  `classes` is a host array; its items pointers are device memory.
+ * No reference counterpart: it takes the per-stage host loop off the
+ * launch path for batches of many small solves.  Returns the first error. */
+typedef struct canary_batch_class {
+  const canary_batch_item* items; /* this class's work-table entries (device) */
+  int64_t nitems;
+  int32_t N;
+  int32_t epw;
+} canary_batch_class;
+#define CANARY_BATCH_DROPIN 0
+#define CANARY_BATCH_P32 1
+int canary_batch_run(int kind, int eqn, int dim, int prec, const void* solves,
+                     const canary_batch_class* classes, int nclasses, int64_t nsteps, int parity,
+                     void* stream);
 
 /* Synchronises `stream`, returns the first recorded device error
  * (CANARY_ERR_BC / CANARY_ERR_VMAP or 0), writes (code, face, element,

@@ -87,6 +87,17 @@ def test_lsrk54_tables_match_the_host_ones():
                              None) == 0 and alt.value == 0  # no steps
 
 
+def test_batch_run_rejects_bad_arguments_before_launch():
+    lib = L.lib()
+    cls = (L.BatchClass * 1)()
+    assert lib.canary_batch_run(7, 0, 3, 0, None, cls, 1, 1, 0, None) == L.ERR_ARG  # kind
+    assert lib.canary_batch_run(L.BATCH_DROPIN, 0, 3, 0, None, cls, 1, 1, 2, None) == L.ERR_ARG
+    assert lib.canary_batch_run(L.BATCH_P32, 0, 3, 0, None, cls, -1, 1, 0, None) == L.ERR_ARG
+    assert lib.canary_batch_run(L.BATCH_DROPIN, 0, 3, 0, None, cls, 1, 1, 0, None) == L.ERR_ARG
+    assert lib.canary_batch_run(L.BATCH_DROPIN, 0, 3, 0, None, cls, 1, 0, 0, None) == 0  # no steps
+    assert lib.canary_batch_run(L.BATCH_P32, 0, 3, 0, None, None, 0, 3, 1, None) == 0  # no classes
+
+
 def test_phys_defaults_are_the_reference_rationals():
     p = L.default_phys()
     assert (p.gamma, p.p0, p.R_gas, p.c_p, p.c_v, p.gravity) == (14 / 10, 100000.0, 28717 / 100,

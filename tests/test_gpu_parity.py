@@ -226,6 +226,55 @@ def test_facerhs_update_equals_face_then_update(cuda, eqn, dim, N):
             assert rel_l2(host(qout, Q.shape), host(q1, Q.shape)) <= 1e-12
 
 
+@pytest.mark.parametrize("eqn,dim,N", [(0, 3, 4), (0, 3, 1), (0, 3, 6), (0, 2, 4), (0, 2, 7),
+                                       (1, 2, 3), (1, 2, 5)])
+def test_null_vmapM_is_the_canonical_face_mask(cuda, eqn, dim, N):
+    """vmapM = None (computed M-side ids) gives the same bits as the map in
+    facerhs, the fused stage (both precisions), the face + update fusion and
+    the native loop, on a mesh with walls and a periodic direction."""
+    print("canonical vmapM eqn=%d dim=%d N=%d" % (eqn, dim, N), flush=True)
+    s = case_setup(eqn, dim, N, periodic=(True,) + (False,) * (dim - 1))
+    ds = DeviceSetup(s)
+    Q = np.asfortranarray(state(s, seed=11))
+    rhs0 = np.asfortranarray(np.random.default_rng(12).standard_normal(Q.shape) * 10)
+    if eqn == 1:
+        rhs0[:, 3, :] = 0.0
+    A, B, _ = P.rk_coefs()
+    out = {}
+    for key, mM in (("map", ds.vmapM), ("map again", ds.vmapM), ("null", None)):
+        r = dev(rhs0)
+        K.facerhs(dim, N, r, dev(Q), ds.sgeo, s.nelem, mM, ds.vmapP, ds.bndy, eqn=eqn,
+                  phys=s.phys)
+        res = {"facerhs rhs": r}
+        for prec in (FP64, FP64_FMA):
+            qo, r = dev(np.zeros_like(Q)), dev(rhs0)
+            K.stage_fused(dim, N, r, dev(Q), qo, ds.vgeo, ds.sgeo, ds.D, s.nelem, mM, ds.vmapP,
+                          ds.bndy, A[4], B[3], 0.01, eqn=eqn, phys=s.phys, prec=prec)
+            res["stage %d Q" % prec], res["stage %d rhs" % prec] = qo, r
+            qo, r = dev(np.zeros_like(Q)), dev(rhs0)
+            K.facerhs_update(dim, N, r, dev(Q), qo, ds.vgeo, ds.sgeo, s.nelem, mM, ds.vmapP,
+                             ds.bndy, A[4], B[3], 0.01, eqn=eqn, phys=s.phys, prec=prec)
+            res["face+update %d Q" % prec], res["face+update %d rhs" % prec] = qo, r
+        q, qa, r = dev(Q), dev(np.zeros_like(Q)), dev(s.zeros_state())
+        res["lsrk54 Q"] = K.lsrk54(dim, N, q, r, ds.vgeo, ds.sgeo, ds.D, s.nelem, mM, ds.vmapP,
+                                   ds.bndy, 1e-4 if eqn == 1 else 0.005, 3, eqn=eqn, phys=s.phys,
+                                   prec=FP64_FMA, Qalt=qa)
+        K.check_errors()
+        out[key] = {k: t.cpu() for k, t in res.items()}
+        assert all(bool(torch.isfinite(t).all()) for t in out[key].values())
+    bad = []
+    for key in ("map again", "null"):
+        for k, a in out["map"].items():
+            b = out[key][k]
+            if not torch.equal(a, b):
+                d = (a - b).abs()
+                bad.append("%s / %s: %d of %d differ, max %.3e, nan %d/%d" % (
+                    key, k, int((a != b).sum()), a.numel(), float(d[torch.isfinite(d)].max())
+                    if bool(torch.isfinite(d).any()) else float("nan"),
+                    int(torch.isnan(a).sum()), int(torch.isnan(b).sum())))
+    assert not bad, "; ".join(bad)
+
+
 def run_gpu_lsrk(s, ds, Q, dt, nsteps, fused):
     A, B, _ = P.rk_coefs()
     q = dev(Q)

@@ -246,3 +246,35 @@ def test_connectivity_invariants(Ne, per):
     assert np.array_equal(vPf[~conn], vMf[~conn])
     cen = m["elemtocoord"].mean(axis=1)
     assert len({tuple(np.round(c, 9)) for c in cen.T}) == cen.shape[1]
+
+
+def _fmask(dim, N):
+    """Face mask the way src/mesh.jl:978-982 states it: the 1-based node
+    numbers reshaped to (Nq,)*dim, dimension f//2 fixed at 0 or N, the rest
+    flattened column-major."""
+    Nq = N + 1
+    p = np.arange(1, Nq ** dim + 1).reshape((Nq,) * dim, order="F")
+    cols = []
+    for f in range(2 * dim):
+        sl = [slice(None)] * dim
+        sl[f // 2] = (f % 2) * N
+        cols.append(p[tuple(sl)].ravel(order="F"))
+    return np.stack(cols, axis=1)  # (Nfp, nface)
+
+
+@pytest.mark.parametrize("dim,N", [(2, 1), (2, 4), (2, 7), (3, 1), (3, 3), (3, 4), (3, 7)])
+def test_vmapM_is_always_the_canonical_face_mask(dim, N):
+    """The contract behind a null vmapM: every vmapM the mesh code builds
+    (walls, periodic, warped, a rank share with ghosts) is
+    Np*(e-1) + fmask[:, f]."""
+    from canary_amd import problems as P
+    from states import small_setup, warped_setup
+    fm = _fmask(dim, N)
+    per = (True,) + (False,) * (dim - 1)
+    shared = P.rank_share(small_setup(0, dim, N, Ne=4, periodic=per), 3, 1)
+    for vM, Np, ne in [(s.vmapM, s.Np, s.nelem) for s in
+                       (small_setup(0, dim, N, Ne=3), small_setup(0, dim, N, Ne=3, periodic=per),
+                        warped_setup(0, dim, N, Ne=3))] + [(shared.vmapM, (N + 1) ** dim,
+                                                            shared.nreal)]:
+        want = fm[:, :, None] + Np * np.arange(ne)[None, None, :]
+        assert np.array_equal(np.asarray(vM).reshape(want.shape, order="F"), want)
