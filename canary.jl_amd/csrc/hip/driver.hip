@@ -93,6 +93,22 @@ extern "C" int canary_batch_run(int kind, int eqn, int dim, int prec, const void
   if (nclasses < 0 || nsteps < 0 || (parity != 0 && parity != 1)) return CANARY_ERR_ARG;
   if (nsteps == 0 || nclasses == 0) return 0;
   if (!solves || !classes) return CANARY_ERR_ARG;
+  // every class checked before the first launch, so a bad one launches nothing
+  for (int c = 0; c < nclasses; ++c) {
+    const canary_batch_class& cl = classes[c];
+    if (cl.nitems < 0) return CANARY_ERR_ARG;
+    if (cl.nitems == 0) continue;
+    if (!cl.items) return CANARY_ERR_ARG;
+    bool found = false;
+    for (int which = 0; which < 3 && !found; ++which) {
+      int e = 0, t = 0;
+      const int rc = kind == CANARY_BATCH_DROPIN
+                         ? canary_batch_variant(eqn, dim, prec, cl.N, which, &e, &t)
+                         : canary_p32_batch_variant(dim, cl.N, which, &e, &t);
+      found = rc == 0 && e == cl.epw;
+    }
+    if (!found) return CANARY_ERR_UNSUPPORTED;
+  }
   double A[5], B[5], C[5];
   lsrk54_tables(A, B, C);
   int par = parity;

@@ -271,7 +271,9 @@ int canary_p32_batch_stage(int dim, int N, int epw, const canary_p32_solve* solv
  * CANARY_BATCH_P32: canary_p32_batch_stage with dim and a canary_p32_solve
  * table.  `classes` is a host array; its items pointers are device memory.
  * No reference counterpart: it takes the per-stage host loop off the
- * launch path for batches of many small solves.  Returns the first error. */
+ * launch path for batches of many small solves.  Every class is checked
+ * (compiled (N, epw) variant, items pointer) before the first launch, so an
+ * argument error launches nothing; a launch error returns at once. */
 typedef struct canary_batch_class {
   const canary_batch_item* items; /* this class's work-table entries (device) */
   int64_t nitems;

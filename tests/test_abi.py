@@ -96,6 +96,17 @@ def test_batch_run_rejects_bad_arguments_before_launch():
     assert lib.canary_batch_run(L.BATCH_DROPIN, 0, 3, 0, None, cls, 1, 1, 0, None) == L.ERR_ARG
     assert lib.canary_batch_run(L.BATCH_DROPIN, 0, 3, 0, None, cls, 1, 0, 0, None) == 0  # no steps
     assert lib.canary_batch_run(L.BATCH_P32, 0, 3, 0, None, None, 0, 3, 1, None) == 0  # no classes
+    # a class with no compiled (N, epw) variant is refused before anything runs
+    # (the items pointer is only checked for null, never read on the host)
+    solves = ctypes.c_void_p(64)
+    for kind, N, epw in ((L.BATCH_DROPIN, 4, 999), (L.BATCH_DROPIN, 9, 1), (L.BATCH_P32, 4, 999)):
+        cls = (L.BatchClass * 2)()
+        cls[0].items, cls[0].nitems, cls[0].N, cls[0].epw = 64, 0, 4, 1
+        cls[1].items, cls[1].nitems, cls[1].N, cls[1].epw = 64, 5, N, epw
+        assert lib.canary_batch_run(kind, 0, 3, 0, solves, cls, 2, 1, 0, None) == L.ERR_UNSUPPORTED
+    cls = (L.BatchClass * 1)()
+    cls[0].items, cls[0].nitems, cls[0].N, cls[0].epw = None, 5, 4, 1
+    assert lib.canary_batch_run(L.BATCH_DROPIN, 0, 3, 0, solves, cls, 1, 1, 0, None) == L.ERR_ARG
 
 
 def test_phys_defaults_are_the_reference_rationals():
