@@ -17,7 +17,8 @@ extern "C" int canary_k_batch_fma(int eqn, int dim, int N, int which, int want, 
     return run<Euler2DFma>(N, which, want, epw, threads, solves, items, nitems, parity, rka, rkb,
                            st);
   if (eqn == CANARY_SWE && dim == 2)
-    return run<SWE2D>(N, which, want, epw, threads, solves, items, nitems, parity, rka, rkb, st);
+    return run<SWE2DFma>(N, which, want, epw, threads, solves, items, nitems, parity, rka, rkb,
+                         st);
   return CANARY_ERR_UNSUPPORTED;
 }
 

@@ -295,6 +295,13 @@ struct SWE2D {
   }
 };
 
+// The same physics for the contracted unit: a type of its own, so that the
+// kernels instantiated there get their own names.  Kernel names are shared by
+// every unit that instantiates a template with the same arguments, and the
+// host launch stub is one symbol; with one type for both units, launches
+// from the contracted unit ran the exact-order unit's code.
+struct SWE2DFma : SWE2D {};
+
 // --------------------------------------------- shallow water, FP32 variant
 // The same fluxes as SWE2D evaluated in FP32 (batched FP32 ensembles).
 // b = still depth is read only and h is already the deviation from it, so

@@ -4,4 +4,4 @@
 // swe2d.hip.
 #include "launch.h"
 
-CANARY_DEFINE_FMA_STAGE(swe2d, canary::SWE2D)
+CANARY_DEFINE_FMA_STAGE(swe2d, canary::SWE2DFma)

@@ -113,3 +113,28 @@ def test_phys_defaults_are_the_reference_rationals():
     p = L.default_phys()
     assert (p.gamma, p.p0, p.R_gas, p.c_p, p.c_v, p.gravity) == (14 / 10, 100000.0, 28717 / 100,
                                                                   100467 / 100, 7175 / 10, 10.0)
+
+
+def test_no_kernel_is_instantiated_in_two_units():
+    """Each unit has its own code object, but a kernel's host launch stub is
+    one (weak) symbol across the library: a kernel instantiated in two units
+    (say the exact-order and the contracted one) launches whichever unit's
+    code the loader picked.  Every stub must come from one unit."""
+    import shutil
+    import subprocess
+    import pytest
+    build = os.path.join(ROOT, "canary.jl_amd", "_build")
+    objs = sorted(glob.glob(os.path.join(build, "hip_*.o")))
+    if not objs or shutil.which("nm") is None:
+        pytest.skip("no built objects or no nm")
+    owner = {}
+    for o in objs:
+        out = subprocess.run(["nm", "--defined-only", o], capture_output=True, text=True,
+                             check=True).stdout
+        for line in out.splitlines():
+            parts = line.split()
+            if len(parts) == 3 and "__device_stub__" in parts[2]:
+                owner.setdefault(parts[2], set()).add(os.path.basename(o))
+    assert len(owner) > 100
+    shared = {s: u for s, u in owner.items() if len(u) > 1}
+    assert not shared, list(shared.items())[:3]
