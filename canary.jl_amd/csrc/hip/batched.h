@@ -70,7 +70,7 @@ __global__ __launch_bounds__((BCfg<Phys, NQ, EPW>::NT)) void batched_kernel(
   using C = FusedCfg<Phys, NQ, EPW, false>;
   using Real = typename Phys::Real;
   __shared__ __attribute__((aligned(16))) char lds[C::LDS];
-  const int64_t w = xcd_block(blockIdx.x, chunk);
+  const int64_t w = stage_block(blockIdx.x, chunk, parity);  // direction alternates per stage
   if (w >= nitems) return;
   const canary_batch_item it = items[w];
   const canary_batch_solve& sv = solves[it.solve];

@@ -97,6 +97,16 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t chunk) {
   return (b & 7) * chunk + (b >> 3);
 }
 
+// The same over a launch of 8*chunk workgroups walked backwards when
+// `reverse` is set.  Stages that ping-pong Q alternate the direction (the
+// caller derives it from the buffer order, or the stage parity), so each
+// stage starts on the elements the previous one touched last, whose lines
+// the memory-side cache still holds (3D N=4, 32^3: 3.8% faster per stage).
+// Each element's result does not depend on the order.
+__device__ __forceinline__ int64_t stage_block(int64_t b, int64_t chunk, int reverse) {
+  return xcd_block(reverse ? 8 * chunk - 1 - b : b, chunk);
+}
+
 // ------------------------------------------------------------------ sweeps
 // r[s] += sum_m D[m, i] F[m, j, k, s] (xi), then eta, then zeta, with the
 // reference CPU order.  sF[d] points at the flux of direction d,
@@ -661,10 +671,10 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, WIDE>::NT)) void stage_ker
     const double* __restrict__ D, int64_t nreal, int64_t ntotal,
     const int64_t* __restrict__ vmapM, const int64_t* __restrict__ vmapP,
     const int64_t* __restrict__ bndy, double rka, double rkb, double dt, int64_t chunk,
-    PhysParams p) {
+    int reverse, PhysParams p) {
   using C = FusedCfg<Phys, NQ, EPW, WIDE>;
   __shared__ __attribute__((aligned(16))) char lds[C::LDS];
-  const int64_t blk = xcd_block(blockIdx.x, chunk);
+  const int64_t blk = stage_block(blockIdx.x, chunk, reverse);
   const int64_t e0 = blk * EPW;
   if (e0 >= nreal) return;  // whole workgroup leaves together
   stage_body<Phys, NQ, EPW, C::NT, PF, PM_LDS>(lds, rhs, Qin, Qout, vgeo, sgeo, D, e0, nreal,
@@ -699,10 +709,10 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, false>::NT)) void face_upd
     const double* __restrict__ vgeo, const double* __restrict__ sgeo, int64_t nreal,
     int64_t ntotal, const int64_t* __restrict__ vmapM, const int64_t* __restrict__ vmapP,
     const int64_t* __restrict__ bndy, double rka, double rkb, double dt, int64_t chunk,
-    PhysParams p) {
+    int reverse, PhysParams p) {
   using C = FusedCfg<Phys, NQ, EPW, false>;
   __shared__ __attribute__((aligned(16))) char lds[C::LDS];
-  const int64_t blk = xcd_block(blockIdx.x, chunk);
+  const int64_t blk = stage_block(blockIdx.x, chunk, reverse);
   const int64_t e0 = blk * EPW;
   if (e0 >= nreal) return;  // whole workgroup leaves together
   stage_body<Phys, NQ, EPW, C::NT, 1, true, false>(lds, rhs, Qin, Qout, vgeo, sgeo, nullptr, e0,

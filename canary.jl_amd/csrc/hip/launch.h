@@ -34,6 +34,12 @@ inline int launch_status() {
   return err == hipSuccess ? 0 : 3 /*CANARY_ERR_LAUNCH*/;
 }
 
+// Element order of a ping-pong stage: the two Q buffers swap roles every
+// stage, so their address order alternates and so does the direction.
+inline int walk_back(const void* qin, const void* qout) {
+  return (uintptr_t)qin > (uintptr_t)qout ? 1 : 0;
+}
+
 template <class Phys, int NQ, int EPW>
 void launch_stage(int64_t nb, const KArgs& a) {
   // measured on 3D N=4 (32^3): one prefetched face item per thread, the
@@ -46,7 +52,7 @@ void launch_stage(int64_t nb, const KArgs& a) {
   stage_kernel<Phys, NQ, EPW, WIDE, PF, PM_LDS><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
       (double*)a.rhs, (const double*)a.Q, (double*)a.Qout, (const double*)a.vgeo,
       (const double*)a.sgeo, a.D, a.nreal, a.ntotal, a.vmapM, a.vmapP, a.bndy, a.rka, a.rkb,
-      a.dt, chunk, a.phys);
+      a.dt, chunk, walk_back(a.Q, a.Qout), a.phys);
 }
 
 template <class Phys, int NQ, int EPW>
@@ -56,7 +62,7 @@ void launch_face_update(int64_t nb, const KArgs& a) {
   face_update_kernel<Phys, NQ, EPW><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
       (double*)a.rhs, (const double*)a.Q, (double*)a.Qout, (const double*)a.vgeo,
       (const double*)a.sgeo, a.nreal, a.ntotal, a.vmapM, a.vmapP, a.bndy, a.rka, a.rkb, a.dt,
-      chunk, a.phys);
+      chunk, walk_back(a.Q, a.Qout), a.phys);
 }
 
 // STAGE_ONLY: the fused kernels alone (stage, face + update) are

@@ -387,10 +387,10 @@ __global__ __launch_bounds__((Cfg32<DIM, NQ, single_epw<DIM, NQ>()>::NT)) void s
     float* __restrict__ rhs, const float* __restrict__ qin, float* __restrict__ qout,
     const double* __restrict__ bg, const float* __restrict__ vg, const float* __restrict__ sg,
     const int32_t* __restrict__ nbr, const int8_t* __restrict__ bnd, const double* __restrict__ D,
-    int64_t nreal, float rka, double bdt, int64_t chunk, P32Params p) {
+    int64_t nreal, float rka, double bdt, int64_t chunk, int reverse, P32Params p) {
   constexpr int EPW = single_epw<DIM, NQ>();
   __shared__ Lds32<DIM, NQ, EPW> L;
-  const int64_t e0 = xcd_block(blockIdx.x, chunk) * EPW;
+  const int64_t e0 = stage_block(blockIdx.x, chunk, reverse) * EPW;
   if (e0 >= nreal) return;
   body<DIM, NQ, EPW>(L, rhs, qin, qout, bg, vg, sg, nbr, bnd, D, e0, nreal, rka, bdt, p);
 }
@@ -401,7 +401,7 @@ __global__ __launch_bounds__((Cfg32<DIM, NQ, EPW>::NT)) void batch32_kernel(
     const canary_p32_solve* __restrict__ solves, const canary_batch_item* __restrict__ items,
     int64_t nitems, int64_t chunk, int parity, double rka, double rkb) {
   __shared__ Lds32<DIM, NQ, EPW> L;
-  const int64_t w = xcd_block(blockIdx.x, chunk);
+  const int64_t w = stage_block(blockIdx.x, chunk, parity);  // direction alternates per stage
   if (w >= nitems) return;
   const canary_batch_item it = items[w];
   const canary_p32_solve& sv = solves[it.solve];
@@ -447,7 +447,9 @@ int stage(void* rhs, const void* qin, void* qout, const void* bg, const void* vg
   P32Params p{(float)ph->gamma, (float)(ph->c_p / ph->c_v), (float)ph->gravity};
   stage32_kernel<DIM, NQ><<<dim3((unsigned)(8 * chunk)), dim3(C::NT), 0, st>>>(
       (float*)rhs, (const float*)qin, (float*)qout, (const double*)bg, (const float*)vg,
-      (const float*)sg, nbr, bnd, D, nreal, (float)rka, rkb * dt, chunk, p);
+      (const float*)sg, nbr, bnd, D, nreal, (float)rka, rkb * dt, chunk,
+      (uintptr_t)qin > (uintptr_t)qout ? 1 : 0,  // q ping-pong: direction alternates
+      p);
   return launch_status();
 }
 
