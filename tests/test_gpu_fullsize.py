@@ -164,3 +164,35 @@ def test_fullsize_conservation(big):
         assert abs(a - b) <= 1e-13 * abs(b)
     # the state did change (the bubble rises): momentum is not zero any more
     assert float(fin.view(s.nelem, 5, Np)[:, 2, :].abs().max()) > 0
+
+
+def test_precision_form_at_48cubed(cuda):
+    """The precision form's benchmark size (48^3, N = 4, 69.12 M DOF): after
+    20 LSRK54 steps its full state (FP64 background + FP32 perturbation) is
+    within 1e-5 relative L2 of the FP64 fused path, per field, and mass
+    sum(MJ rho) moves by less than the FP32 rounding of the perturbation."""
+    from canary_amd.perturb import Perturb32
+    print("precision form 48^3 N=4", flush=True)
+    s = P.bubble3d(Ne=48, N=4, gravity=9.81, dtheta=2.0, jitter=1e-3, seed=1)
+    Q0 = P.bubble_state(s, dtheta=0.0, jitter=0.0)
+    dt = P.cfl(s, s.Q) / s.N ** np.sqrt(2)
+    ds = DeviceSetup(s)
+    q, q2, r = dev(s.Q), torch.empty(s.Q.size, dtype=torch.float64, device="cuda"), \
+        dev(s.zeros_state())
+    fin = K.lsrk54(3, s.N, q, r, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM, ds.vmapP, ds.bndy,
+                   dt, 20, phys=s.phys, path=PATH_FUSED, prec=FP64, Qalt=q2)
+    K.check_errors()
+    want = host(fin, s.Q.shape)
+    del q, q2, r, fin, ds
+    torch.cuda.empty_cache()
+    pf = Perturb32(s, Q0, s.Q)
+    pf.run(dt, 20)
+    pf.check()
+    got = pf.full_state().cpu().numpy().reshape(s.Q.shape, order="F")
+    errs = {k: np.linalg.norm(got[:, k] - want[:, k]) / np.linalg.norm(want[:, k])
+            for k in range(5)}
+    print("relative L2 per field:", errs, flush=True)
+    assert max(errs.values()) <= 1e-5
+    MJ = s.vgeo[:, 9, :]
+    m0, m1 = float((MJ * s.Q[:, 3, :]).sum()), float((MJ * got[:, 3, :]).sum())
+    assert abs(m1 - m0) <= 1e-9 * abs(m0)
