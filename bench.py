@@ -49,6 +49,74 @@ def algorithmic_bytes(dim, N, nstate=None, word=8):
     return word * (4 * ns * Np + nvol * Np + nface * Nfp * nsurf) + 2 * nface * Nfp * 8 + nface * 8
 
 
+def swe_bytes(N):
+    """Shallow-water bytes per element-stage by the same rule: Q read with b
+    (4 fields), written without it (3), dQ r+w (3 fields), 4 metric terms +
+    MJ + MJI, normals + sMJ, vmapM + vmapP and elemtobndy."""
+    Np, Nfp = (N + 1) ** 2, N + 1
+    return 8 * ((4 + 3 + 2 * 3) * Np + 6 * Np + 4 * Nfp * 3) + 2 * 4 * Nfp * 8 + 4 * 8
+
+
+def two_d_form(ne, N, steps, warmup):
+    """Single large 2D solves through the library's LSRK54 loop (fused
+    stage, launched from C): the Euler set2c isentropic vortex and shallow
+    water with a bump, ne^2 quads, contracted and exact-order arithmetic."""
+    import numpy as np
+    import torch
+    import canary_amd.kernels as K
+    import canary_amd.problems as P
+    from canary_amd._lib import FP64, FP64_FMA, PATH_FUSED, SWE, EULER
+    from canary_amd.streams import OwnedStream
+
+    dev = torch.device("cuda:0")
+
+    def d(a):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to(dev)
+
+    own = OwnedStream(dev)
+    out = {}
+    for name, make in (("euler2d_vortex", lambda: P.vortex2d(Ne=ne, N=N)),
+                       ("swe2d_bump", lambda: P.swe2d(Ne=ne, N=N))):
+        s = make()
+        eqn = EULER if name.startswith("euler") else SWE
+        dt = (0.5 * P.cfl(s, s.Q) / N ** np.sqrt(2)) if eqn == EULER else P.swe_courant(s, s.Q)
+        vgeo, sgeo, D = d(s.vgeo), d(s.sgeo), d(s.D)
+        vM, vP, bnd = d(s.vmapM), d(s.vmapP), d(s.elemtobndy)
+        Q0 = d(s.Q)
+        dof = s.nelem * s.Np * (3 if eqn == SWE else 4)
+        nbytes = (swe_bytes(N) if eqn == SWE else algorithmic_bytes(2, N)) * s.nelem
+        res = {"workload": "%s, %d^2 quads, N=%d, periodic" % (s.label, ne, N), "dof": dof,
+               "bytes_per_element_stage": nbytes // s.nelem}
+        for tag, prec in (("fp64_fma", FP64_FMA), ("fp64", FP64)):
+            q, q2, r = Q0.clone(), torch.empty_like(Q0), torch.zeros_like(Q0)
+            torch.cuda.synchronize()
+
+            def loop(n):
+                with torch.cuda.stream(own.stream):
+                    return K.lsrk54(2, N, q, r, vgeo, sgeo, D, s.nelem, vM, vP, bnd, dt, n,
+                                    eqn=eqn, phys=s.phys, path=PATH_FUSED, prec=prec, Qalt=q2)
+            loop(warmup)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(own.stream)
+            fin = loop(steps)
+            e1.record(own.stream)
+            torch.cuda.synchronize()
+            K.check_errors()
+            if not bool(torch.isfinite(fin).all()):
+                raise SystemExit("non-finite 2D state")
+            t = e0.elapsed_time(e1) / 1e3 / (5 * steps)
+            res[tag] = {"us_per_stage": round(t * 1e6, 2),
+                        "gdof_stage_s": round(dof / t / 1e9, 3),
+                        "hbm_frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4)}
+            del q, q2, r, fin
+        out[name] = res
+        print("2D form %s: %s" % (name, res["fp64_fma"]), file=sys.stderr, flush=True)
+        del vgeo, sgeo, D, vM, vP, bnd, Q0, s
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +131,9 @@ def main():
                     help="elements per direction of the FP32-vs-FP64 precision-form run (0: skip)")
     ap.add_argument("--precision-steps", type=str, default="1,100,1000",
                     help="LSRK steps after which the FP32 form's error is reported")
+    ap.add_argument("--two-d-ne", type=int, default=512,
+                    help="quads per direction of the single-solve 2D Euler / shallow-water "
+                         "runs (0: skip)")
     ap.add_argument("--batched-solves", type=int, default=64,
                     help="shallow-water solves of the batched-form run (0: skip)")
     ap.add_argument("--batched-steps", type=int, default=20)
@@ -315,6 +386,8 @@ def main():
         torch.cuda.empty_cache()
         extra["precision_form"] = precision_form(args.precision_ne, s.N, args.precision_steps)
     torch.cuda.empty_cache()
+    if args.two_d_ne > 0:
+        extra["two_d_single_solves"] = two_d_form(args.two_d_ne, s.N, args.steps, args.warmup)
     if args.batched_solves > 0:
         extra["batched_form"] = form_child("batched", ["--batched-solves", str(args.batched_solves),
                                                        "--batched-steps", str(args.batched_steps)])
