@@ -11,7 +11,8 @@ not depend on size:
   * the paths agree bit for bit: fused exact-order stage vs the three
     kernels, vmapM = NULL vs the map, graph replay vs eager launches, the
     library loop vs the Python stage loop;
-  * two LSRK54 steps against the oracle's lowstorageRK (exact order);
+  * two LSRK54 steps against the oracle's lowstorageRK (exact order), and
+    100 steps in both arithmetic forms within 1e-10 (the target's bar);
   * mass (sum of MJ rho) and rho-theta are conserved: walls, no sources for
     them, and discrete operators whose contributions cancel in pairs.
 """
@@ -139,6 +140,30 @@ def test_fullsize_two_steps_against_oracle(big):
     dQ = host(fin, s.Q.shape) - s.Q
     assert np.linalg.norm(dQ) > 0
     assert rel_l2(dQ, Qo - s.Q) <= 1e-9
+
+
+def test_fullsize_100_steps_against_oracle(big):
+    """The target's own bar at the target's own size: 100 LSRK54 steps of the
+    32^3 N=4 bubble, the library loop against the oracle's lowstorageRK
+    (about 20 s on 16 host threads), both arithmetic forms within 1e-10."""
+    s, ds, dt = big
+    print("full-size 100 LSRK54 steps vs oracle", flush=True)
+    Qo, ro = s.Q.copy(order="F"), s.zeros_state()
+    O.lsrk(s, Qo, ro, dt, 100, nthreads=16)
+    for prec in (FP64, FP64_FMA):
+        q, q2, r = dev(s.Q), torch.empty(s.Q.size, dtype=torch.float64, device="cuda"), \
+            dev(s.zeros_state())
+        fin = K.lsrk54(3, s.N, q, r, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM, ds.vmapP,
+                       ds.bndy, dt, 100, phys=s.phys, path=PATH_FUSED, prec=prec, Qalt=q2)
+        K.check_errors()
+        got = host(fin, s.Q.shape)
+        err = rel_l2(got, Qo)
+        print("prec %d: relative L2 after 100 steps %.3e" % (prec, err), flush=True)
+        assert err <= 1e-10
+        # per field, so the small momenta are held to the bar on their own
+        for k in range(5):
+            assert rel_l2(got[:, k], Qo[:, k]) <= 1e-10, k
+        del q, q2, r, fin
 
 
 def test_fullsize_conservation(big):
