@@ -7,6 +7,7 @@ and the shallow-water driver).
     i = b.add(setup, dt)
     b.finalize(autotune=True)     # work table; elements per workgroup chosen per N by timing
     b.run(nsteps)                 # LSRK54 steps, one hipGraph replay per step
+    b.run(nsteps, graph=False)    # the same steps from the library's own loop
     Q = b.state(i)                # float64 (Np, nstate, nelem), reference layout
 
 Solves may differ in N, mesh, element count, dt and physical constants.  A

@@ -13,8 +13,9 @@ A request names an equation set / problem, a mesh size, N and an end time
   * keeps every admitted state resident on the GPU in one batch per
     dimension (Batch32: FP32 perturbation form; or Batch: plain FP64), so
     the requests on the GPU advance together, the 2D and 3D batches on their
-    own streams (launched stage by stage: replaying captured step graphs
-    inside this loop is switched off, see Server),
+    own streams (k steps per call of the library's batch loop,
+    canary_batch_run: replaying captured step graphs inside this loop is
+    switched off, see Server),
   * retires a request when it reaches its end time (its state is copied to
     the host) and admits waiting ones in the freed room.
 
