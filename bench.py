@@ -730,6 +730,21 @@ def served_form(nreq, rate, seed=2024):
     wall = time.time() - t0
     st = summary(out_reqs, srv.makespan)
     print("served %d requests in %.1f s" % (nreq, wall), file=sys.stderr, flush=True)
+    # the same stream with work tiers: requests of at most 2e8 DOF-steps in
+    # a batch of their own, the larger ones advanced every 4th iteration
+    # while small ones are resident; results must not change
+    srv_t = Server("fp32", tier_work=2e8, tier_ratio=4)
+    out_t = srv_t.serve(poisson_stream(n=nreq, rate=rate, seed=seed))
+    st_t = summary(out_t, srv_t.makespan)
+    by_rid = {r.rid: r.result for r in out_t}
+    tiered = {k: st_t[k] for k in ("p50_latency_s", "p99_latency_s", "max_latency_s",
+                                   "makespan_s", "aggregate_gdof_stage_s")}
+    tiered = {k: round(v, 4) for k, v in tiered.items()}
+    tiered["same_results"] = all(np.array_equal(r.result, by_rid[r.rid]) for r in out_reqs)
+    del out_t, by_rid, srv_t
+    print("served with work tiers: p50 %.3f s p99 %.3f s" % (tiered["p50_latency_s"],
+                                                             tiered["p99_latency_s"]),
+          file=sys.stderr, flush=True)
     # sequential plain FP64 solves of the same requests
     timed = _gpu_timer()
     A, B, _ = P.rk_coefs()
@@ -769,6 +784,7 @@ def served_form(nreq, rate, seed=2024):
                         "hexes, N 3..5), 50-500 LSRK54 steps each; FP32 perturbation form"
                         % (nreq, rate, seed, nk["vortex2d"], nreq - nk["vortex2d"]),
             **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()},
+            "with_work_tiers": tiered,
             "sequential_fp64_gpu_s": round(t_seq, 4),
             "sequential_fp64_gdof_stage_s": round(st["dof_stages"] / t_seq / 1e9, 3),
             "fp32_rel_l2_vs_fp64": {"max": float("%.3e" % max(errs)),

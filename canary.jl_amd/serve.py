@@ -109,10 +109,18 @@ def make_setup(req):
 
 class Server:
     def __init__(self, precision="fp32", hbm_budget=64 << 30, max_solves=256, step_chunk=8,
-                 prep_workers=8, graph=False, epw=None, keep_setups=False, device="cuda"):
+                 prep_workers=8, graph=False, epw=None, keep_setups=False, device="cuda",
+                 tier_work=None, tier_ratio=2):
         if precision not in ("fp32", "fp64"):
             raise ValueError("precision is 'fp32' (perturbation form) or 'fp64'")
         self.precision = precision
+        # tier_work (DOF x steps): requests with at most this much work go to
+        # a batch of their own, advanced every iteration, the others every
+        # tier_ratio-th iteration while small ones are resident, so small
+        # requests do not wait a whole-batch step for each of their steps
+        # (None: one batch per dimension, every request in lock step)
+        self.tier_work = None if tier_work is None else float(tier_work)
+        self.tier_ratio = max(1, int(tier_ratio))
         self.hbm_budget = int(hbm_budget)
         self.max_solves = int(max_solves)
         self.step_chunk = int(step_chunk)
@@ -133,14 +141,22 @@ class Server:
         self.streams = {}
         self.trace = []              # (time, active solves, resident bytes)
 
-    def _batch(self, dim):
-        b = self.batches.get(dim)
+    def _batch(self, dim, tier=0):
+        # the tiers of one dimension share its stream: every change of a
+        # batch's tables stays ordered after that stream's launches
+        b = self.batches.get((dim, tier))
         if b is None:
             b = Batch32(dim, device=self.device) if self.precision == "fp32" else \
                 Batch(P._lib.EULER, dim, FP64, device=self.device)
-            self.batches[dim] = b
-            self.streams[dim] = OwnedStream(self.device)
+            self.batches[(dim, tier)] = b
+            if dim not in self.streams:
+                self.streams[dim] = OwnedStream(self.device)
         return b
+
+    def _tier(self, r, s):
+        if self.tier_work is None:
+            return 0
+        return 0 if s.ndof() * r.nsteps <= self.tier_work else 1
 
     def footprint(self, s):
         return self._batch(s.dim).footprint(s)
@@ -189,7 +205,7 @@ class Server:
                     ready.pop(0)
                     s, dt = make_setup_device(r, s)
                     r.dt = dt
-                    b = self._batch(s.dim)
+                    b = self._batch(s.dim, self._tier(r, s))
                     sid = b.add(s, dt)
                     s.dev = s.Qd = s.Q0d = None  # the batch holds what it needs
                     active[r.rid] = (r, b, sid, need)
@@ -197,7 +213,7 @@ class Server:
                     r.admitted = clock() - t0
                     admitted = True
                 if admitted:
-                    for dim, b in self.batches.items():
+                    for (dim, _), b in self.batches.items():
                         if b.solves and b.dirty:
                             self._finalize(dim, b)
                 if not active:
@@ -207,20 +223,27 @@ class Server:
                     elif nxt < len(reqs):
                         time.sleep(max(0.0, min(0.002, reqs[nxt].arrival - (clock() - t0))))
                     continue
-                # advance every resident solve by k steps (k <= the nearest end time)
-                k = min(self.step_chunk, min(r.nsteps - b.solves[sid].steps
-                                             for r, b, sid, _ in active.values()))
+                # advance the resident solves of each batch by k steps (k <= the
+                # nearest end time in that batch); with tiers, the large-work
+                # batch only every tier_ratio-th iteration while small ones wait
+                rem = {}
+                for r, b, sid, _ in active.values():
+                    left = r.nsteps - b.solves[sid].steps
+                    rem[id(b)] = min(rem.get(id(b), left), left)
+                small = any(b.solves for (_, t), b in self.batches.items() if t == 0)
+                it = len(self.trace)
                 cur = torch.cuda.current_stream(self.device)
-                for dim, b in self.batches.items():
+                for (dim, tier), b in self.batches.items():
                     if not b.solves:
+                        continue
+                    if tier > 0 and small and it % self.tier_ratio:
                         continue
                     st = self.streams[dim].stream
                     st.wait_stream(cur)
                     with torch.cuda.stream(st):
-                        b.run(k, graph=self.graph)
-                for dim, b in self.batches.items():
-                    if b.solves:
-                        cur.wait_stream(self.streams[dim].stream)
+                        b.run(min(self.step_chunk, rem[id(b)]), graph=self.graph)
+                for dim, st in self.streams.items():
+                    cur.wait_stream(st.stream)
                 self.trace.append((clock() - t0, len(active), resident))
                 # retire finished requests
                 fin = [rid for rid, (r, b, sid, _) in active.items()
@@ -232,7 +255,7 @@ class Server:
                     resident -= need
                     done += 1
                 if fin:
-                    for dim, b in self.batches.items():
+                    for (dim, _), b in self.batches.items():
                         if b.solves and b.dirty:
                             self._finalize(dim, b)
             for b in self.batches.values():

@@ -81,6 +81,24 @@ def test_served_in_mix_equals_served_alone(cuda):
         assert max_ulp(got.result, r.result) == 0
 
 
+def test_work_tiers_give_the_same_results(cuda):
+    """With work tiers (small requests in a batch of their own, the large
+    ones advanced every third iteration) every result equals the untiered
+    server's, bit for bit."""
+    reqs = small_stream(seed=12, n=8)
+    hosts = {r.rid: make_setup_host(r) for r in reqs}
+    works = sorted(hosts[r.rid].ndof() * r.nsteps for r in reqs)
+    flat = Server("fp32", step_chunk=2).serve(reqs, setups=hosts)
+    again = [Request(rid=r.rid, kind=r.kind, Ne=r.Ne, N=r.N, nsteps=r.nsteps,
+                     arrival=r.arrival, seed=r.seed) for r in reqs]
+    srv = Server("fp32", step_chunk=2, tier_work=works[len(works) // 2], tier_ratio=3)
+    tiered = srv.serve(again, setups=hosts)
+    assert {t for (_, t), b in srv.batches.items()} == {0, 1}
+    by = {r.rid: r for r in flat}
+    for r in tiered:
+        assert max_ulp(r.result, by[r.rid].result) == 0, r.rid
+
+
 @pytest.mark.parametrize("kind", ["bubble3d", "vortex2d"])
 def test_device_preparation_matches_host(cuda, kind):
     """Geometry bit-identical; initial states within a few ulps (device libm);
