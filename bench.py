@@ -724,26 +724,27 @@ def served_form(nreq, rate, seed=2024):
     # each new order inside the stream
     Server("fp32").serve(poisson_stream(n=8, rate=1000.0, seed=1, vortex_ne=(2, 3),
                                         bubble_ne=(2, 2), steps=(2, 3)))
-    srv = Server("fp32", keep_setups=True)
+    # work tiers: requests of at most 5e8 DOF-steps in a batch of their own,
+    # the larger ones advanced every 8th iteration while small ones are
+    # resident (swept on this stream: 2e8 / 5e8 x 4 / 8)
+    srv = Server("fp32", keep_setups=True, tier_work=5e8, tier_ratio=8)
     t0 = time.time()
     out_reqs = srv.serve(reqs)
     wall = time.time() - t0
     st = summary(out_reqs, srv.makespan)
     print("served %d requests in %.1f s" % (nreq, wall), file=sys.stderr, flush=True)
-    # the same stream with work tiers: requests of at most 2e8 DOF-steps in
-    # a batch of their own, the larger ones advanced every 4th iteration
-    # while small ones are resident; results must not change
-    srv_t = Server("fp32", tier_work=2e8, tier_ratio=4)
-    out_t = srv_t.serve(poisson_stream(n=nreq, rate=rate, seed=seed))
-    st_t = summary(out_t, srv_t.makespan)
-    by_rid = {r.rid: r.result for r in out_t}
-    tiered = {k: st_t[k] for k in ("p50_latency_s", "p99_latency_s", "max_latency_s",
-                                   "makespan_s", "aggregate_gdof_stage_s")}
-    tiered = {k: round(v, 4) for k, v in tiered.items()}
-    tiered["same_results"] = all(np.array_equal(r.result, by_rid[r.rid]) for r in out_reqs)
-    del out_t, by_rid, srv_t
-    print("served with work tiers: p50 %.3f s p99 %.3f s" % (tiered["p50_latency_s"],
-                                                             tiered["p99_latency_s"]),
+    # the same stream in lock step (one batch per dimension); results must
+    # not change
+    srv_l = Server("fp32")
+    out_l = srv_l.serve(poisson_stream(n=nreq, rate=rate, seed=seed))
+    st_l = summary(out_l, srv_l.makespan)
+    by_rid = {r.rid: r.result for r in out_l}
+    lock = {k: round(st_l[k], 4) for k in ("p50_latency_s", "p99_latency_s", "max_latency_s",
+                                           "makespan_s", "aggregate_gdof_stage_s")}
+    lock["same_results"] = all(np.array_equal(r.result, by_rid[r.rid]) for r in out_reqs)
+    del out_l, by_rid, srv_l
+    print("served in lock step: p50 %.3f s p99 %.3f s" % (lock["p50_latency_s"],
+                                                          lock["p99_latency_s"]),
           file=sys.stderr, flush=True)
     # sequential plain FP64 solves of the same requests
     timed = _gpu_timer()
@@ -781,10 +782,12 @@ def served_form(nreq, rate, seed=2024):
     nk = {"vortex2d": sum(r.kind == "vortex2d" for r in out_reqs)}
     return {"workload": "%d requests, Poisson arrivals (mean %g/s, seed %d): %d 2D isentropic "
                         "vortices (16^2-64^2 quads, N 3..7), %d 3D rising bubbles (8^3-24^3 "
-                        "hexes, N 3..5), 50-500 LSRK54 steps each; FP32 perturbation form"
+                        "hexes, N 3..5), 50-500 LSRK54 steps each; FP32 perturbation form, "
+                        "work tiers"
                         % (nreq, rate, seed, nk["vortex2d"], nreq - nk["vortex2d"]),
             **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()},
-            "with_work_tiers": tiered,
+            "work_tiers": {"tier_work_dof_steps": 5e8, "tier_ratio": 8},
+            "lock_step": lock,
             "sequential_fp64_gpu_s": round(t_seq, 4),
             "sequential_fp64_gdof_stage_s": round(st["dof_stages"] / t_seq / 1e9, 3),
             "fp32_rel_l2_vs_fp64": {"max": float("%.3e" % max(errs)),
