@@ -349,14 +349,16 @@ struct VolumeCfg {
 template <class Phys, int NQ, int EPW>
 __global__ __launch_bounds__((VolumeCfg<Phys, NQ, EPW>::NT)) void volume_kernel(
     double* __restrict__ rhs, const double* __restrict__ Q, const double* __restrict__ vgeo,
-    const double* __restrict__ D, int64_t nreal, PhysParams p) {
+    const double* __restrict__ D, int64_t nreal, int64_t chunk, int reverse, PhysParams p) {
   using S = Shape<Phys, NQ>;
   constexpr int Np = S::Np, DIM = S::DIM, NS = Phys::NS, NSR = Phys::NSR, NV = Phys::NVGEO;
   __shared__ double sD[NQ * NQ];
   __shared__ double sF[DIM][EPW][NSR * Np];
+  const int64_t blk = stage_block(blockIdx.x, chunk, reverse);
+  if (blk * EPW >= nreal) return;  // whole workgroup leaves together
   const int t = threadIdx.x;
   const int slot = t / Np, n = t - slot * Np;
-  const int64_t e = (int64_t)blockIdx.x * EPW + slot;
+  const int64_t e = blk * EPW + slot;
   const bool valid = slot < EPW && e < nreal;
   if (t < NQ * NQ) sD[t] = D[t];
   double r[NSR];
@@ -398,13 +400,15 @@ __global__ __launch_bounds__(256) void update_kernel(double* __restrict__ rhs,
                                                      double* __restrict__ Q,
                                                      const double* __restrict__ vgeo,
                                                      int64_t nreal, double rka, double rkb,
-                                                     double dt, int advection) {
+                                                     double dt, int advection, int64_t chunk,
+                                                     int reverse) {
   using S = Shape<Phys, NQ>;
   constexpr int Np = S::Np, NS = Phys::NS, NSR = Phys::NSR, NV = Phys::NVGEO;
   const int64_t total = nreal * Np;
   const double bdt = rkb * dt;
-  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
-       g += (int64_t)gridDim.x * blockDim.x) {
+  {
+    const int64_t g = stage_block(blockIdx.x, chunk, reverse) * 256 + threadIdx.x;
+    if (g >= total) return;
     const int64_t e = g / Np;
     const int n = (int)(g - e * Np);
     const double MJI = vgeo[n + Np * (Phys::iMJI + (size_t)NV * e)];
@@ -700,10 +704,10 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, false>::NT)) void face_ker
     double* __restrict__ rhs, const double* __restrict__ Q, const double* __restrict__ sgeo,
     int64_t nreal, int64_t ntotal, const int64_t* __restrict__ vmapM,
     const int64_t* __restrict__ vmapP, const int64_t* __restrict__ bndy, int64_t chunk,
-    PhysParams p) {
+    int reverse, PhysParams p) {
   using C = FusedCfg<Phys, NQ, EPW, false>;
   __shared__ __attribute__((aligned(16))) char lds[C::LDS];
-  const int64_t blk = xcd_block(blockIdx.x, chunk);
+  const int64_t blk = stage_block(blockIdx.x, chunk, reverse);
   const int64_t e0 = blk * EPW;
   if (e0 >= nreal) return;  // whole workgroup leaves together
   stage_body<Phys, NQ, EPW, C::NT, 1, true, false, false>(lds, rhs, Q, nullptr, nullptr, sgeo,
