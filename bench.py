@@ -117,6 +117,62 @@ def two_d_form(ne, N, steps, warmup):
     return out
 
 
+def sweep_form(steps=10, warmup=10):
+    """The measured loop (contracted fused stage, library LSRK54 loop) on the
+    3D bubble for N = 2..7, each at about 20 M DOF, with its roofline fraction
+    on the same byte rule."""
+    import numpy as np
+    import torch
+    import canary_amd.kernels as K
+    import canary_amd.problems as P
+    from canary_amd._lib import FP64_FMA, PATH_FUSED
+    from canary_amd.streams import OwnedStream
+
+    dev = torch.device("cuda:0")
+
+    def d(a):
+        return torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order="F"))).to(dev)
+
+    own = OwnedStream(dev)
+    out = []
+    for N, ne in ((2, 53), (3, 40), (4, 32), (5, 27), (6, 23), (7, 20)):
+        s = P.bubble3d(Ne=ne, N=N, gravity=9.81, dtheta=2.0, jitter=1e-3, seed=1)
+        dt = P.cfl(s, s.Q) / N ** np.sqrt(2)
+        vgeo, sgeo, D = d(s.vgeo), d(s.sgeo), d(s.D)
+        vM, vP, bnd = d(s.vmapM), d(s.vmapP), d(s.elemtobndy)
+        q, q2, r = d(s.Q), torch.empty(s.Q.size, dtype=torch.float64, device=dev), \
+            torch.zeros(s.Q.size, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+
+        def loop(n):
+            with torch.cuda.stream(own.stream):
+                return K.lsrk54(3, N, q, r, vgeo, sgeo, D, s.nelem, vM, vP, bnd, dt, n,
+                                phys=s.phys, path=PATH_FUSED, prec=FP64_FMA, Qalt=q2)
+        loop(warmup)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(own.stream)
+        fin = loop(steps)
+        e1.record(own.stream)
+        torch.cuda.synchronize()
+        K.check_errors()
+        if not bool(torch.isfinite(fin).all()):
+            raise SystemExit("non-finite state in the sweep at N=%d" % N)
+        t = e0.elapsed_time(e1) / 1e3 / (5 * steps)
+        dof = s.nelem * s.Np * 5
+        out.append({"N": N, "elements": s.nelem, "dof": dof, "us_per_stage": round(t * 1e6, 2),
+                    "gdof_stage_s": round(dof / t / 1e9, 3),
+                    "bytes_per_element_stage": algorithmic_bytes(3, N),
+                    "hbm_frac": round(algorithmic_bytes(3, N) * s.nelem / t / 1e9 / HBM_PEAK_GBS,
+                                      4)})
+        print("sweep N=%d: %s" % (N, out[-1]), file=sys.stderr, flush=True)
+        del vgeo, sgeo, D, vM, vP, bnd, q, q2, r, fin, s
+        torch.cuda.empty_cache()
+    return {"workload": "3D Euler set2c rising bubble, about 20 M DOF per N, contracted fused "
+                        "stage through canary_lsrk54", "steps": steps, "warmup": warmup,
+            "runs": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,7 +196,7 @@ def main():
     ap.add_argument("--served-requests", type=int, default=64,
                     help="requests of the served-form run (0: skip)")
     ap.add_argument("--served-rate", type=float, default=200.0)
-    ap.add_argument("--form", choices=["batched", "served"], default=None,
+    ap.add_argument("--form", choices=["batched", "served", "sweep"], default=None,
                     help="run only that added form and print its JSON (used by the main run, "
                          "which measures each added form in a child process)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
@@ -157,6 +213,8 @@ def main():
         load_package()
         if args.form == "batched":
             res = batched_form(args.batched_solves, args.batched_steps)
+        elif args.form == "sweep":
+            res = sweep_form()
         else:
             res = served_form(args.served_requests, args.served_rate)
         print(json.dumps(res), flush=True)
