@@ -520,8 +520,8 @@ __device__ __forceinline__ void stage_body(
   //    loads are unconditional (lanes without work read in-bounds element
   //    e0, node 0, and ignore it): a load under a branch makes every later
   //    wait count for the path without it, i.e. wait for everything.
-  FaceItem<Phys> fit[KP];
-  bool fval[KP];
+  FaceItem<Phys> fit[KP > 0 ? KP : 1];  // KP = 0: every face item after the volume work
+  bool fval[KP > 0 ? KP : 1];
 #pragma unroll
   for (int k = 0; k < KP; ++k) {
     const int item = t + k * NT;

@@ -56,9 +56,12 @@ inline int walk_back(const void* qin, const void* qout) {
 template <class Phys, int NQ, int EPW>
 void launch_stage(int64_t nb, const KArgs& a) {
   // measured on 3D N=4 (32^3): one prefetched face item per thread, the
-  // nodal pressure cached in LDS, workgroup sized by the volume nodes
+  // nodal pressure cached in LDS, workgroup sized by the volume nodes.
+  // 3D N=6 prefetches none: 90 VGPRs instead of 112 make it five waves per
+  // SIMD, so three of its 384-lane workgroups fit a CU instead of two
+  // (23^3: 298 -> 266 us per stage; every other N is slower without it)
   constexpr bool WIDE = false;
-  constexpr int PF = 1;
+  constexpr int PF = (Phys::DIM == 3 && NQ == 7) ? 0 : 1;
   constexpr bool PM_LDS = true;
   constexpr int NT = FusedCfg<Phys, NQ, EPW, WIDE>::NT;
   const int64_t chunk = (nb + 7) / 8;
