@@ -83,7 +83,9 @@ __global__ __launch_bounds__((BCfg<Phys, NQ, EPW>::NT)) void batched_kernel(
     return;
   }
   const PhysParams p = params_of(sv.phys);
-  stage_body<Phys, NQ, EPW, C::NT, 1, true>(
+  // 3D N=6 prefetches no face item (fewer VGPRs: a third 384-lane
+  // workgroup per CU), as the single-solve stage does
+  stage_body<Phys, NQ, EPW, C::NT, (Phys::DIM == 3 && NQ == 7 && EPW == 1) ? 0 : 1, true>(
       lds, (Real*)sv.rhs, (const Real*)sv.Q[parity], (Real*)sv.Q[1 - parity],
       (const Real*)sv.vgeo, (const Real*)sv.sgeo, sv.D, it.e0, sv.nreal, sv.nelem_total,
       sv.vmapM, sv.vmapP, sv.elemtobndy, rka, rkb, sv.dt, p);
