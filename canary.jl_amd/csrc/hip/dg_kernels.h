@@ -17,7 +17,8 @@
 // buoyancy source, then the xi sweep n = 1..Nq, the eta sweep, the zeta
 // sweep; then the face contributions in ascending face number; then the
 // update Q + ((rkb*dt)*rhs)*MJI and rhs*rka.  With -ffp-contract=off the
-// only differences from the CPU oracle come from pow() (OCML vs libm).
+// only differences from the CPU oracle come from pow() (the series of
+// phys.h on the device, libm on the host).
 //
 // Face contributions are scattered into a staged copy of the element's rhs
 // in LDS in `DIM` rounds, faces (2d, 2d+1) per round: the two faces of one

@@ -1,7 +1,7 @@
 """GPU parity: the HIP kernels (through the C ABI) against the CPU oracle.
 
 Bar: the exact-order FP64 kernels differ from the oracle only through pow()
-(OCML on the GPU, libm on the host).  So
+(a series on the GPU, exact at 1; libm on the host).  So
   * on states whose pressure needs no pow (R rho theta / p0 == 1 exactly),
     and for the shallow-water equations (no pow), results are bit-identical;
   * on generic states the relative L2 difference per RHS is <= 1e-12 and,
