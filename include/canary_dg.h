@@ -53,10 +53,12 @@ extern "C" {
 #define CANARY_SWE 1     /* (H u, H v, h, b), b read only                    */
 #define CANARY_FP64 0
 #define CANARY_FP32 1
-/* FP64 with a*b+c contracted into fused multiply-adds and the Euler pressure
- * evaluated as exp(kappa log x): fewer FP64 instructions, not the reference's
- * bit pattern (within 1e-12 relative L2 per stage of it).  Accepted by
- * canary_stage_fused only; CANARY_FP64 is the exact reference order. */
+/* FP64 with a*b+c contracted into fused multiply-adds and 1/rho as a
+ * Newton-refined reciprocal: fewer FP64 instructions, not the reference's
+ * bit pattern (within 1e-12 relative L2 per stage of it).  Accepted by the
+ * fused entry points and canary_lsrk54; CANARY_FP64 is the exact reference
+ * order (both evaluate the Euler pressure's pow as exp(kappa log x) by
+ * series, exact at x = 1). */
 #define CANARY_FP64_FMA 2
 
 /* Physical constants passed at run time (reference defaults in
@@ -123,7 +125,8 @@ int canary_updatesolution(int eqn, int dim, int N, int prec, void* rhs, void* Q,
  *   is being overwritten.  Ghost elements (nreal..nelem_total-1) of Qin are
  *   read, never written.  rhs is updated in place.
  *   prec: CANARY_FP64 (bit-identical to the three calls above) or
- *   CANARY_FP64_FMA (contracted; the faster form). */
+ *   CANARY_FP64_FMA (contracted: fewer FP64 instructions; on the 3D N=4
+ *   benchmark both run at the same speed, the stage being bound by HBM). */
 int canary_stage_fused(int eqn, int dim, int N, int prec, void* rhs, const void* Qin,
                        void* Qout, const void* vgeo, const void* sgeo,
                        const double* D, int64_t nreal, int64_t nelem_total,
