@@ -119,6 +119,7 @@ _SIGS = {
     "canary_transferrecvQ": (INT, [INT, INT, INT, P, P, I64, I64, INT, P]),
     "canary_l2energysquared_device": (INT, [INT, INT, INT, INT, P, P, I64, P, P, P]),
     "canary_cfl_device": (INT, [INT, INT, INT, P, P, I64, C.POINTER(Phys), P, P, P]),
+    "canary_cfl_device_noalloc": (INT, [INT, INT, INT, P, P, I64, C.POINTER(Phys), P, P]),
     "canary_courant_device": (INT, [INT, INT, P, P, I64, C.POINTER(Phys), D, P, P, P]),
     "canary_ic_bubble_device": (INT, [INT, INT, I64, P, C.POINTER(Bubble), C.POINTER(Phys), P, P]),
     "canary_ic_vortex2d_device": (INT, [INT, I64, P, C.POINTER(Vortex), D, P, P]),

@@ -17,6 +17,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../../include/canary_dg.h"
 
 namespace canary {
@@ -227,6 +229,34 @@ __global__ __launch_bounds__(RED_THREADS) void finish_kernel(const double* __res
 
 int status() { return hipGetLastError() == hipSuccess ? 0 : CANARY_ERR_LAUNCH; }
 
+// Partials of canary_cfl_device_noalloc: the library's own copy of the
+// `work` buffer, one per device (the loader places it with the code object;
+// it stays until the process ends).  Host code must not pass this name to a
+// kernel -- on the host it is only a stand-in for the symbol; the device
+// address comes from hipGetSymbolAddress, kept per device below.
+static __device__ __align__(256) double scratch[CANARY_REDUCE_WORK];
+static_assert(RED_BLOCKS <= CANARY_REDUCE_WORK, "one partial per reduction workgroup");
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES];
+
+// The current device's address of `scratch`, looked up on the first call
+// made with that device current and kept.
+int scratch_ptr(double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return CANARY_ERR_LAUNCH;
+  if (dev >= MAX_DEVICES) return CANARY_ERR_CAPACITY;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return CANARY_ERR_LAUNCH;
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 int64_t grid_for(int64_t work) {
   int64_t b = (work + 255) / 256;
   return b < 1 ? 1 : (b > 65536 ? 65536 : b);
@@ -350,6 +380,23 @@ extern "C" int canary_cfl_device(int eqn, int dim, int N, const double* vgeo, co
   cfl_kernel<<<RED_BLOCKS, RED_THREADS, 0, st>>>(eqn, dim, Np, vgeo, Q, nelem, ph, work);
   finish_kernel<true><<<1, RED_THREADS, 0, st>>>(work, out);
   return status();
+}
+
+// canary_cfl_device with the partials in the per-device static array: no
+// allocation and no workspace argument.  The reduction writes all
+// RED_BLOCKS partials on every call before the finishing kernel reads them,
+// so nothing carries over between calls and nothing needs clearing; the
+// partial count does not depend on nelem, so no input size exceeds the
+// array.  Calls that overlap on one device share the array and race.
+extern "C" int canary_cfl_device_noalloc(int eqn, int dim, int N, const double* vgeo,
+                                         const double* Q, int64_t nelem,
+                                         const canary_phys* phys, double* out, void* stream) {
+  const bool ok = (eqn == CANARY_EULER && (dim == 2 || dim == 3)) || (eqn == CANARY_SWE && dim == 2);
+  if (!ok || N < 1 || nelem < 0 || !out || (nelem > 0 && (!Q || !vgeo))) return CANARY_ERR_ARG;
+  double* work = nullptr;
+  const int rc = scratch_ptr(&work);
+  if (rc != 0) return rc;
+  return canary_cfl_device(eqn, dim, N, vgeo, Q, nelem, phys, work, out, stream);
 }
 
 extern "C" int canary_courant_device(int dim, int N, const double* vgeo, const double* Q,

@@ -322,3 +322,17 @@ def cfl(dim, N, vgeo, Q, nelem, eqn=EULER, phys=None, stream=None):
                                  _ptr(out), _stream(stream))
     check(rc, "cfl_device")
     return out
+
+
+def cfl_noalloc(dim, N, vgeo, Q, nelem, out, eqn=EULER, phys=None, stream=None):
+    """cfl() into the caller's 0-dimensional CUDA tensor `out` through the
+    entry point that allocates nothing (partials in the library's static
+    device array; calls overlapping on one device race)."""
+    Np = (N + 1) ** dim
+    _need(Q, "Q", Np * _NS[(eqn, dim)] * nelem)
+    _need(vgeo, "vgeo", Np * _NV[dim] * nelem)
+    _need(out, "out", 1)
+    rc = lib().canary_cfl_device_noalloc(eqn, dim, N, _ptr(vgeo), _ptr(Q), nelem, _phys(phys),
+                                         _ptr(out), _stream(stream))
+    check(rc, "cfl_device_noalloc")
+    return out

@@ -463,6 +463,17 @@ int canary_l2energysquared_device(int Np, int nstate, int nvgeo, int iMJ, const 
 int canary_cfl_device(int eqn, int dim, int N, const double* vgeo, const double* Q,
                       int64_t nelem, const canary_phys* phys, double* work, double* out,
                       void* stream);
+/* canary_cfl_device with no `work` argument and no allocation: the
+ * CANARY_REDUCE_WORK partials go to a static device array of the library
+ * (8 KiB per device, held until the process ends; its address is looked up
+ * on the first call made with each device current).  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * hipGraph capture, serving loops that admit requests without hipMalloc).
+ * The partial count is fixed, so every nelem fits.  Calls that overlap on
+ * one device share the array and race: issue them on one stream. */
+int canary_cfl_device_noalloc(int eqn, int dim, int N, const double* vgeo, const double* Q,
+                              int64_t nelem, const canary_phys* phys, double* out,
+                              void* stream);
 /* Courant-number step of the Euler sets: min over nodes of
  * courant * dx / (|u| + c) / N with dx = sqrt(sum_d (1/(2 xi^d_d))^2), the
  * form of the shallow-water example's courantnumber (swe2d.jl:100-124) with
