@@ -384,6 +384,25 @@ def main():
     extra["fma_vs_exact_rel_l2_after_%d_steps" % (args.warmup + args.steps)] = float(
         "%.3e" % (torch.linalg.vector_norm(qf - qx) / torch.linalg.vector_norm(qx)).item())
     del qe, qe2, re, hold_e, qx
+    # the library loop over the prepared arrays (canary_prepare_geometry, once),
+    # both arithmetic forms, from the same initial state
+    prep = K.prepare_geometry(3, s.N, vgeo, sgeo, vM, vP, bnd, s.nelem)
+    K.check_errors()
+    for name, prec in (("prepared_exact_order", FP64), ("prepared_contracted", FP64_FMA)):
+        qa_, qb_, r_ = Q0.clone(), torch.empty_like(Q0), torch.zeros_like(Q0)
+        torch.cuda.synchronize()
+
+        def prun(n, qa_=qa_, qb_=qb_, r_=r_, prec=prec):
+            with torch.cuda.stream(own.stream):
+                return K.lsrk54_prepared(3, s.N, qa_, qb_, r_, prep, D, s.nelem, dt, n,
+                                         phys=s.phys, prec=prec)
+        prun(args.warmup)
+        t_p, _ = timed_own(lambda: prun(args.steps))
+        K.check_errors()
+        extra[name + "_gdof_stage_s"] = round(dof * stages / t_p / 1e9, 3)
+        extra[name + "_us_per_stage"] = round(t_p / stages * 1e6, 2)
+        del qa_, qb_, r_
+    del prep
     # the measured loop with vmapM = NULL (M-side ids from the canonical face
     # mask, the map not read); same bits as with the map
     qc, qc2, rc_ = Q0.clone(), torch.empty_like(Q0), torch.zeros_like(Q0)
@@ -598,6 +617,7 @@ def precision_form(ne, N, steps_csv, timed_steps=10):
         return t
 
     pf = Perturb32(s, Q0, s.Q)
+    pfm = Perturb32(s, Q0, s.Q, mixed=True)
 
     def timed(fn):
         torch.cuda.synchronize()
@@ -609,33 +629,46 @@ def precision_form(ne, N, steps_csv, timed_steps=10):
         return e0.elapsed_time(e1) / 1e3
 
     marks = sorted(int(v) for v in steps_csv.split(","))
-    errors = {}
+    errors, errors_m = {}, {}
     done = 0
-    t64 = t32 = None
-    for m in marks:
-        n = m - done
-        if t64 is None and n >= timed_steps:
-            t64f = time_fma()
-            t64 = timed(lambda: run64(timed_steps)) / (5 * timed_steps)
-            t32 = timed(lambda: pf.run(dt, timed_steps)) / (5 * timed_steps)
-            n -= timed_steps
-        run64(n)
-        pf.run(dt, n)
-        done = m
-        K.check_errors()
-        pf.check()
-        a = pf.full_state().cpu().numpy().reshape(s.Q.shape, order="F")
+    t64 = t32 = t32m = None
+
+    def errs(p):
+        a = p.full_state().cpu().numpy().reshape(s.Q.shape, order="F")
         b = st["q"].cpu().numpy().reshape(s.Q.shape, order="F")
-        errors[str(m)] = {nm: float("%.3e" % (np.linalg.norm(a[:, k] - b[:, k]) / np.linalg.norm(b[:, k])))
-                          for k, nm in ((3, "rho"), (0, "rhou"), (2, "rhow"), (4, "rhotheta"))}
-    if t64 is None:
-        run64(1)
-        pf.run(dt, 1)
+        return {nm: float("%.3e" % (np.linalg.norm(a[:, k] - b[:, k]) / np.linalg.norm(b[:, k])))
+                for k, nm in ((3, "rho"), (0, "rhou"), (2, "rhow"), (4, "rhotheta"))}
+
+    def time_all():
         t64f = time_fma()
         t64 = timed(lambda: run64(timed_steps)) / (5 * timed_steps)
         t32 = timed(lambda: pf.run(dt, timed_steps)) / (5 * timed_steps)
+        t32m = timed(lambda: pfm.run(dt, timed_steps)) / (5 * timed_steps)
+        return t64f, t64, t32, t32m
+
+    for m in marks:
+        n = m - done
+        if t64 is None and n >= timed_steps:
+            t64f, t64, t32, t32m = time_all()
+            n -= timed_steps
+        run64(n)
+        pf.run(dt, n)
+        pfm.run(dt, n)
+        done = m
+        K.check_errors()
+        pf.check()
+        pfm.check()
+        errors[str(m)] = errs(pf)
+        errors_m[str(m)] = errs(pfm)
+    if t64 is None:
+        run64(1)
+        pf.run(dt, 1)
+        pfm.run(dt, 1)
+        t64f, t64, t32, t32m = time_all()
     dof = s.nelem * s.Np * 5
     b32 = Perturb32.bytes_per_element_stage(N)
+    b32m = Perturb32.bytes_per_element_stage(N, mixed=True)
+    b64 = algorithmic_bytes(3, N)
     return {"workload": "3D bubble %d^3 hexes N=%d, FP32 perturbation about an FP64 hydrostatic "
                         "background vs FP64 fused stage" % (ne, N),
             "fp64_ms_per_stage": round(t64 * 1e3, 4), "fp32_ms_per_stage": round(t32 * 1e3, 4),
@@ -646,7 +679,21 @@ def precision_form(ne, N, steps_csv, timed_steps=10):
             "fp64_gdof_stage_s": round(dof / t64 / 1e9, 3),
             "fp32_bytes_per_element_stage": b32,
             "fp32_hbm_frac": round(b32 * s.nelem / t32 / 1e9 / HBM_PEAK_GBS, 4),
-            "rel_l2_vs_fp64": errors, "dt": dt}
+            "rel_l2_vs_fp64": errors,
+            "mixed": {"what": "FP32 state q' and RHS arithmetic; metric terms, MJ, MJI, normals, "
+                              "sMJ and the LSRK register dq in FP64",
+                      "ms_per_stage": round(t32m * 1e3, 4),
+                      "speedup_vs_fp64": round(t64 / t32m, 3),
+                      "gdof_stage_s": round(dof / t32m / 1e9, 3),
+                      "bytes_per_element_stage": b32m,
+                      "byte_ratio_fp64_over_mixed": round(b64 / b32m, 3),
+                      "hbm_frac": round(b32m * s.nelem / t32m / 1e9 / HBM_PEAK_GBS, 4),
+                      "rel_l2_vs_fp64": errors_m},
+            "carries_the_speedup": "compact (FP32 metric terms, normals and LSRK register: "
+                                   "%d B per element-stage against %d B for the FP64 stage); the "
+                                   "mixed variant moves %d B, so bytes cap it at %.2fx"
+                                   % (b32, b64, b32m, b64 / b32m),
+            "dt": dt}
 
 
 def form_child(name, extra_args, timeout=480):
@@ -866,9 +913,13 @@ def cpu_baseline(s, sample_elems):
     host: one LSRK stage (volumerhs!, facerhs!, updatesolution!) over the first
     `sample_elems` elements of the same mesh, single thread and on up to 16
     threads, median of 5 after a warm-up; the host's CPU model and the
-    threads it offers this process are reported beside it."""
+    threads it offers this process are reported beside it.  The pressure's
+    pow is the C library's here, as the reference calls its libm pow (the
+    oracle's correctly rounded pow, its parity default, is about 6x slower
+    per call and is not what the reference spends)."""
     import numpy as np
     from oracle import oracle as O
+    O.set_pow_mode(0)
     n = min(sample_elems, s.nelem)
     Q = np.asfortranarray(s.Q.copy())
     rhs = s.zeros_state()
@@ -886,6 +937,7 @@ def cpu_baseline(s, sample_elems):
     ncpu = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 8)
     stage_time(ncpu)
     tn = sorted(stage_time(ncpu) for _ in range(5))[2]
+    O.set_pow_mode(1)
     model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -901,7 +953,8 @@ def cpu_baseline(s, sample_elems):
             "kind": "port",
             "sample": "LSRK stages (volumerhs!, facerhs!, updatesolution!) over %d of %d "
                       "elements, median of 5 after a warm-up, single thread; C restatement of "
-                      "the Julia CPU kernels (gcc -O2 -ffp-contract=off)" % (n, s.nelem),
+                      "the Julia CPU kernels (gcc -O2 -ffp-contract=off, C library pow as the "
+                      "reference's libm pow)" % (n, s.nelem),
             "all_cores": {"value": round(n * s.Np * 5 / tn / 1e9, 5), "cores": ncpu}}
 
 

@@ -57,7 +57,7 @@ class BatchSolve(C.Structure):
                 ("sgeo", C.c_void_p), ("D", C.c_void_p), ("vmapM", C.c_void_p),
                 ("vmapP", C.c_void_p), ("elemtobndy", C.c_void_p), ("nreal", C.c_int64),
                 ("nelem_total", C.c_int64), ("dt", C.c_double), ("phys", Phys),
-                ("N", C.c_int32), ("reserved", C.c_int32)]
+                ("N", C.c_int32), ("advection", C.c_int32)]
 
 
 class P32Solve(C.Structure):
@@ -92,12 +92,18 @@ _SIGS = {
     "canary_facerhs": (INT, [INT, INT, INT, INT, P, P, P, I64, I64, P, P, P, C.POINTER(Phys), P]),
     "canary_updatesolution": (INT, [INT, INT, INT, INT, P, P, P, I64, D, D, D, INT, P]),
     "canary_stage_fused": (INT, [INT, INT, INT, INT, P, P, P, P, P, P, I64, I64, P, P, P,
-                                 D, D, D, C.POINTER(Phys), P]),
+                                 D, D, D, INT, C.POINTER(Phys), P]),
     "canary_facerhs_update": (INT, [INT, INT, INT, INT, P, P, P, P, P, I64, I64, P, P, P,
-                                    D, D, D, C.POINTER(Phys), P]),
+                                    D, D, D, INT, C.POINTER(Phys), P]),
     "canary_lsrk54": (INT, [INT, INT, INT, INT, INT, I64, D, P, P, P, P, P, P, I64, I64, P, P,
                             P, C.POINTER(Phys), INT, INT, C.POINTER(INT), P]),
     "canary_lsrk54_coefs": (None, [P, P, P]),
+    "canary_pow_register": (INT, [D, P]),
+    "canary_prepare_geometry": (INT, [INT, INT, I64, I64, P, P, P, P, P, P, P, P, P]),
+    "canary_stage_prepared": (INT, [INT, INT, INT, INT, P, P, P, P, P, P, P, I64, I64, D, D, D,
+                                    INT, C.POINTER(Phys), P]),
+    "canary_lsrk54_prepared": (INT, [INT, INT, INT, INT, I64, D, P, P, P, P, P, P, P, I64,
+                                     C.POINTER(Phys), INT, INT, C.POINTER(INT), P]),
     "canary_pack_nodes": (INT, [INT, INT, INT, P, P, P, I64, I64, INT, P]),
     "canary_unpack_nodes": (INT, [INT, INT, INT, P, P, P, I64, I64, INT, P]),
     "canary_batch_run": (INT, [INT, INT, INT, INT, P, P, INT, I64, INT, P]),
@@ -107,6 +113,10 @@ _SIGS = {
     "canary_stage_perturb32": (INT, [INT, INT, P, P, P, P, P, P, P, P, P, I64, D, D, D,
                                      C.POINTER(Phys), P]),
     "canary_perturb32_reconstruct": (INT, [INT, INT, I64, P, P, P, P]),
+    "canary_perturb_mixed_prepare": (INT, [INT, INT, I64, I64, P, P, P, P, P, P, P,
+                                           C.POINTER(Phys), P, P, P, P, P, P, P]),
+    "canary_stage_perturb_mixed": (INT, [INT, INT, P, P, P, P, P, P, P, P, P, I64, D, D, D,
+                                         C.POINTER(Phys), P]),
     "canary_p32_batch_variant": (INT, [INT, INT, INT, C.POINTER(INT), C.POINTER(INT)]),
     "canary_p32_batch_stage": (INT, [INT, INT, INT, P, P, I64, INT, D, D, P]),
     "canary_batch_variant": (INT, [INT, INT, INT, INT, INT, C.POINTER(INT), C.POINTER(INT)]),
@@ -144,6 +154,8 @@ _SIGS = {
     "canary_computemetric": (INT, [INT, INT, I64, P, P, P, P]),
     "canary_computegeometry": (INT, [INT, INT, I64, P, P, P, P]),
     "canary_hash_jitter": (D, [C.c_uint64, D, D, D]),
+    "canary_pow_cr": (D, [D, D]),
+    "canary_pow_cr_table": (D, [D, D]),
     "canary_ic_bubble": (INT, [INT, INT, I64, P, C.POINTER(Bubble), C.POINTER(Phys), P]),
     "canary_ic_vortex2d": (INT, [INT, I64, P, C.POINTER(Vortex), D, P]),
     "canary_ic_swe_bumps": (INT, [INT, I64, P, D, INT, P, D, D, P]),

@@ -27,7 +27,7 @@ from . import _lib
 from ._lib import EULER, SWE, FP64, FP32, FP64_FMA, check, lib
 from .perturb import Perturb32
 from .problems import device_arrays, rk_coefs
-from .streams import OwnedStream
+from .streams import OwnedStream, reap
 
 _NS = {(EULER, 3): 5, (EULER, 2): 4, (SWE, 2): 4}
 _NSR = {(EULER, 3): 5, (EULER, 2): 4, (SWE, 2): 3}
@@ -190,6 +190,7 @@ class _BatchBase:
             # use after free
             torch.cuda.synchronize(self.device)
         self._graphs = {}
+        reap()  # streams of collected runtimes, destroyed here (no capture in flight)
         self.dirty = False
         self._table = self._solve_table()
         parts, self.groups, off = [], [], 0
@@ -386,13 +387,16 @@ class Batch(_BatchBase):
     def _bytes_per_element_stage(self, v):
         return bytes_per_element_stage(self.eqn, self.dim, v.N, self._word())
 
-    def add(self, s, dt, Q=None):
+    def add(self, s, dt, Q=None, advection=False):
         """Upload one solve (a problems.Setup; Q defaults to s.Q).  Returns its
         id.  May be called between steps of a running batch: the state goes
         into the current parity buffer and the work table is rebuilt by the
-        next finalize()."""
+        next finalize().  advection (shallow water): the solve keeps its
+        velocity through each update, as updatesolution's flag does."""
         if s.eqn != self.eqn or s.dim != self.dim:
             raise ValueError("solve (eqn %d, dim %d) does not match the batch" % (s.eqn, s.dim))
+        if advection and s.eqn != _lib.SWE:
+            raise ValueError("advection is a shallow-water mode")
         if not self._variants(s.N):
             raise _lib.CanaryError(_lib.ERR_UNSUPPORTED, "batched form at N=%d" % s.N)
         if Q is None:
@@ -411,6 +415,9 @@ class Batch(_BatchBase):
         v.sgeo = arr["sgeo"].to(rd)
         v.D, v.vmapM, v.vmapP, v.bndy = arr["D"], arr["vmapM"], arr["vmapP"], arr["bndy"]
         v.phys = s.phys
+        v.advection = bool(advection)
+        if s.eqn == _lib.EULER:  # the pressure's pow table, before any capture
+            lib().canary_pow_register(s.phys.c_p / s.phys.c_v, None)
         return self._register(v)
 
     def _fill_row(self, t, v):
@@ -423,6 +430,7 @@ class Batch(_BatchBase):
         t.dt = v.dt
         t.phys = v.phys
         t.N = v.N
+        t.advection = 1 if v.advection else 0
 
     def footprint(self, s):
         """Device bytes a solve of this setup holds in the batch."""

@@ -6,7 +6,9 @@ Every HIP translation unit is compiled with
 ``-Xclang -target-feature -Xclang -packed-fp32-ops`` (no packed FP32 ops are
 emitted).  The exact-order FP64 kernels and the host metric code are built
 with ``-ffp-contract=off`` so that no multiply-add is fused and the
-evaluation order of the reference CPU kernels is kept.
+evaluation order of the reference CPU kernels is kept.  The host objects
+use the x86-64 FMA instructions (-mfma) only where the code calls fma()
+(the correctly rounded pow); no expression is contracted.
 """
 import argparse
 import concurrent.futures as cf
@@ -24,7 +26,7 @@ LIB = os.path.join(HERE, "libcanary_dg.so")
 HOST_SOURCES = ["host/quadrature.cpp", "host/mesh.cpp", "host/metric.cpp", "host/problems.cpp"]
 HIP_SOURCES = ["hip/runtime.hip", "hip/euler3d.hip", "hip/euler2d.hip", "hip/swe2d.hip",
                "hip/batched.hip", "hip/geometry.hip", "hip/aux.hip", "hip/prep.hip",
-               "hip/driver.hip"]
+               "hip/prepared.hip", "hip/driver.hip"]
 # kernels allowed to contract a*b+c into FMA
 HIP_FAST_SOURCES = ["hip/perturb32.hip", "hip/euler3d_fma.hip", "hip/euler2d_fma.hip",
                     "hip/swe2d_fma.hip", "hip/batched_fma.hip"]
@@ -44,7 +46,7 @@ def _cmds():
     cmds = []
     for s in HOST_SOURCES:
         o = os.path.join(OBJ, s.replace("/", "_") + ".o")
-        cmds.append((o, ["g++", "-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-fopenmp",
+        cmds.append((o, ["g++", "-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-mfma", "-fopenmp",
                          "-fno-fast-math", "-Wall", "-Wno-unused-function", *inc,
                          "-c", os.path.join(CSRC, s), "-o", o]))
     hip = _hipcc()

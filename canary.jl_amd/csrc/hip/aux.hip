@@ -20,6 +20,7 @@
 #include <mutex>
 
 #include "../../../include/canary_dg.h"
+#include "crpow.h"
 
 namespace canary {
 namespace aux {
@@ -131,7 +132,7 @@ __device__ __forceinline__ double local_dt(int eqn, int dim, int64_t Np, const d
     const double* vg = vgeo + (size_t)Np * 14 * e;
     const double rho = q[n + Np * 3], U = q[n], V = q[n + Np], W = q[n + 2 * Np];
     const double E = q[n + Np * 4];
-    const double P = ph.p0 * pow(ph.R_gas * E / ph.p0, ph.c_p / ph.c_v);
+    const double P = ph.p0 * canary::crpow::pow_cr(ph.R_gas * E / ph.p0, ph.c_p / ph.c_v);
     const double c = rho * sqrt(ph.gamma * P / rho);
     const double a = fabs(U * vg[n] + V * vg[n + 3 * Np] + W * vg[n + 6 * Np]) + c;
     const double b = fabs(U * vg[n + Np] + V * vg[n + 4 * Np] + W * vg[n + 7 * Np]) + c;
@@ -144,7 +145,7 @@ __device__ __forceinline__ double local_dt(int eqn, int dim, int64_t Np, const d
     const double* q = Q + (size_t)Np * 4 * e;
     const double* vg = vgeo + (size_t)Np * 8 * e;
     const double rho = q[n + Np * 2], U = q[n], V = q[n + Np], E = q[n + Np * 3];
-    const double P = ph.p0 * pow(ph.R_gas * E / ph.p0, ph.c_p / ph.c_v);
+    const double P = ph.p0 * canary::crpow::pow_cr(ph.R_gas * E / ph.p0, ph.c_p / ph.c_v);
     const double c = rho * sqrt(ph.gamma * P / rho);
     const double a = fabs(U * vg[n] + V * vg[n + 2 * Np]) + c;
     const double b = fabs(U * vg[n + Np] + V * vg[n + 3 * Np]) + c;
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(RED_THREADS) void courant_kernel(int dim, int N, in
       const double h = 1.0 / (2 * vg[n + Np * (c * (dim + 1))]);  // xi_x, eta_y(, zeta_z)
       dx2 = dx2 + h * h;
     }
-    const double P = ph.p0 * pow(ph.R_gas * rt / ph.p0, ph.c_p / ph.c_v);
+    const double P = ph.p0 * canary::crpow::pow_cr(ph.R_gas * rt / ph.p0, ph.c_p / ph.c_v);
     const double cs = sqrt(ph.gamma * P / rho);
     const double v = courant * sqrt(dx2) / (sqrt(vel2) + cs) / N;
     acc = v < acc ? v : acc;

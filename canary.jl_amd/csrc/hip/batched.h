@@ -14,6 +14,7 @@
 
 #include "../../../include/canary_dg.h"
 #include "dg_kernels.h"
+#include "pow_registry.h"
 
 namespace canary {
 namespace batch {
@@ -49,6 +50,7 @@ __device__ __forceinline__ PhysParams params_of(const canary_phys& c) {
   p.kappa = c.c_p / c.c_v;
   p.swe_g = c.swe_gravity;
   p.dnl = c.delta_nl;
+  p.ktab = nullptr;
   return p;
 }
 
@@ -66,7 +68,8 @@ struct BCfg {
 template <class Phys, int NQ, int EPW>
 __global__ __launch_bounds__((BCfg<Phys, NQ, EPW>::NT)) void batched_kernel(
     const canary_batch_solve* __restrict__ solves, const canary_batch_item* __restrict__ items,
-    int64_t nitems, int64_t chunk, int parity, double rka, double rkb) {
+    int64_t nitems, int64_t chunk, int parity, double rka, double rkb,
+    const crpow::KappaTable* __restrict__ ktabs, int nktab) {
   using C = FusedCfg<Phys, NQ, EPW, false>;
   using Real = typename Phys::Real;
   __shared__ __attribute__((aligned(16))) char lds[C::LDS];
@@ -82,13 +85,19 @@ __global__ __launch_bounds__((BCfg<Phys, NQ, EPW>::NT)) void batched_kernel(
     if (threadIdx.x == 0) record_error(1 /*CANARY_ERR_ARG*/, it.solve, it.e0 + 1, sv.nreal);
     return;
   }
-  const PhysParams p = params_of(sv.phys);
+  PhysParams p = params_of(sv.phys);
+  if (Phys::HAS_P)  // the registered pow table of this solve's kappa, if any
+    for (int i = 0; i < nktab; ++i)
+      if (ktabs[i].kappa == p.kappa) {
+        p.ktab = ktabs + i;
+        break;
+      }
   // 3D N=6 prefetches no face item (fewer VGPRs: a third 384-lane
   // workgroup per CU), as the single-solve stage does
   stage_body<Phys, NQ, EPW, C::NT, (Phys::DIM == 3 && NQ == 7 && EPW == 1) ? 0 : 1, true>(
       lds, (Real*)sv.rhs, (const Real*)sv.Q[parity], (Real*)sv.Q[1 - parity],
       (const Real*)sv.vgeo, (const Real*)sv.sgeo, sv.D, it.e0, sv.nreal, sv.nelem_total,
-      sv.vmapM, sv.vmapP, sv.elemtobndy, rka, rkb, sv.dt, p);
+      sv.vmapM, sv.vmapP, sv.elemtobndy, rka, rkb, sv.dt, p, sv.advection);
 }
 
 template <class Phys, int NQ, int WHICH>
@@ -107,8 +116,10 @@ int run_which(int* epw, int* threads, const canary_batch_solve* solves,
     }
     const int64_t chunk = (nitems + 7) / 8;
     if (chunk > ((int64_t)1 << 27)) return CANARY_ERR_ARG;
+    int nktab = 0;
+    const crpow::KappaTable* ktabs = Phys::HAS_P ? kappa_tables(&nktab) : nullptr;
     batched_kernel<Phys, NQ, E><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, st>>>(
-        solves, items, nitems, chunk, parity, rka, rkb);
+        solves, items, nitems, chunk, parity, rka, rkb, ktabs, nktab);
     return hipGetLastError() == hipSuccess ? 0 : CANARY_ERR_LAUNCH;
   }
 }

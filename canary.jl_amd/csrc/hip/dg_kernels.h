@@ -16,9 +16,9 @@
 // the reference CPU kernels (euler3d_set2c.jl:144-358): rhs, then the
 // buoyancy source, then the xi sweep n = 1..Nq, the eta sweep, the zeta
 // sweep; then the face contributions in ascending face number; then the
-// update Q + ((rkb*dt)*rhs)*MJI and rhs*rka.  With -ffp-contract=off the
-// only differences from the CPU oracle come from pow() (the series of
-// phys.h on the device, libm on the host).
+// update Q + ((rkb*dt)*rhs)*MJI and rhs*rka.  With -ffp-contract=off and
+// the correctly rounded pow of crpow.h (the oracle's pow is correctly
+// rounded too) the exact-order kernels give the oracle's bits.
 //
 // Face contributions are scattered into a staged copy of the element's rhs
 // in LDS in `DIM` rounds, faces (2d, 2d+1) per round: the two faces of one
@@ -273,6 +273,55 @@ __device__ __forceinline__ void face_meta(FaceItem<Phys>& it,
   face_resolve<Phys, NQ>(it, e, fi, ntotal);
 }
 
+// ---------------------------------------------------- prepared geometry
+// The compact arrays canary_prepare_geometry builds once from the reference
+// ones (vmapM the canonical face mask, every face in the orientation
+// `mappings` supports, MJI = 1 / MJ, elemtobndy 0 or 1 -- all checked
+// there):
+//   pvgeo (Np, DIM*DIM + 1, nelem): metric terms and MJ (no MJI, no x, y, z)
+//   psgeo (DIM + 1, Nfp, NFACE, nelem): normal and sMJ (no vMJI)
+//   pface int32 (NFACE, nelem): -1 for a wall face (bc 1), else
+//         NFACE * eP + fP, the neighbour element and its face (0-based):
+//         vmapP[fn, f, e] = Np * eP + fmask[fn, fP] + 1 (src/mesh.jl:994-995)
+template <class Phys>
+struct Prepared {
+  static constexpr int DIM = Phys::DIM;
+  static constexpr int NV = DIM * DIM + 1;  // pvgeo fields
+  static constexpr int NSG = DIM + 1;       // psgeo fields
+};
+
+// face_load over the prepared arrays: the face code goes to idP and is
+// turned into node ids by face_decode once it has arrived.
+template <class Phys, int NQ>
+__device__ __forceinline__ void face_load_prep(FaceItem<Phys>& it,
+                                               const typename Phys::Real* __restrict__ psgeo,
+                                               const int32_t* __restrict__ pface, int64_t e,
+                                               int fi) {
+  using S = Shape<Phys, NQ>;
+  constexpr int DIM = Phys::DIM, NSG = Prepared<Phys>::NSG;
+  const int f = fi / S::Nfp;
+  const size_t fo = (size_t)fi + (size_t)S::NFN * (size_t)e;
+  const typename Phys::Real* sg = psgeo + (size_t)NSG * fo;
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) it.nrm[c] = sg[c];
+  it.sMJ = sg[DIM];
+  it.idP = pface[f + S::NFACE * e];
+}
+
+template <class Phys, int NQ>
+__device__ __forceinline__ void face_decode(FaceItem<Phys>& it, int64_t e, int fi) {
+  using S = Shape<Phys, NQ>;
+  constexpr int DIM = Phys::DIM;
+  const int f = fi / S::Nfp, fn = fi - f * S::Nfp;
+  const int64_t code = it.idP;
+  const bool wall = code < 0;
+  const int64_t eP = wall ? e : code / S::NFACE;
+  const int fP = wall ? f : (int)(code - eP * S::NFACE);
+  it.idM = (int64_t)S::Np * e + canon_vid<DIM, NQ>(f, fn) + 1;
+  it.idP = (int64_t)S::Np * eP + canon_vid<DIM, NQ>(fP, fn) + 1;
+  it.bcode = wall ? 1 : 0;
+}
+
 template <class Phys, int NQ>
 __device__ __forceinline__ void face_gather(FaceItem<Phys>& it,
                                             const typename Phys::Real* __restrict__ Q) {
@@ -484,20 +533,25 @@ struct FusedCfg {
 // MJI alone and D is not read.  UPDATE = false (with VOLUME = false) is
 // facerhs! alone: rhs -= face terms, Q untouched, neither vgeo nor Qout read.
 template <class Phys, int NQ, int EPW, int NT, int PF, bool PM_LDS, bool VOLUME = true,
-          bool UPDATE = true>
+          bool UPDATE = true, bool PREP = false>
 __device__ __forceinline__ void stage_body(
     char* __restrict__ lds, typename Phys::Real* __restrict__ rhs,
     const typename Phys::Real* __restrict__ Qin, typename Phys::Real* __restrict__ Qout,
     const typename Phys::Real* __restrict__ vgeo, const typename Phys::Real* __restrict__ sgeo,
     const double* __restrict__ D, int64_t e0, int64_t nreal, int64_t ntotal,
     const int64_t* __restrict__ vmapM, const int64_t* __restrict__ vmapP,
-    const int64_t* __restrict__ bndy, double rka, double rkb, double dt, const PhysParams& p) {
+    const int64_t* __restrict__ bndy, double rka, double rkb, double dt, const PhysParams& p,
+    int advection = 0) {
   using S = Shape<Phys, NQ>;
   using C = FusedCfg<Phys, NQ, EPW, false>;
   using Real = typename Phys::Real;
-  constexpr int Np = S::Np, DIM = S::DIM, NS = Phys::NS, NSR = Phys::NSR, NV = Phys::NVGEO;
+  constexpr int Np = S::Np, DIM = S::DIM, NS = Phys::NS, NSR = Phys::NSR;
+  constexpr int NV = PREP ? Prepared<Phys>::NV : Phys::NVGEO;
   constexpr int NFN = S::NFN, Nfp = S::Nfp;
   constexpr int KF = (EPW * NFN + NT - 1) / NT;
+  static_assert(!PREP || (VOLUME && UPDATE), "prepared arrays: the whole stage only");
+  // prepared arrays: psgeo in `sgeo`, the face codes in `vmapP`'s place
+  const int32_t* __restrict__ pface = reinterpret_cast<const int32_t*>(vmapP);
   constexpr int KP = PF < KF ? PF : KF;
   constexpr bool SP = Phys::HAS_P && PM_LDS;
   constexpr int BUF = C::BUF;
@@ -528,8 +582,12 @@ __device__ __forceinline__ void stage_body(
     const int item = t + k * NT;
     const int sl = item / NFN;
     fval[k] = item < EPW * NFN && e0 + sl < nreal;
-    face_load<Phys, NQ>(fit[k], sgeo, vmapM, vmapP, bndy, fval[k] ? e0 + sl : e0,
-                        fval[k] ? item - sl * NFN : 0);
+    if constexpr (PREP)
+      face_load_prep<Phys, NQ>(fit[k], sgeo, pface, fval[k] ? e0 + sl : e0,
+                               fval[k] ? item - sl * NFN : 0);
+    else
+      face_load<Phys, NQ>(fit[k], sgeo, vmapM, vmapP, bndy, fval[k] ? e0 + sl : e0,
+                          fval[k] ? item - sl * NFN : 0);
   }
   double dval = 0.0;
   if constexpr (VOLUME) dval = D[t < NQ * NQ ? t : 0];
@@ -550,7 +608,7 @@ __device__ __forceinline__ void stage_body(
     MJ = vg[nl + Np * Phys::iMJ];
   }
   Real MJI = 0;
-  if constexpr (UPDATE) MJI = vg[nl + Np * Phys::iMJI];
+  if constexpr (UPDATE && !PREP) MJI = vg[nl + Np * Phys::iMJI];
   const Real* re = rhs + (size_t)NS * Np * (size_t)el;
 #pragma unroll
   for (int s = 0; s < NSR; ++s) r[s] = re[nl + Np * s];
@@ -562,10 +620,13 @@ __device__ __forceinline__ void stage_body(
   for (int k = 0; k < KP; ++k) {
     const int item = t + k * NT;
     const int sl = item / NFN;
+    if constexpr (PREP)
+      face_decode<Phys, NQ>(fit[k], fval[k] ? e0 + sl : e0, fval[k] ? item - sl * NFN : 0);
     face_resolve<Phys, NQ>(fit[k], fval[k] ? e0 + sl : e0, fval[k] ? item - sl * NFN : 0,
                            ntotal);
     face_gather_any<Phys, NQ>(fit[k], Qin, fval[k], e0);
   }
+
   if (VOLUME && t < NQ * NQ) sD[t] = (Real)dval;
   Real Fv[DIM][NSR];
   if (valid) {
@@ -623,8 +684,18 @@ __device__ __forceinline__ void stage_body(
     auto qm_at = [&](int s, int v) { return sQ[sl * NS * Np + s * Np + v]; };
     auto pm_at = [&](int v, const Real* qm) { return pm_of(sl, v, qm); };
     Real c[NSR];
-    const int vid = face_contribution<Phys, NQ>(Qin, sgeo, vmapM, vmapP, bndy, e0 + sl, fi,
-                                                ntotal, p, qm_at, pm_at, c);
+    int vid;
+    if constexpr (PREP) {
+      FaceItem<Phys> it;
+      face_load_prep<Phys, NQ>(it, sgeo, pface, e0 + sl, fi);
+      face_decode<Phys, NQ>(it, e0 + sl, fi);
+      face_resolve<Phys, NQ>(it, e0 + sl, fi, ntotal);
+      face_gather<Phys, NQ>(it, Qin);
+      vid = face_finish<Phys>(it, p, qm_at, pm_at, c);
+    } else {
+      vid = face_contribution<Phys, NQ>(Qin, sgeo, vmapM, vmapP, bndy, e0 + sl, fi, ntotal, p,
+                                        qm_at, pm_at, c);
+    }
     sV[sl * NFN + fi] = vid < 0 ? NOVID : (VidT)vid;
 #pragma unroll
     for (int s = 0; s < NSR; ++s) sB[sl * BUF + fi * NSR + s] = c[s];
@@ -644,17 +715,33 @@ __device__ __forceinline__ void stage_body(
     lds_barrier();
   }
   // 5. update: Qout = Q + ((rkb*dt)*rhs)*MJI, rhs *= rka, in LDS, by the
-  //    thread that holds the node's MJI.
+  //    thread that holds the node's MJI.  Shallow water with `advection`
+  //    keeps the velocity: u = Hu / (h + b) before the update, Hu = (h + b) u
+  //    after it (shallow_water.jl:274-297).
   if (UPDATE && valid) {
+    // prepared arrays hold no MJI: the reference's is 1 ./ MJ (IEEE division)
+    if constexpr (PREP) MJI = Real(1) / MJ;
     const Real bdt = (Real)(rkb * dt);
     const Real ka = (Real)rka;
+    Real* qn = sQ + slot * NS * Np + n;
+    Real uu = 0, vv = 0;
+    if (Phys::HAS_ADVECTION && advection) {
+      const Real Ht = qn[Np * 2] + qn[Np * 3];
+      uu = qn[0] / Ht;
+      vv = qn[Np] / Ht;
+    }
 #pragma unroll
     for (int s = 0; s < NSR; ++s) {
       Real& rr = sB[slot * BUF + NFN * NSR + s * Np + n];
-      Real& qq = sQ[slot * NS * Np + s * Np + n];
+      Real& qq = qn[s * Np];
       const Real rs = rr;
       qq = qq + bdt * rs * MJI;
       rr = rs * ka;
+    }
+    if (Phys::HAS_ADVECTION && advection) {
+      const Real Ht = qn[Np * 2] + qn[Np * 3];
+      qn[0] = Ht * uu;
+      qn[Np] = Ht * vv;
     }
   }
   lds_barrier();
@@ -686,14 +773,34 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, WIDE>::NT)) void stage_ker
     const double* __restrict__ D, int64_t nreal, int64_t ntotal,
     const int64_t* __restrict__ vmapM, const int64_t* __restrict__ vmapP,
     const int64_t* __restrict__ bndy, double rka, double rkb, double dt, int64_t chunk,
-    int reverse, PhysParams p) {
+    int reverse, PhysParams p, int advection) {
   using C = FusedCfg<Phys, NQ, EPW, WIDE>;
   __shared__ __attribute__((aligned(16))) char lds[C::LDS];
   const int64_t blk = stage_block(blockIdx.x, chunk, reverse);
   const int64_t e0 = blk * EPW;
   if (e0 >= nreal) return;  // whole workgroup leaves together
   stage_body<Phys, NQ, EPW, C::NT, PF, PM_LDS>(lds, rhs, Qin, Qout, vgeo, sgeo, D, e0, nreal,
-                                               ntotal, vmapM, vmapP, bndy, rka, rkb, dt, p);
+                                               ntotal, vmapM, vmapP, bndy, rka, rkb, dt, p,
+                                               advection);
+}
+
+// The fused stage over the prepared arrays (see Prepared): the same
+// arithmetic and order, fewer bytes.
+template <class Phys, int NQ, int EPW, int PF>
+__global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, false>::NT)) void stage_prep_kernel(
+    double* __restrict__ rhs, const double* __restrict__ Qin, double* __restrict__ Qout,
+    const double* __restrict__ pvgeo, const double* __restrict__ psgeo,
+    const int32_t* __restrict__ pface, const double* __restrict__ D, int64_t nreal,
+    int64_t ntotal, double rka, double rkb, double dt, int64_t chunk, int reverse, PhysParams p,
+    int advection) {
+  using C = FusedCfg<Phys, NQ, EPW, false>;
+  __shared__ __attribute__((aligned(16))) char lds[C::LDS];
+  const int64_t blk = stage_block(blockIdx.x, chunk, reverse);
+  const int64_t e0 = blk * EPW;
+  if (e0 >= nreal) return;  // whole workgroup leaves together
+  stage_body<Phys, NQ, EPW, C::NT, PF, true, true, true, true>(
+      lds, rhs, Qin, Qout, pvgeo, psgeo, D, e0, nreal, ntotal, nullptr,
+      reinterpret_cast<const int64_t*>(pface), nullptr, rka, rkb, dt, p, advection);
 }
 
 // facerhs! alone (rhs -= sMJ * Rusanov flux of Q): the fused stage body with
@@ -724,7 +831,7 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, false>::NT)) void face_upd
     const double* __restrict__ vgeo, const double* __restrict__ sgeo, int64_t nreal,
     int64_t ntotal, const int64_t* __restrict__ vmapM, const int64_t* __restrict__ vmapP,
     const int64_t* __restrict__ bndy, double rka, double rkb, double dt, int64_t chunk,
-    int reverse, PhysParams p) {
+    int reverse, PhysParams p, int advection) {
   using C = FusedCfg<Phys, NQ, EPW, false>;
   __shared__ __attribute__((aligned(16))) char lds[C::LDS];
   const int64_t blk = stage_block(blockIdx.x, chunk, reverse);
@@ -732,7 +839,7 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, false>::NT)) void face_upd
   if (e0 >= nreal) return;  // whole workgroup leaves together
   stage_body<Phys, NQ, EPW, C::NT, 1, true, false>(lds, rhs, Qin, Qout, vgeo, sgeo, nullptr, e0,
                                                    nreal, ntotal, vmapM, vmapP, bndy, rka, rkb,
-                                                   dt, p);
+                                                   dt, p, advection);
 }
 
 }  // namespace canary

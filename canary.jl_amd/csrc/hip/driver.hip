@@ -32,8 +32,11 @@ void lsrk54_tables(double* a, double* b, double* c) {
   c[4] = (double)2802321613138 / (double)2924317926251;
 }
 
+constexpr int PATH_PREPARED = 3;  // canary_stage_prepared over the prepared arrays
+
 struct Run {
   int eqn, dim, N, prec, path, advection;
+  const int32_t* pface;
   double dt;
   void* Q[2];
   void* rhs;
@@ -65,10 +68,15 @@ int step(Run& r, int& cur) {
       if (!rc)
         rc = canary_updatesolution(r.eqn, r.dim, r.N, r.prec, r.rhs, q, r.vgeo, r.nreal, rka, rkb,
                                    r.dt, r.advection, r.stream);
+    } else if (r.path == PATH_PREPARED) {
+      rc = canary_stage_prepared(r.eqn, r.dim, r.N, r.prec, r.rhs, r.Q[cur], r.Q[1 - cur], r.vgeo,
+                                 r.sgeo, r.pface, r.D, r.nreal, r.ntotal, rka, rkb, r.dt,
+                                 r.advection, r.phys, r.stream);
+      cur = 1 - cur;
     } else if (r.path == CANARY_PATH_FUSED) {
       rc = canary_stage_fused(r.eqn, r.dim, r.N, r.prec, r.rhs, r.Q[cur], r.Q[1 - cur], r.vgeo,
                               r.sgeo, r.D, r.nreal, r.ntotal, r.vmapM, r.vmapP, r.bndy, rka, rkb,
-                              r.dt, r.phys, r.stream);
+                              r.dt, r.advection, r.phys, r.stream);
       cur = 1 - cur;
     } else {
       rc = canary_volumerhs(r.eqn, r.dim, r.N, r.prec == CANARY_FP64_FMA ? CANARY_FP64 : r.prec,
@@ -76,12 +84,27 @@ int step(Run& r, int& cur) {
       if (!rc)
         rc = canary_facerhs_update(r.eqn, r.dim, r.N, r.prec, r.rhs, r.Q[cur], r.Q[1 - cur],
                                    r.vgeo, r.sgeo, r.nreal, r.ntotal, r.vmapM, r.vmapP, r.bndy,
-                                   rka, rkb, r.dt, r.phys, r.stream);
+                                   rka, rkb, r.dt, r.advection, r.phys, r.stream);
       cur = 1 - cur;
     }
     if (rc) return rc;
   }
   return 0;
+}
+
+int run_loop(Run& r, int64_t nsteps, int graph, int* result_in_alt);
+
+// The Euler pressure's pow table, made before any graph capture begins (a
+// capture in progress cannot make one, and its kernels would then take the
+// general path).
+void register_kappa(int eqn, const canary_phys* phys, void* stream) {
+  if (eqn != CANARY_EULER) return;
+  canary_phys d;
+  if (!phys) {
+    canary_phys_default(&d);
+    phys = &d;
+  }
+  canary_pow_register(phys->c_p / phys->c_v, stream);
 }
 
 }  // namespace
@@ -151,11 +174,16 @@ extern "C" int canary_lsrk54(int eqn, int dim, int N, int prec, int path, int64_
   if (path != CANARY_PATH_KERNELS && path != CANARY_PATH_FUSED && path != CANARY_PATH_FACE_UPDATE)
     return CANARY_ERR_ARG;
   if (path == CANARY_PATH_KERNELS && prec != CANARY_FP64) return CANARY_ERR_UNSUPPORTED;
-  if (advection && (path != CANARY_PATH_KERNELS || eqn != CANARY_SWE)) return CANARY_ERR_UNSUPPORTED;
+  if (advection && eqn != CANARY_SWE) return CANARY_ERR_ARG;
   if (path != CANARY_PATH_KERNELS && (!Qalt || Qalt == Q)) return CANARY_ERR_ARG;
+  // the loop exchanges no ghost elements: they would be frozen in Q and, on
+  // the ping-pong paths, never written in Qalt
+  if (nelem_total != nreal) return CANARY_ERR_ARG;
+  register_kappa(eqn, phys, stream);
   *result_in_alt = 0;
   if (nsteps == 0 || nreal == 0) return 0;
   Run r;
+  r.pface = nullptr;
   r.eqn = eqn;
   r.dim = dim;
   r.N = N;
@@ -176,6 +204,50 @@ extern "C" int canary_lsrk54(int eqn, int dim, int N, int prec, int path, int64_
   r.bndy = elemtobndy;
   r.phys = phys;
   r.stream = stream;
+  return run_loop(r, nsteps, graph, result_in_alt);
+}
+
+extern "C" int canary_lsrk54_prepared(int eqn, int dim, int N, int prec, int64_t nsteps,
+                                      double dt, void* Q, void* Qalt, void* rhs,
+                                      const void* pvgeo, const void* psgeo,
+                                      const int32_t* pface, const double* D, int64_t nreal,
+                                      const canary_phys* phys, int advection, int graph,
+                                      int* result_in_alt, void* stream) {
+  if (!result_in_alt || nsteps < 0) return CANARY_ERR_ARG;
+  if (advection && eqn != CANARY_SWE) return CANARY_ERR_ARG;
+  if (!Qalt || Qalt == Q) return CANARY_ERR_ARG;
+  register_kappa(eqn, phys, stream);
+  *result_in_alt = 0;
+  if (nsteps == 0 || nreal == 0) return 0;
+  Run r;
+  r.eqn = eqn;
+  r.dim = dim;
+  r.N = N;
+  r.prec = prec;
+  r.path = PATH_PREPARED;
+  r.advection = advection;
+  r.pface = pface;
+  r.dt = dt;
+  r.Q[0] = Q;
+  r.Q[1] = Qalt;
+  r.rhs = rhs;
+  r.vgeo = pvgeo;
+  r.sgeo = psgeo;
+  r.D = D;
+  r.nreal = nreal;
+  r.ntotal = nreal;
+  r.vmapM = nullptr;
+  r.vmapP = nullptr;
+  r.bndy = nullptr;
+  r.phys = phys;
+  r.stream = stream;
+  return run_loop(r, nsteps, graph, result_in_alt);
+}
+
+namespace {
+
+int run_loop(Run& r, int64_t nsteps, int graph, int* result_in_alt) {
+  void* stream = r.stream;
   lsrk54_tables(r.A, r.B, r.C);
   int cur = 0;
   // graph: two steps (an even number of stages, so the ping-pong buffers are
@@ -220,3 +292,5 @@ extern "C" int canary_lsrk54(int eqn, int dim, int N, int prec, int path, int64_
   *result_in_alt = cur;
   return 0;
 }
+
+}  // namespace

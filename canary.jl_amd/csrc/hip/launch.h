@@ -10,7 +10,8 @@
 
 namespace canary {
 
-enum Op { OP_VOLUME = 0, OP_FACE = 1, OP_UPDATE = 2, OP_STAGE = 3, OP_FACE_UPDATE = 4 };
+enum Op { OP_VOLUME = 0, OP_FACE = 1, OP_UPDATE = 2, OP_STAGE = 3, OP_FACE_UPDATE = 4,
+          OP_STAGE_PREP = 5 };
 
 struct KArgs {
   void* rhs;
@@ -27,6 +28,7 @@ struct KArgs {
   int advection;
   PhysParams phys;
   hipStream_t stream;
+  const int32_t* pface;  // prepared face codes (OP_STAGE_PREP; vgeo, sgeo prepared too)
 };
 
 constexpr int kMaxN = 7;  // N = 1..7 compiled in
@@ -68,7 +70,18 @@ void launch_stage(int64_t nb, const KArgs& a) {
   stage_kernel<Phys, NQ, EPW, WIDE, PF, PM_LDS><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
       (double*)a.rhs, (const double*)a.Q, (double*)a.Qout, (const double*)a.vgeo,
       (const double*)a.sgeo, a.D, a.nreal, a.ntotal, a.vmapM, a.vmapP, a.bndy, a.rka, a.rkb,
-      a.dt, chunk, walk_back(a.Q, a.Qout), a.phys);
+      a.dt, chunk, walk_back(a.Q, a.Qout), a.phys, a.advection);
+}
+
+template <class Phys, int NQ, int EPW>
+void launch_stage_prep(int64_t nb, const KArgs& a) {
+  constexpr int PF = (Phys::DIM == 3 && NQ == 7) ? 0 : 1;  // as launch_stage
+  constexpr int NT = FusedCfg<Phys, NQ, EPW, false>::NT;
+  const int64_t chunk = (nb + 7) / 8;
+  stage_prep_kernel<Phys, NQ, EPW, PF><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
+      (double*)a.rhs, (const double*)a.Q, (double*)a.Qout, (const double*)a.vgeo,
+      (const double*)a.sgeo, a.pface, a.D, a.nreal, a.ntotal, a.rka, a.rkb, a.dt, chunk,
+      walk_back(a.Q, a.Qout), a.phys, a.advection);
 }
 
 template <class Phys, int NQ, int EPW>
@@ -78,7 +91,7 @@ void launch_face_update(int64_t nb, const KArgs& a) {
   face_update_kernel<Phys, NQ, EPW><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
       (double*)a.rhs, (const double*)a.Q, (double*)a.Qout, (const double*)a.vgeo,
       (const double*)a.sgeo, a.nreal, a.ntotal, a.vmapM, a.vmapP, a.bndy, a.rka, a.rkb, a.dt,
-      chunk, inplace_direction(), a.phys);
+      chunk, inplace_direction(), a.phys, a.advection);
 }
 
 // STAGE_ONLY: the fused kernels alone (stage, face + update) are
@@ -93,6 +106,8 @@ int launch(int op, const KArgs& a) {
   if constexpr (STAGE_ONLY) {
     if (op == OP_STAGE)
       launch_stage<Phys, NQ, EPW>(nb, a);
+    else if (op == OP_STAGE_PREP)
+      launch_stage_prep<Phys, NQ, EPW>(nb, a);
     else if (op == OP_FACE_UPDATE)
       launch_face_update<Phys, NQ, EPW>(nb, a);
     else
@@ -127,6 +142,9 @@ int launch(int op, const KArgs& a) {
       }
       case OP_STAGE:
         launch_stage<Phys, NQ, EPW>(nb, a);
+        break;
+      case OP_STAGE_PREP:
+        launch_stage_prep<Phys, NQ, EPW>(nb, a);
         break;
       case OP_FACE_UPDATE:
         launch_face_update<Phys, NQ, EPW>(nb, a);

@@ -27,6 +27,14 @@
 // vmapM must be the canonical face mask of `mappings` (checked by prepare).
 // Beside the single-solve stage there is a batched stage over a work table
 // of (solve, first element) entries, as in batched.hip.
+//
+// The mixed variant (MIXED = true; canary_perturb_mixed_*) keeps in FP64
+// what the compact one rounds: the metric terms, MJ and MJI (vg64, double
+// (Np, dim*dim + 2, nelem)), the normals and sMJ (sg64, 4 doubles per face
+// node) and the LSRK register dq (rhs, double).  The state q' and the
+// volume and face RHS arithmetic stay FP32: the stage's own RHS L(q') is
+// summed in FP32 from zero, then dq = dq + L and q' + ((rkb*dt) dq) MJI are
+// formed in FP64.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -83,12 +91,12 @@ __device__ __forceinline__ float pprime(float P0, float rt0, float rtp, float ka
 }
 
 // ------------------------------------------------------------ prepare
-template <int DIM, int NQ>
+template <int DIM, int NQ, class GT>  // GT: geometry word (float compact, double mixed)
 __global__ __launch_bounds__(256) void prepare_kernel(
     int64_t nelem, int64_t ntotal, const double* __restrict__ vgeo, const double* __restrict__ sgeo,
     const int64_t* __restrict__ vmapM, const int64_t* __restrict__ vmapP,
     const int64_t* __restrict__ bndy, const double* __restrict__ Q0, const double* __restrict__ Q,
-    double kappa_d, double p0, double R, float* __restrict__ vg, float* __restrict__ sg,
+    double kappa_d, double p0, double R, GT* __restrict__ vg, GT* __restrict__ sg,
     int32_t* __restrict__ nbr, int8_t* __restrict__ bnd, double* __restrict__ bg,
     float* __restrict__ qp) {
   using G = Dims<DIM>;
@@ -99,16 +107,16 @@ __global__ __launch_bounds__(256) void prepare_kernel(
     const int64_t e = g / Np;
     const int n = (int)(g - e * Np);
     const double* v = vgeo + (size_t)G::NVG * Np * e;
-    float* o = vg + (size_t)G::NV * Np * e;
+    GT* o = vg + (size_t)G::NV * Np * e;
 #pragma unroll
-    for (int c = 0; c < G::NV; ++c) o[n + Np * c] = (float)v[n + Np * c];
+    for (int c = 0; c < G::NV; ++c) o[n + Np * c] = (GT)v[n + Np * c];
     const double* q0 = Q0 + (size_t)NS * Np * e;
     const double* q = Q + (size_t)NS * Np * e;
     const double rho0 = q0[n + G::RHO * Np], rt0 = q0[n + G::E * Np];
     double* b = bg + (size_t)3 * Np * e;
     b[n] = rho0;
     b[n + Np] = rt0;
-    b[n + 2 * Np] = p0 * pow(R * rt0 / p0, kappa_d);
+    b[n + 2 * Np] = p0 * crpow::pow_cr(R * rt0 / p0, kappa_d);
     float* qo = qp + (size_t)NS * Np * e;
 #pragma unroll
     for (int s = 0; s < NS; ++s) qo[n + Np * s] = (float)(q[n + Np * s] - q0[n + Np * s]);
@@ -118,11 +126,11 @@ __global__ __launch_bounds__(256) void prepare_kernel(
     const int fi = (int)(g - e * NFN);
     const int f = fi / Nfp, fn = fi - f * Nfp;
     const double* s = sgeo + (size_t)G::NSG * g;
-    float* so = sg + (size_t)4 * g;
-    so[0] = (float)s[0];
-    so[1] = (float)s[1];
-    so[2] = DIM == 3 ? (float)s[2] : 0.f;
-    so[3] = (float)s[DIM];
+    GT* so = sg + (size_t)4 * g;
+    so[0] = (GT)s[0];
+    so[1] = (GT)s[1];
+    so[2] = DIM == 3 ? (GT)s[2] : (GT)0;
+    so[3] = (GT)s[DIM];
     const int64_t idM = vmapM[g], idP = vmapP[g];
     if (idM != (int64_t)Np * e + fmask<DIM, NQ>(f, fn) + 1) p32_error(5, f + 1, e + 1, idM);
     int64_t pn = idP - 1;
@@ -179,15 +187,35 @@ struct Lds32 {
   float B[EPW][C::BUF];         // fluxes, then face terms + rhs
 };
 
+// Face record of the geometry word GT: (nx, ny, nz, sMJ) as floats.
+__device__ __forceinline__ void load_face(const float* sg, size_t fo, float nn[4]) {
+  const float4 s4 = *reinterpret_cast<const float4*>(sg + 4 * fo);
+  nn[0] = s4.x;
+  nn[1] = s4.y;
+  nn[2] = s4.z;
+  nn[3] = s4.w;
+}
+__device__ __forceinline__ void load_face(const double* sg, size_t fo, float nn[4]) {
+  const double2 a = *reinterpret_cast<const double2*>(sg + 4 * fo);
+  const double2 b = *reinterpret_cast<const double2*>(sg + 4 * fo + 2);
+  nn[0] = (float)a.x;
+  nn[1] = (float)a.y;
+  nn[2] = (float)b.x;
+  nn[3] = (float)b.y;
+}
+
 // One stage of the EPW-element block at e0 (NT threads, NT >= EPW * Np).
-template <int DIM, int NQ, int EPW>
-__device__ __forceinline__ void body(Lds32<DIM, NQ, EPW>& L, float* __restrict__ rhs,
+// GT: geometry word (vg, sg); RT: LSRK register word (rhs).  float/float is
+// the compact form; double/double the mixed one.
+template <int DIM, int NQ, int EPW, class GT = float, class RT = float>
+__device__ __forceinline__ void body(Lds32<DIM, NQ, EPW>& L, RT* __restrict__ rhs,
                                      const float* __restrict__ qin, float* __restrict__ qout,
-                                     const double* __restrict__ bg, const float* __restrict__ vg,
-                                     const float* __restrict__ sg, const int32_t* __restrict__ nbr,
+                                     const double* __restrict__ bg, const GT* __restrict__ vg,
+                                     const GT* __restrict__ sg, const int32_t* __restrict__ nbr,
                                      const int8_t* __restrict__ bnd, const double* __restrict__ D,
-                                     int64_t e0, int64_t nreal, float rka, double bdt,
+                                     int64_t e0, int64_t nreal, double rka, double bdt,
                                      const P32Params& p) {
+  constexpr bool REG64 = sizeof(RT) == 8;
   using C = Cfg32<DIM, NQ, EPW>;
   using G = Dims<DIM>;
   constexpr int Np = C::Np, Nfp = C::Nfp, NFN = C::NFN, NT = C::NT, NS = G::NS;
@@ -205,7 +233,8 @@ __device__ __forceinline__ void body(Lds32<DIM, NQ, EPW>& L, float* __restrict__
   const int fsl = t / NFN, ffi = t - fsl * NFN;
   const bool fvalid = t < EPW * NFN && e0 + fsl < nreal;
   const size_t fo = fvalid ? (size_t)ffi + (size_t)NFN * (size_t)(e0 + fsl) : (size_t)NFN * (size_t)e0;
-  const float4 s4 = *reinterpret_cast<const float4*>(sg + 4 * fo);
+  float fn4[4];
+  load_face(sg, fo, fn4);
   const int32_t pnode_l = nbr[fo];
   const int fbc_l = bnd[(fvalid ? ffi / Nfp : 0) + NF * (fvalid ? e0 + fsl : e0)];
   const double dval = D[t < NQ * NQ ? t : 0];
@@ -215,18 +244,22 @@ __device__ __forceinline__ void body(Lds32<DIM, NQ, EPW>& L, float* __restrict__
   const float* qe = qin + (size_t)NS * Np * el;
 #pragma unroll
   for (int s = 0; s < NS; ++s) q[s] = qe[nl + Np * s];
-  const float* v = vg + (size_t)G::NV * Np * el;
+  const GT* v = vg + (size_t)G::NV * Np * el;
 #pragma unroll
-  for (int c = 0; c < DIM * DIM; ++c) met[c] = v[nl + Np * c];
-  const float MJ = v[nl + Np * DIM * DIM];
+  for (int c = 0; c < DIM * DIM; ++c) met[c] = (float)v[nl + Np * c];
+  const float MJ = (float)v[nl + Np * DIM * DIM];
   const double MJI = (double)v[nl + Np * (DIM * DIM + 1)];
   const double* b = bg + (size_t)3 * Np * el;
   const double b0 = b[nl], b1 = b[nl + Np], b2 = b[nl + 2 * Np];
-  const float* re = rhs + (size_t)NS * Np * el;
+  const RT* re = rhs + (size_t)NS * Np * el;
+  // compact: the register is the FP32 accumulator; mixed: the FP64
+  // register waits in rreg and the stage's RHS is summed from zero
+  RT rreg[NS];
 #pragma unroll
-  for (int s = 0; s < NS; ++s) r[s] = re[nl + Np * s];
+  for (int s = 0; s < NS; ++s) rreg[s] = re[nl + Np * s];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) r[s] = REG64 ? 0.f : (float)rreg[s];
   // the P-side gather of the first face item, as soon as its map is here
-  const float fn4[4] = {s4.x, s4.y, s4.z, s4.w};
   const int fbc = fvalid ? fbc_l : 1;
   const bool gat = fvalid && fbc == 0;
   const int32_t pnode = gat ? pnode_l : (int32_t)(e0 * Np);
@@ -337,8 +370,8 @@ __device__ __forceinline__ void body(Lds32<DIM, NQ, EPW>& L, float* __restrict__
     const int sl = item / NFN, fi = item - sl * NFN;
     if (e0 + sl >= nreal) continue;
     const size_t fo = (size_t)fi + (size_t)NFN * (size_t)(e0 + sl);
-    const float4 s4 = *reinterpret_cast<const float4*>(sg + 4 * fo);
-    const float nn[4] = {s4.x, s4.y, s4.z, s4.w};
+    float nn[4];
+    load_face(sg, fo, nn);
     const int bc = bnd[fi / Nfp + NF * (e0 + sl)];
     float ql[NS];
 #pragma unroll
@@ -367,12 +400,19 @@ __device__ __forceinline__ void body(Lds32<DIM, NQ, EPW>& L, float* __restrict__
   }
   if (valid) {  // non-temporal stores: not read again in this stage
     float* qo = qout + (size_t)NS * Np * e;
-    float* ro = rhs + (size_t)NS * Np * e;
+    RT* ro = rhs + (size_t)NS * Np * e;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       const float rs = L.B[slot][NFN * NS + s * Np + n];
-      __builtin_nontemporal_store((float)((double)q[s] + (bdt * (double)rs) * MJI), &qo[n + Np * s]);
-      __builtin_nontemporal_store(rka * rs, &ro[n + Np * s]);
+      if constexpr (REG64) {
+        const double dq = (double)rreg[s] + (double)rs;
+        __builtin_nontemporal_store((float)((double)q[s] + (bdt * dq) * MJI), &qo[n + Np * s]);
+        __builtin_nontemporal_store(rka * dq, &ro[n + Np * s]);
+      } else {
+        __builtin_nontemporal_store((float)((double)q[s] + (bdt * (double)rs) * MJI),
+                                    &qo[n + Np * s]);
+        __builtin_nontemporal_store((float)rka * rs, &ro[n + Np * s]);
+      }
     }
   }
 }
@@ -392,7 +432,22 @@ __global__ __launch_bounds__((Cfg32<DIM, NQ, single_epw<DIM, NQ>()>::NT)) void s
   __shared__ Lds32<DIM, NQ, EPW> L;
   const int64_t e0 = stage_block(blockIdx.x, chunk, reverse) * EPW;
   if (e0 >= nreal) return;
-  body<DIM, NQ, EPW>(L, rhs, qin, qout, bg, vg, sg, nbr, bnd, D, e0, nreal, rka, bdt, p);
+  body<DIM, NQ, EPW>(L, rhs, qin, qout, bg, vg, sg, nbr, bnd, D, e0, nreal, (double)rka, bdt, p);
+}
+
+// the mixed variant: FP64 geometry and LSRK register
+template <int DIM, int NQ>
+__global__ __launch_bounds__((Cfg32<DIM, NQ, single_epw<DIM, NQ>()>::NT)) void stage_mixed_kernel(
+    double* __restrict__ rhs, const float* __restrict__ qin, float* __restrict__ qout,
+    const double* __restrict__ bg, const double* __restrict__ vg, const double* __restrict__ sg,
+    const int32_t* __restrict__ nbr, const int8_t* __restrict__ bnd, const double* __restrict__ D,
+    int64_t nreal, double rka, double bdt, int64_t chunk, int reverse, P32Params p) {
+  constexpr int EPW = single_epw<DIM, NQ>();
+  __shared__ Lds32<DIM, NQ, EPW> L;
+  const int64_t e0 = stage_block(blockIdx.x, chunk, reverse) * EPW;
+  if (e0 >= nreal) return;
+  body<DIM, NQ, EPW, double, double>(L, rhs, qin, qout, bg, vg, sg, nbr, bnd, D, e0, nreal, rka,
+                                     bdt, p);
 }
 
 // batched: one workgroup per work-table entry, entries of one XCD contiguous
@@ -416,12 +471,12 @@ __global__ __launch_bounds__((Cfg32<DIM, NQ, EPW>::NT)) void batch32_kernel(
   const P32Params p = params_of(sv.phys);
   body<DIM, NQ, EPW>(L, (float*)sv.rhs, (const float*)sv.q[parity], (float*)sv.q[1 - parity],
                      (const double*)sv.bg, (const float*)sv.vg, (const float*)sv.sg, sv.nbr,
-                     sv.bnd, sv.D, it.e0, sv.nreal, (float)rka, rkb * sv.dt, p);
+                     sv.bnd, sv.D, it.e0, sv.nreal, rka, rkb * sv.dt, p);
 }
 
 int launch_status() { return hipGetLastError() == hipSuccess ? 0 : CANARY_ERR_LAUNCH; }
 
-template <int DIM, int NQ>
+template <int DIM, int NQ, class GT = float>
 int prepare(int64_t nelem, int64_t ntotal, const void* vgeo, const void* sgeo, const int64_t* vmapM,
             const int64_t* vmapP, const int64_t* bndy, const void* Q0, const void* Q,
             const canary_phys* ph, void* vg, void* sg, int32_t* nbr, int8_t* bnd, void* bg, void* qp,
@@ -429,10 +484,36 @@ int prepare(int64_t nelem, int64_t ntotal, const void* vgeo, const void* sgeo, c
   const int64_t work = nelem * ipow(NQ, DIM);
   int64_t blocks = (work + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  prepare_kernel<DIM, NQ><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
+  prepare_kernel<DIM, NQ, GT><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
       nelem, ntotal, (const double*)vgeo, (const double*)sgeo, vmapM, vmapP, bndy,
-      (const double*)Q0, (const double*)Q, ph->c_p / ph->c_v, ph->p0, ph->R_gas, (float*)vg,
-      (float*)sg, nbr, bnd, (double*)bg, (float*)qp);
+      (const double*)Q0, (const double*)Q, ph->c_p / ph->c_v, ph->p0, ph->R_gas, (GT*)vg,
+      (GT*)sg, nbr, bnd, (double*)bg, (float*)qp);
+  return launch_status();
+}
+
+template <int DIM, int NQ>
+int prepare_mixed(int64_t nelem, int64_t ntotal, const void* vgeo, const void* sgeo,
+                  const int64_t* vmapM, const int64_t* vmapP, const int64_t* bndy, const void* Q0,
+                  const void* Q, const canary_phys* ph, void* vg, void* sg, int32_t* nbr,
+                  int8_t* bnd, void* bg, void* qp, hipStream_t st) {
+  return prepare<DIM, NQ, double>(nelem, ntotal, vgeo, sgeo, vmapM, vmapP, bndy, Q0, Q, ph, vg, sg,
+                                  nbr, bnd, bg, qp, st);
+}
+
+template <int DIM, int NQ>
+int stage_mixed(void* rhs, const void* qin, void* qout, const void* bg, const void* vg,
+                const void* sg, const int32_t* nbr, const int8_t* bnd, const double* D,
+                int64_t nreal, double rka, double rkb, double dt, const canary_phys* ph,
+                hipStream_t st) {
+  constexpr int EPW = single_epw<DIM, NQ>();
+  using C = Cfg32<DIM, NQ, EPW>;
+  const int64_t nb = (nreal + EPW - 1) / EPW;
+  const int64_t chunk = (nb + 7) / 8;
+  P32Params p{(float)ph->gamma, (float)(ph->c_p / ph->c_v), (float)ph->gravity};
+  stage_mixed_kernel<DIM, NQ><<<dim3((unsigned)(8 * chunk)), dim3(C::NT), 0, st>>>(
+      (double*)rhs, (const float*)qin, (float*)qout, (const double*)bg, (const double*)vg,
+      (const double*)sg, nbr, bnd, D, nreal, rka, rkb * dt, chunk,
+      (uintptr_t)qin > (uintptr_t)qout ? 1 : 0, p);
   return launch_status();
 }
 
@@ -589,6 +670,49 @@ extern "C" int canary_stage_perturb32(int dim, int N, void* rhs, const void* qin
   hipStream_t st = (hipStream_t)stream;
   P32_DISPATCH(dim, N, stage,
                (rhs, qin, qout, bg, vg32, sg32, nbr, bnd8, D, nreal, rka, rkb, dt, phys, st))
+}
+
+extern "C" int canary_perturb_mixed_prepare(int dim, int N, int64_t nelem, int64_t nelem_total,
+                                           const void* vgeo, const void* sgeo,
+                                           const int64_t* vmapM, const int64_t* vmapP,
+                                           const int64_t* elemtobndy, const void* Q0,
+                                           const void* Q, const canary_phys* phys, void* vg64,
+                                           void* sg64, int32_t* nbr, int8_t* bnd8, void* bg,
+                                           void* qp, void* stream) {
+  if (nelem < 0 || nelem_total < nelem) return CANARY_ERR_ARG;
+  if (nelem == 0) return 0;
+  if (!vgeo || !sgeo || !vmapM || !vmapP || !elemtobndy || !Q0 || !Q || !vg64 || !sg64 || !nbr ||
+      !bnd8 || !bg || !qp)
+    return CANARY_ERR_ARG;
+  canary_phys d;
+  if (!phys) {
+    canary_phys_default(&d);
+    phys = &d;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  P32_DISPATCH(dim, N, prepare_mixed, (nelem, nelem_total, vgeo, sgeo, vmapM, vmapP, elemtobndy,
+                                       Q0, Q, phys, vg64, sg64, nbr, bnd8, bg, qp, st))
+}
+
+extern "C" int canary_stage_perturb_mixed(int dim, int N, void* rhs64, const void* qin,
+                                          void* qout, const void* bg, const void* vg64,
+                                          const void* sg64, const int32_t* nbr,
+                                          const int8_t* bnd8, const double* D, int64_t nreal,
+                                          double rka, double rkb, double dt,
+                                          const canary_phys* phys, void* stream) {
+  if (nreal < 0) return CANARY_ERR_ARG;
+  if (nreal == 0) return 0;
+  if (!rhs64 || !qin || !qout || !bg || !vg64 || !sg64 || !nbr || !bnd8 || !D || qin == qout)
+    return CANARY_ERR_ARG;
+  if (((uintptr_t)sg64 & 15) != 0) return CANARY_ERR_ARG;  // read as double2 pairs
+  canary_phys d;
+  if (!phys) {
+    canary_phys_default(&d);
+    phys = &d;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  P32_DISPATCH(dim, N, stage_mixed,
+               (rhs64, qin, qout, bg, vg64, sg64, nbr, bnd8, D, nreal, rka, rkb, dt, phys, st))
 }
 
 extern "C" int canary_perturb32_reconstruct(int dim, int N, int64_t nelem, const void* qp,

@@ -16,11 +16,14 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include "crpow.h"
+
 namespace canary {
 
 struct PhysParams {
   double gamma, p0, R, cp, cv, g, kappa;  // kappa = cp / cv
   double swe_g, dnl;
+  const crpow::KappaTable* ktab;  // pow table of kappa (device memory), or null
 };
 
 // Julia's max(a, b) for Float64: NaN if either is NaN, else the larger.
@@ -49,7 +52,8 @@ struct FastOps {
     e = fma(-x, r, 1.0);
     return fma(r, e, r);
   }
-  __device__ static double powk(double x, double k) {
+  __device__ static double powk(double x, const PhysParams& p) {
+    const double k = p.kappa;
     double lx;
     if (x > 0.70710678118654752 && x < 1.4142135623730950) {
       const double s = (x - 1) * recip(x + 1);
@@ -91,13 +95,16 @@ struct FastOps {
 // ExactOps: the exact-order kernels (compiled with -ffp-contract=off): IEEE
 // division, and every expression of the reference in its association with
 // no multiply-add fused.  pow is a library routine the reference does not
-// fix (Julia's own, glibc's in the oracle, OCML's on the device differ in
-// their last bits); it is the series above, which is exact at x = 1 (so
-// states whose pressure needs no pow stay bit-identical to the oracle) and
-// made the 3D N=4 exact-order stage 7% faster than OCML's pow.
+// fix (Julia's libm pow, glibc's and OCML's differ in their last bits); here
+// it is the correctly rounded x^y of crpow.h, so the pressure is a function
+// of the state alone and the oracle, whose pow is correctly rounded too,
+// gets the same bits (with kappa's table, when the host has made one, the
+// same bits come from the cheaper fast phase).
 struct ExactOps {
   __device__ static double recip(double x) { return 1 / x; }
-  __device__ static double powk(double x, double k) { return FastOps::powk(x, k); }
+  __device__ static double powk(double x, const PhysParams& p) {
+    return crpow::pow_cr_tab(x, p.kappa, p.ktab);
+  }
 };
 
 // ------------------------------------------------------------ 3D Euler set2c
@@ -110,7 +117,7 @@ struct Euler3DT {
   enum { U = 0, V = 1, W = 2, RHO = 3, E = 4 };
 
   __device__ static double pressure(const double* q, const PhysParams& p) {
-    return p.p0 * Ops::powk(p.R * q[E] / p.p0, p.kappa);
+    return p.p0 * Ops::powk(p.R * q[E] / p.p0, p);
   }
   // met[d + DIM*c] = d(xi_d)/d(x_c) in the vgeo field order.
   __device__ static void volume_flux(const double* q, double P, const double* met, double MJ,
@@ -181,7 +188,7 @@ struct Euler2DT {
   enum { U = 0, V = 1, RHO = 2, E = 3 };
 
   __device__ static double pressure(const double* q, const PhysParams& p) {
-    return p.p0 * Ops::powk(p.R * q[E] / p.p0, p.kappa);
+    return p.p0 * Ops::powk(p.R * q[E] / p.p0, p);
   }
   __device__ static void volume_flux(const double* q, double P, const double* met, double MJ,
                                      const PhysParams&, double F[DIM][NSR]) {
@@ -315,7 +322,7 @@ struct SWE2DF32 {
   using Real = float;
   static constexpr int DIM = 2, NS = 4, NSR = 3, NVGEO = 8, NSGEO = 4;
   static constexpr int iMJ = 4, iMJI = 5;
-  static constexpr bool HAS_P = false, B_FROM_M = true, HAS_ADVECTION = false;
+  static constexpr bool HAS_P = false, B_FROM_M = true, HAS_ADVECTION = true;
   enum { U = 0, V = 1, H = 2, B = 3 };
 
   __device__ static float pressure(const float*, const PhysParams&) { return 0.f; }

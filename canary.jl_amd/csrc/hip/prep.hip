@@ -11,6 +11,7 @@
 #include <stdint.h>
 
 #include "../../../include/canary_dg.h"
+#include "crpow.h"
 
 namespace canary {
 namespace prep {
@@ -61,7 +62,7 @@ __global__ __launch_bounds__(256) void bubble_kernel(int dim, int64_t Np, int64_
     if (b.jitter != 0.0) th = th + b.jitter * hash_jitter(b.seed, x, y, z);
     const double pik = 1.0 - g / (c_p * th) * vert;
     const double c = c_v / R_gas;
-    const double rho = p0 / (R_gas * th) * pow(pik, c);
+    const double rho = p0 / (R_gas * th) * canary::crpow::pow_cr(pik, c);
     double* q = Q + (size_t)Np * ns * e;
     const int RHO = dim, E = dim + 1;
     q[n + Np * RHO] = rho;
@@ -90,11 +91,11 @@ __global__ __launch_bounds__(256) void vortex_kernel(int64_t Np, int64_t nelem,
     const double du = -v.beta / (2 * M_PI) * dy * f;
     const double dv = v.beta / (2 * M_PI) * dx * f;
     const double T = v.p_inf / v.rho_inf - (gam - 1) * v.beta * v.beta / (8 * gam * M_PI * M_PI) * f * f;
-    const double Kent = v.p_inf / pow(v.rho_inf, gam);
-    const double rho = pow(T / Kent, 1.0 / (gam - 1));
-    const double p = Kent * pow(rho, gam);
+    const double Kent = v.p_inf / canary::crpow::pow_cr(v.rho_inf, gam);
+    const double rho = canary::crpow::pow_cr(T / Kent, 1.0 / (gam - 1));
+    const double p = Kent * canary::crpow::pow_cr(rho, gam);
     const double u = v.u0 + du, w = v.v0 + dv;
-    const double rth = pow(p, 1.0 / gam);
+    const double rth = canary::crpow::pow_cr(p, 1.0 / gam);
     double* q = Q + (size_t)Np * ns * e;
     q[n + Np * 0] = rho * u;
     q[n + Np * 1] = rho * w;

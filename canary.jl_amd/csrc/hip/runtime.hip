@@ -3,8 +3,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../../include/canary_dg.h"
 #include "launch.h"
+#include "pow_registry.h"
 
 extern "C" {
 int canary_k_euler3d(int op, int N, const canary::KArgs* a);
@@ -24,6 +27,74 @@ int canary_k_batch_status(int64_t* info, int clear);
 int canary_k_batch_fma_status(int64_t* info, int clear);
 int canary_k_geometry_status(int64_t* info, int clear);
 int canary_k_aux_status(int64_t* info, int clear);
+int canary_k_prepared_status(int64_t* info, int clear);
+}
+
+namespace canary {
+
+// The pow tables of every registered kappa, per device (see pow_registry.h).
+static __device__ crpow::KappaTable g_ktab[kMaxKappa];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_ktab_mu;
+crpow::KappaTable* g_ktab_base[kMaxDevices];
+double g_ktab_kappa[kMaxDevices][kMaxKappa];
+int g_ktab_count[kMaxDevices];
+
+bool same_bits(double a, double b) {
+  uint64_t x, y;
+  memcpy(&x, &a, 8);
+  memcpy(&y, &b, 8);
+  return x == y;
+}
+
+// device index and the array's address on it (looked up once per device)
+int ktab_device(int* dev) {
+  if (hipGetDevice(dev) != hipSuccess || *dev < 0 || *dev >= kMaxDevices) return -1;
+  if (!g_ktab_base[*dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(g_ktab)) != hipSuccess || !p) return -1;
+    g_ktab_base[*dev] = static_cast<crpow::KappaTable*>(p);
+  }
+  return 0;
+}
+}  // namespace
+
+const crpow::KappaTable* kappa_table(double kappa, hipStream_t stream) {
+  std::lock_guard<std::mutex> lock(g_ktab_mu);
+  int dev = -1;
+  if (ktab_device(&dev)) return nullptr;
+  for (int i = 0; i < g_ktab_count[dev]; ++i)
+    if (same_bits(g_ktab_kappa[dev][i], kappa)) return g_ktab_base[dev] + i;
+  if (g_ktab_count[dev] >= kMaxKappa) return nullptr;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (stream && (hipStreamIsCapturing(stream, &cs) != hipSuccess ||
+                 cs != hipStreamCaptureStatusNone))
+    return nullptr;
+  static crpow::KappaTable host;  // built under the lock
+  if (!crpow::kappa_table_build(kappa, &host)) return nullptr;
+  const int slot = g_ktab_count[dev];
+  if (hipMemcpy(g_ktab_base[dev] + slot, &host, sizeof(host), hipMemcpyHostToDevice) != hipSuccess)
+    return nullptr;
+  g_ktab_kappa[dev][slot] = kappa;
+  g_ktab_count[dev] = slot + 1;
+  return g_ktab_base[dev] + slot;
+}
+
+const crpow::KappaTable* kappa_tables(int* n) {
+  std::lock_guard<std::mutex> lock(g_ktab_mu);
+  int dev = -1;
+  *n = 0;
+  if (ktab_device(&dev)) return nullptr;
+  *n = g_ktab_count[dev];
+  return g_ktab_base[dev];
+}
+
+}  // namespace canary
+
+extern "C" int canary_pow_register(double kappa, void* stream) {
+  return canary::kappa_table(kappa, (hipStream_t)stream) ? 0 : CANARY_ERR_UNSUPPORTED;
 }
 
 namespace {
@@ -41,7 +112,7 @@ KFn select(int eqn, int dim, int prec) {
   return nullptr;
 }
 
-canary::PhysParams to_params(const canary_phys* ph) {
+canary::PhysParams to_params(const canary_phys* ph, void* stream = nullptr, int eqn = -1) {
   canary_phys d;
   if (!ph) {
     canary_phys_default(&d);
@@ -57,6 +128,7 @@ canary::PhysParams to_params(const canary_phys* ph) {
   p.kappa = ph->c_p / ph->c_v;
   p.swe_g = ph->swe_gravity;
   p.dnl = ph->delta_nl;
+  p.ktab = eqn == CANARY_EULER ? canary::kappa_table(p.kappa, (hipStream_t)stream) : nullptr;
   return p;
 }
 
@@ -88,7 +160,7 @@ extern "C" int canary_volumerhs(int eqn, int dim, int N, int prec, void* rhs, co
   a.D = D;
   a.nreal = nreal;
   a.ntotal = nreal;
-  a.phys = to_params(phys);
+  a.phys = to_params(phys, stream, eqn);
   a.stream = (hipStream_t)stream;
   return fn(canary::OP_VOLUME, N, &a);
 }
@@ -112,7 +184,7 @@ extern "C" int canary_facerhs(int eqn, int dim, int N, int prec, void* rhs, cons
   a.vmapM = vmapM;
   a.vmapP = vmapP;
   a.bndy = elemtobndy;
-  a.phys = to_params(phys);
+  a.phys = to_params(phys, stream, eqn);
   a.stream = (hipStream_t)stream;
   return fn(canary::OP_FACE, N, &a);
 }
@@ -146,10 +218,11 @@ extern "C" int canary_stage_fused(int eqn, int dim, int N, int prec, void* rhs, 
                                   const double* D, int64_t nreal, int64_t nelem_total,
                                   const int64_t* vmapM, const int64_t* vmapP,
                                   const int64_t* elemtobndy, double rka, double rkb, double dt,
-                                  const canary_phys* phys, void* stream) {
+                                  int advection, const canary_phys* phys, void* stream) {
   KFn fn;
   int rc = check_common(eqn, dim, N, prec, nreal, &fn, true);
   if (rc) return rc;
+  if (advection && eqn != CANARY_SWE) return CANARY_ERR_ARG;
   if (nreal == 0) return 0;
   if (nelem_total < nreal) return CANARY_ERR_ARG;
   if (!rhs || !Qin || !Qout || !vgeo || !sgeo || !D || !vmapP || !elemtobndy)
@@ -170,7 +243,8 @@ extern "C" int canary_stage_fused(int eqn, int dim, int N, int prec, void* rhs, 
   a.rka = rka;
   a.rkb = rkb;
   a.dt = dt;
-  a.phys = to_params(phys);
+  a.advection = advection;
+  a.phys = to_params(phys, stream, eqn);
   a.stream = (hipStream_t)stream;
   return fn(canary::OP_STAGE, N, &a);
 }
@@ -180,10 +254,11 @@ extern "C" int canary_facerhs_update(int eqn, int dim, int N, int prec, void* rh
                                      const void* sgeo, int64_t nreal, int64_t nelem_total,
                                      const int64_t* vmapM, const int64_t* vmapP,
                                      const int64_t* elemtobndy, double rka, double rkb, double dt,
-                                     const canary_phys* phys, void* stream) {
+                                     int advection, const canary_phys* phys, void* stream) {
   KFn fn;
   int rc = check_common(eqn, dim, N, prec, nreal, &fn, true);
   if (rc) return rc;
+  if (advection && eqn != CANARY_SWE) return CANARY_ERR_ARG;
   if (nreal == 0) return 0;
   if (nelem_total < nreal) return CANARY_ERR_ARG;
   if (!rhs || !Qin || !Qout || !vgeo || !sgeo || !vmapP || !elemtobndy)
@@ -203,20 +278,54 @@ extern "C" int canary_facerhs_update(int eqn, int dim, int N, int prec, void* rh
   a.rka = rka;
   a.rkb = rkb;
   a.dt = dt;
-  a.phys = to_params(phys);
+  a.advection = advection;
+  a.phys = to_params(phys, stream, eqn);
   a.stream = (hipStream_t)stream;
   return fn(canary::OP_FACE_UPDATE, N, &a);
+}
+
+extern "C" int canary_stage_prepared(int eqn, int dim, int N, int prec, void* rhs,
+                                     const void* Qin, void* Qout, const void* pvgeo,
+                                     const void* psgeo, const int32_t* pface, const double* D,
+                                     int64_t nreal, int64_t nelem_total, double rka, double rkb,
+                                     double dt, int advection, const canary_phys* phys,
+                                     void* stream) {
+  KFn fn;
+  int rc = check_common(eqn, dim, N, prec, nreal, &fn, true);
+  if (rc) return rc;
+  if (advection && eqn != CANARY_SWE) return CANARY_ERR_ARG;
+  if (nreal == 0) return 0;
+  if (nelem_total < nreal) return CANARY_ERR_ARG;
+  if (!rhs || !Qin || !Qout || !pvgeo || !psgeo || !pface || !D) return CANARY_ERR_ARG;
+  if (Qin == (const void*)Qout) return CANARY_ERR_ARG;
+  canary::KArgs a = {};
+  a.rhs = rhs;
+  a.Q = Qin;
+  a.Qout = Qout;
+  a.vgeo = pvgeo;
+  a.sgeo = psgeo;
+  a.pface = pface;
+  a.D = D;
+  a.nreal = nreal;
+  a.ntotal = nelem_total;
+  a.rka = rka;
+  a.rkb = rkb;
+  a.dt = dt;
+  a.advection = advection;
+  a.phys = to_params(phys, stream, eqn);
+  a.stream = (hipStream_t)stream;
+  return fn(canary::OP_STAGE_PREP, N, &a);
 }
 
 extern "C" int canary_check_errors(void* stream, int64_t* info) {
   if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return CANARY_ERR_LAUNCH;
   int64_t tmp[4];
   int first = 0;
-  int (*readers[11])(int64_t*, int) = {
+  int (*readers[12])(int64_t*, int) = {
       canary_k_euler3d_status,     canary_k_euler2d_status,     canary_k_swe2d_status,
       canary_k_euler3d_fma_status, canary_k_euler2d_fma_status, canary_k_swe2d_fma_status,
       canary_k_perturb32_status,   canary_k_batch_status,       canary_k_batch_fma_status,
-      canary_k_geometry_status,    canary_k_aux_status};
+      canary_k_geometry_status,    canary_k_aux_status,         canary_k_prepared_status};
   for (auto rd : readers) {
     int rc = rd(tmp, 1);
     if (rc && !first) {

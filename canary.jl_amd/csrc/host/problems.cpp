@@ -23,6 +23,7 @@
 #include <cstring>
 
 #include "../../../include/canary_dg.h"
+#include "../hip/crpow.h"
 
 namespace {
 
@@ -89,7 +90,7 @@ extern "C" int canary_ic_bubble(int dim, int N, int64_t nelem, const double* vge
       if (b->jitter != 0.0) th = th + b->jitter * canary_hash_jitter(b->seed, x, y, z);
       double pik = 1.0 - g / (c_p * th) * vert;
       double c = c_v / R_gas;
-      double rho = p0 / (R_gas * th) * std::pow(pik, c);
+      double rho = p0 / (R_gas * th) * canary::crpow::pow_cr(pik, c);
       double* q = Q + (size_t)Np * ns * e;
       // set2nc (rho, u=0, v=0, w=0, theta) converted to set2c
       if (dim == 3) {
@@ -132,11 +133,11 @@ extern "C" int canary_ic_vortex2d(int N, int64_t nelem, const double* vgeo,
       double T = v->p_inf / v->rho_inf -
                  (gam - 1) * v->beta * v->beta / (8 * gam * M_PI * M_PI) * f * f;
       // isentropic: p / rho^gamma = p_inf / rho_inf^gamma
-      double Kent = v->p_inf / std::pow(v->rho_inf, gam);
-      double rho = std::pow(T / Kent, 1.0 / (gam - 1));
-      double p = Kent * std::pow(rho, gam);
+      double Kent = v->p_inf / canary::crpow::pow_cr(v->rho_inf, gam);
+      double rho = canary::crpow::pow_cr(T / Kent, 1.0 / (gam - 1));
+      double p = Kent * canary::crpow::pow_cr(rho, gam);
       double u = v->u0 + du, w = v->v0 + dv;
-      double rth = std::pow(p, 1.0 / gam);  // p0 = R = 1
+      double rth = canary::crpow::pow_cr(p, 1.0 / gam);  // p0 = R = 1
       double* q = Q + (size_t)Np * ns * e;
       q[n + Np * 0] = rho * u;
       q[n + Np * 1] = rho * w;
@@ -189,7 +190,7 @@ extern "C" double canary_cfl(int eqn, int dim, int N, const double* vgeo, const 
         const double* q = Q + (size_t)Np * 5 * e;
         const double* vg = vgeo + (size_t)Np * 14 * e;
         double rho = q[n + Np * 3], U = q[n], V = q[n + Np], W = q[n + 2 * Np], E = q[n + Np * 4];
-        double P = ph->p0 * std::pow(ph->R_gas * E / ph->p0, ph->c_p / ph->c_v);
+        double P = ph->p0 * canary::crpow::pow_cr(ph->R_gas * E / ph->p0, ph->c_p / ph->c_v);
         double c = rho * std::sqrt(ph->gamma * P / rho);
         double a = std::fabs(U * vg[n] + V * vg[n + 3 * Np] + W * vg[n + 6 * Np]) + c;
         double b = std::fabs(U * vg[n + Np] + V * vg[n + 4 * Np] + W * vg[n + 7 * Np]) + c;
@@ -201,7 +202,7 @@ extern "C" double canary_cfl(int eqn, int dim, int N, const double* vgeo, const 
         const double* q = Q + (size_t)Np * 4 * e;
         const double* vg = vgeo + (size_t)Np * 8 * e;
         double rho = q[n + Np * 2], U = q[n], V = q[n + Np], E = q[n + Np * 3];
-        double P = ph->p0 * std::pow(ph->R_gas * E / ph->p0, ph->c_p / ph->c_v);
+        double P = ph->p0 * canary::crpow::pow_cr(ph->R_gas * E / ph->p0, ph->c_p / ph->c_v);
         double c = rho * std::sqrt(ph->gamma * P / rho);
         double a = std::fabs(U * vg[n] + V * vg[n + 2 * Np]) + c;
         double b = std::fabs(U * vg[n + Np] + V * vg[n + 3 * Np]) + c;
@@ -234,4 +235,23 @@ extern "C" double canary_l2energysquared(int Np, int nstate, int nvgeo, int iMJ,
         energy += vgeo[i + (size_t)Np * (iMJ + (size_t)nvgeo * e)] * (q * q);
       }
   return energy;
+}
+
+// x^y correctly rounded: the pow of the exact-order kernels, on the host.
+extern "C" double canary_pow_cr(double x, double y) { return canary::crpow::pow_cr(x, y); }
+
+// The same through the fixed-exponent table of y (the kernels' path for a
+// gas's kappa), the table built on the host for this call.
+extern "C" double canary_pow_cr_table(double x, double y) {
+  static canary::crpow::KappaTable kt;
+  static double built = -1.0;
+  static bool ok = false;
+#pragma omp critical(canary_pow_table)
+  {
+    if (built != y) {
+      ok = canary::crpow::kappa_table_build(y, &kt);
+      built = y;
+    }
+  }
+  return canary::crpow::pow_cr_tab(x, y, ok ? &kt : nullptr);
 }

@@ -115,12 +115,14 @@ def updatesolution(dim, N, rhs, Q, vgeo, elems, rka, rkb, dt, eqn=EULER, advecti
 
 def stage_fused(dim, N, rhs, Qin, Qout, vgeo, sgeo, D, elems, vmapM, vmapP, elemtobndy,
                 rka, rkb, dt, eqn=EULER, phys=None, nelem_total=None, stream=None,
-                prec=FP64):
+                prec=FP64, advection=False):
     """One LSRK stage (volume + faces + update) in one pass; Qin -> Qout.
 
     prec: FP64 (the reference evaluation order, bit-identical to volumerhs +
     facerhs + updatesolution) or FP64_FMA (multiply-adds fused, pressure via
-    exp/log; within 1e-12 relative L2 per stage, faster)."""
+    exp/log; within 1e-12 relative L2 per stage, faster).  advection
+    (shallow water): the velocity is kept through the update, as in
+    updatesolution."""
     if prec not in (FP64, FP64_FMA):
         raise ValueError("stage_fused: prec must be FP64 or FP64_FMA")
     n = _nreal(elems)
@@ -141,15 +143,16 @@ def stage_fused(dim, N, rhs, Qin, Qout, vgeo, sgeo, D, elems, vmapM, vmapP, elem
     rc = lib().canary_stage_fused(eqn, dim, N, prec, _ptr(rhs), _ptr(Qin), _ptr(Qout),
                                   _ptr(vgeo), _ptr(sgeo), _ptr(D), n, ntot, pM,
                                   _ptr(vmapP), _ptr(elemtobndy), float(rka), float(rkb),
-                                  float(dt), _phys(phys), _stream(stream))
+                                  float(dt), 1 if advection else 0, _phys(phys), _stream(stream))
     check(rc, "stage_fused")
 
 
 def facerhs_update(dim, N, rhs, Qin, Qout, vgeo, sgeo, elems, vmapM, vmapP, elemtobndy,
                    rka, rkb, dt, eqn=EULER, phys=None, nelem_total=None, stream=None,
-                   prec=FP64):
+                   prec=FP64, advection=False):
     """facerhs! then updatesolution! of one stage in one pass (after a
-    separate volumerhs); Qin -> Qout, rhs in place.  prec as for stage_fused."""
+    separate volumerhs); Qin -> Qout, rhs in place.  prec and advection as
+    for stage_fused."""
     if prec not in (FP64, FP64_FMA):
         raise ValueError("facerhs_update: prec must be FP64 or FP64_FMA")
     n = _nreal(elems)
@@ -169,7 +172,7 @@ def facerhs_update(dim, N, rhs, Qin, Qout, vgeo, sgeo, elems, vmapM, vmapP, elem
     rc = lib().canary_facerhs_update(eqn, dim, N, prec, _ptr(rhs), _ptr(Qin), _ptr(Qout),
                                      _ptr(vgeo), _ptr(sgeo), n, ntot, pM, _ptr(vmapP),
                                      _ptr(elemtobndy), float(rka), float(rkb), float(dt),
-                                     _phys(phys), _stream(stream))
+                                     1 if advection else 0, _phys(phys), _stream(stream))
     check(rc, "facerhs_update")
 
 
@@ -203,6 +206,77 @@ def lsrk54(dim, N, Q, rhs, vgeo, sgeo, D, elems, vmapM, vmapP, elemtobndy, dt, n
                              _ptr(elemtobndy), _phys(phys), 1 if advection else 0,
                              1 if graph else 0, C.byref(alt), _stream(stream))
     check(rc, "lsrk54")
+    return Qalt if alt.value else Q
+
+
+def prepare_geometry(dim, N, vgeo, sgeo, vmapM, vmapP, elemtobndy, nelem, nelem_total=None,
+                     stream=None):
+    """The prepared arrays of the fused stage (pvgeo, psgeo, pface tensors),
+    built on the device from the reference ones; the checks it makes are
+    reported by check_errors()."""
+    Np, Nfp, nf = (N + 1) ** dim, (N + 1) ** (dim - 1), 2 * dim
+    ntot = nelem if nelem_total is None else int(nelem_total)
+    _need(vgeo, "vgeo", Np * _NV[dim] * nelem)
+    _need(sgeo, "sgeo", _NSG[dim] * Nfp * nf * nelem)
+    _need(vmapM, "vmapM", Nfp * nf * nelem, torch.int64)
+    _need(vmapP, "vmapP", Nfp * nf * nelem, torch.int64)
+    _need(elemtobndy, "elemtobndy", nf * nelem, torch.int64)
+    dv = vgeo.device
+    out = {"pvgeo": torch.empty(Np * (dim * dim + 1) * nelem, dtype=torch.float64, device=dv),
+           "psgeo": torch.empty((dim + 1) * Nfp * nf * nelem, dtype=torch.float64, device=dv),
+           "pface": torch.empty(nf * nelem, dtype=torch.int32, device=dv)}
+    rc = lib().canary_prepare_geometry(dim, N, nelem, ntot, _ptr(vgeo), _ptr(sgeo), _ptr(vmapM),
+                                       _ptr(vmapP), _ptr(elemtobndy), _ptr(out["pvgeo"]),
+                                       _ptr(out["psgeo"]), _ptr(out["pface"]), _stream(stream))
+    check(rc, "prepare_geometry")
+    return out
+
+
+def stage_prepared(dim, N, rhs, Qin, Qout, prep, D, elems, rka, rkb, dt, eqn=EULER, phys=None,
+                   nelem_total=None, stream=None, prec=FP64, advection=False):
+    """stage_fused over the prepared arrays `prep` (prepare_geometry)."""
+    if prec not in (FP64, FP64_FMA):
+        raise ValueError("stage_prepared: prec must be FP64 or FP64_FMA")
+    n = _nreal(elems)
+    Np, Nfp, nf = (N + 1) ** dim, (N + 1) ** (dim - 1), 2 * dim
+    ns = _NS[(eqn, dim)]
+    ntot = n if nelem_total is None else int(nelem_total)
+    _need(rhs, "rhs", Np * ns * n)
+    _need(Qin, "Qin", Np * ns * ntot)
+    _need(Qout, "Qout", Np * ns * n)
+    _need(prep["pvgeo"], "pvgeo", Np * (dim * dim + 1) * n)
+    _need(prep["psgeo"], "psgeo", (dim + 1) * Nfp * nf * n)
+    _need(prep["pface"], "pface", nf * n, torch.int32)
+    _need(D, "D", (N + 1) ** 2)
+    if Qin.data_ptr() == Qout.data_ptr():
+        raise ValueError("stage_prepared needs distinct Qin and Qout")
+    rc = lib().canary_stage_prepared(eqn, dim, N, prec, _ptr(rhs), _ptr(Qin), _ptr(Qout),
+                                     _ptr(prep["pvgeo"]), _ptr(prep["psgeo"]),
+                                     _ptr(prep["pface"]), _ptr(D), n, ntot, float(rka),
+                                     float(rkb), float(dt), 1 if advection else 0, _phys(phys),
+                                     _stream(stream))
+    check(rc, "stage_prepared")
+
+
+def lsrk54_prepared(dim, N, Q, Qalt, rhs, prep, D, elems, dt, nsteps, eqn=EULER, phys=None,
+                    prec=FP64, advection=False, graph=False, stream=None):
+    """lsrk54 over the prepared arrays; returns the tensor holding the final
+    state (Q or Qalt)."""
+    n = _nreal(elems)
+    Np, Nfp, nf = (N + 1) ** dim, (N + 1) ** (dim - 1), 2 * dim
+    ns = _NS[(eqn, dim)]
+    for t, nm in ((Q, "Q"), (Qalt, "Qalt"), (rhs, "rhs")):
+        _need(t, nm, Np * ns * n)
+    _need(prep["pvgeo"], "pvgeo", Np * (dim * dim + 1) * n)
+    _need(prep["psgeo"], "psgeo", (dim + 1) * Nfp * nf * n)
+    _need(prep["pface"], "pface", nf * n, torch.int32)
+    alt = C.c_int(0)
+    rc = lib().canary_lsrk54_prepared(eqn, dim, N, prec, int(nsteps), float(dt), _ptr(Q),
+                                      _ptr(Qalt), _ptr(rhs), _ptr(prep["pvgeo"]),
+                                      _ptr(prep["psgeo"]), _ptr(prep["pface"]), _ptr(D), n,
+                                      _phys(phys), 1 if advection else 0, 1 if graph else 0,
+                                      C.byref(alt), _stream(stream))
+    check(rc, "lsrk54_prepared")
     return Qalt if alt.value else Q
 
 

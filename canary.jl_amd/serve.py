@@ -31,7 +31,7 @@ import torch
 from . import problems as P
 from .batch import Batch, Batch32
 from ._lib import FP64
-from .streams import OwnedStream
+from .streams import OwnedStream, reap
 
 
 @dataclass
@@ -166,6 +166,7 @@ class Server:
         setups: optional {rid: host setup (make_setup_host)} prepared
         beforehand (then the host part is not in the latency)."""
         reqs = sorted(requests, key=lambda r: r.arrival)
+        reap()
         pool = cf.ThreadPoolExecutor(max_workers=self.prep_workers)
         futures, ready, active = {}, [], {}
         resident = 0
@@ -262,6 +263,7 @@ class Server:
                 b.check()
         finally:
             pool.shutdown(wait=True)
+            reap()
         self.makespan = clock() - t0
         return reqs
 

@@ -4,13 +4,34 @@ Each one is created by the library (hipStreamCreateWithFlags, non-blocking,
 default priority) and wrapped as a torch ExternalStream, so every runtime
 stream is distinct: streams drawn from torch's shared pool wrap around after
 32 and can alias a stream that still carries other work, which would then
-join a graph capture.  The stream is released when the object is collected.
+join a graph capture.
+
+A collected OwnedStream is not destroyed by its finalizer: CPython runs a
+finalizer in whichever thread triggers the collection -- a host worker
+thread of the server, say -- which could then call hipStreamSynchronize /
+hipStreamDestroy while the main thread captures or launches a graph.  The
+finalizer only queues the stream; reap() destroys the queued ones, and the
+runtimes call it where nothing else is in flight (before a server starts,
+after it drains, when a batch is built).
 """
 import ctypes as C
 
 import torch
 
 from ._lib import check, lib
+
+_graveyard = []  # streams of collected OwnedStream objects (list.append is atomic)
+
+
+def reap():
+    """Destroy the streams queued by collected OwnedStream objects.  Call it
+    from the thread that launches work, with no graph capture in progress."""
+    n = len(_graveyard)
+    ptrs = _graveyard[:n]
+    del _graveyard[:n]
+    for p in ptrs:
+        lib().canary_stream_destroy(C.c_void_p(p))
+    return len(ptrs)
 
 
 class OwnedStream:
@@ -21,6 +42,7 @@ class OwnedStream:
         self.stream = torch.cuda.ExternalStream(self.ptr, device=torch.device(device))
 
     def __del__(self):
-        if getattr(self, "ptr", None):
-            lib().canary_stream_destroy(C.c_void_p(self.ptr))
+        ptr = getattr(self, "ptr", None)
+        if ptr:
             self.ptr = None
+            _graveyard.append(ptr)

@@ -126,13 +126,15 @@ int canary_updatesolution(int eqn, int dim, int N, int prec, void* rhs, void* Q,
  *   read, never written.  rhs is updated in place.
  *   prec: CANARY_FP64 (bit-identical to the three calls above) or
  *   CANARY_FP64_FMA (contracted: fewer FP64 instructions; on the 3D N=4
- *   benchmark both run at the same speed, the stage being bound by HBM). */
+ *   benchmark both run at the same speed, the stage being bound by HBM).
+ *   advection: shallow water only, as in canary_updatesolution (the
+ *   velocity kept through the update, shallow_water.jl:274-297). */
 int canary_stage_fused(int eqn, int dim, int N, int prec, void* rhs, const void* Qin,
                        void* Qout, const void* vgeo, const void* sgeo,
                        const double* D, int64_t nreal, int64_t nelem_total,
                        const int64_t* vmapM, const int64_t* vmapP,
                        const int64_t* elemtobndy, double rka, double rkb, double dt,
-                       const canary_phys* phys, void* stream);
+                       int advection, const canary_phys* phys, void* stream);
 
 /* facerhs! ; updatesolution! of one stage in a single pass, after a
  * separate canary_volumerhs (the opt-in face + update fusion):
@@ -140,23 +142,57 @@ int canary_stage_fused(int eqn, int dim, int N, int prec, void* rhs, const void*
  *   the stage loop (euler3d_set2c.jl:849-854), with Q ping-pong as in
  *   canary_stage_fused (Qin != Qout; ghosts of Qin read, never written).
  *   prec: CANARY_FP64 (bit-identical to canary_facerhs + canary_updatesolution)
- *   or CANARY_FP64_FMA (contracted).  No SWE `advection` mode. */
+ *   or CANARY_FP64_FMA (contracted).  advection as in canary_stage_fused. */
 int canary_facerhs_update(int eqn, int dim, int N, int prec, void* rhs, const void* Qin,
                           void* Qout, const void* vgeo, const void* sgeo, int64_t nreal,
                           int64_t nelem_total, const int64_t* vmapM, const int64_t* vmapP,
                           const int64_t* elemtobndy, double rka, double rkb, double dt,
-                          const canary_phys* phys, void* stream);
+                          int advection, const canary_phys* phys, void* stream);
+
+/* ---- prepared geometry (opt-in) -------------------------------------------
+ * Compact copies of the reference arrays, built once per mesh, that the
+ * fused stage reads instead of vgeo / sgeo / vmapM / vmapP / elemtobndy:
+ * 34,824 bytes per element-stage at 3D N=4 against 38,248 with the
+ * reference arrays.  No reference counterpart (the reference kernels read
+ * the arrays computegeometry and mappings build, euler3d_set2c.jl:101-139,
+ * src/mesh.jl:973-1001); everything left out is checked to be unused by
+ * the stage or a function of what is kept:
+ *   pvgeo double (Np, dim*dim+1, nelem): metric terms and MJ (MJI = 1 ./ MJ
+ *         checked bit for bit and recomputed by the stage; x, y, z unused)
+ *   psgeo double (dim+1, Nfp, 2 dim, nelem): normal and sMJ (vMJI unused)
+ *   pface int32 (2 dim, nelem): -1 for a wall (elemtobndy 1), else
+ *         2 dim * eP + fP: vmapP[:, f, e] = Np eP + fmask[:, fP] + 1 checked
+ *         for every node (the only orientation mappings builds, mesh.jl:991)
+ * vmapM must be the canonical face mask and elemtobndy 0 or 1.  A failed
+ * check is reported by canary_check_errors (CANARY_ERR_ARG for MJI,
+ * CANARY_ERR_VMAP, CANARY_ERR_ORIENTATION, CANARY_ERR_BC); the outputs are
+ * then not to be used.  Ghost elements (nreal..nelem_total-1) may be
+ * neighbours. */
+int canary_prepare_geometry(int dim, int N, int64_t nelem, int64_t nelem_total, const void* vgeo,
+                            const void* sgeo, const int64_t* vmapM, const int64_t* vmapP,
+                            const int64_t* elemtobndy, void* pvgeo, void* psgeo, int32_t* pface,
+                            void* stream);
+/* canary_stage_fused over the prepared arrays: the same arithmetic in the
+ * same order (prec CANARY_FP64: bit-identical to the three reference calls;
+ * CANARY_FP64_FMA: the contracted form), fewer bytes. */
+int canary_stage_prepared(int eqn, int dim, int N, int prec, void* rhs, const void* Qin,
+                          void* Qout, const void* pvgeo, const void* psgeo, const int32_t* pface,
+                          const double* D, int64_t nreal, int64_t nelem_total, double rka,
+                          double rkb, double dt, int advection, const canary_phys* phys,
+                          void* stream);
 
 /* ---- native time loop ---------------------------------------------------
  * lowstorageRK (euler3d_set2c.jl:754-891, euler2d_set2c.jl:639-775,
  * shallow_water.jl:615-747): nsteps LSRK54 steps of one solve through the
  * entry points above, stage loop as the reference's (:819-855).
  *   path CANARY_PATH_KERNELS: volumerhs, facerhs, updatesolution per stage,
- *        Q updated in place (Qalt unused), prec CANARY_FP64 only; `advection`
- *        (shallow water) as in updatesolution;
+ *        Q updated in place (Qalt unused), prec CANARY_FP64 only;
  *   path CANARY_PATH_FUSED: canary_stage_fused, Q and Qalt ping-pong;
  *   path CANARY_PATH_FACE_UPDATE: canary_volumerhs + canary_facerhs_update,
  *        Q and Qalt ping-pong.
+ *   `advection` (shallow water, any path) as in canary_updatesolution.
+ *   The loop exchanges no ghost elements, so nelem_total must equal nreal
+ *   (CANARY_ERR_ARG otherwise); a sharded caller runs its own stage loop.
  *   *result_in_alt: 1 if the final state is in Qalt, else 0 (in Q).
  *   graph != 0 and a non-null stream: two steps are captured once into a
  *   hipGraph and replayed (the call then returns after the steps ran). */
@@ -168,6 +204,21 @@ int canary_lsrk54(int eqn, int dim, int N, int prec, int path, int64_t nsteps, d
                   const double* D, int64_t nreal, int64_t nelem_total, const int64_t* vmapM,
                   const int64_t* vmapP, const int64_t* elemtobndy, const canary_phys* phys,
                   int advection, int graph, int* result_in_alt, void* stream);
+/* canary_lsrk54 over canary_stage_prepared (Q and Qalt ping-pong; no ghost
+ * elements, as canary_lsrk54). */
+int canary_lsrk54_prepared(int eqn, int dim, int N, int prec, int64_t nsteps, double dt, void* Q,
+                           void* Qalt, void* rhs, const void* pvgeo, const void* psgeo,
+                           const int32_t* pface, const double* D, int64_t nreal,
+                           const canary_phys* phys, int advection, int graph, int* result_in_alt,
+                           void* stream);
+/* Makes the table of the correctly rounded pow for one gas exponent
+ * kappa = cp/cv on the current device (once; kappa in [1, 2], up to 16 of
+ * them).  The exact-order Euler entries make it themselves on first use;
+ * a caller that captures their launches into a graph, or runs batches
+ * (whose constants live in device memory), registers each kappa first.
+ * Without a table the kernels give the same bits more slowly.  0, or
+ * CANARY_ERR_UNSUPPORTED when no table was made.  No reference counterpart. */
+int canary_pow_register(double kappa, void* stream);
 /* RKA, RKB, RKC of LSRK54 as the reference's rationals (euler3d_set2c.jl:772-788);
  * any pointer may be NULL. */
 void canary_lsrk54_coefs(double* rka, double* rkb, double* rkc);
@@ -194,9 +245,25 @@ int canary_stage_perturb32(int dim, int N, void* rhs, const void* qin, void* qou
                            const int32_t* nbr, const int8_t* bnd8, const double* D,
                            int64_t nreal, double rka, double rkb, double dt,
                            const canary_phys* phys, void* stream);
-/* Q (FP64, (Np,NS,nelem)) = Q0 + qp */
+/* Q (FP64, (Np,NS,nelem)) = Q0 + qp (both variants) */
 int canary_perturb32_reconstruct(int dim, int N, int64_t nelem, const void* qp, const void* bg,
                                  void* Q, void* stream);
+/* Mixed variant: the same FP32 state q' and FP32 RHS arithmetic, with FP64
+ * kept where the compact arrays round: metric terms, MJ, MJI
+ *   vg64 double (Np,dim*dim+2,nelem),  sg64 double (4,Nfp,NF,nelem) nx ny nz sMJ,
+ * and the LSRK register rhs64 double (Np,NS,nelem) (dq = dq + L(q') and the
+ * update q' + ((rkb*dt) dq) MJI in FP64, q' rounded to FP32 once).  No
+ * reference counterpart (see the precision form above). */
+int canary_perturb_mixed_prepare(int dim, int N, int64_t nelem, int64_t nelem_total,
+                                 const void* vgeo, const void* sgeo, const int64_t* vmapM,
+                                 const int64_t* vmapP, const int64_t* elemtobndy, const void* Q0,
+                                 const void* Q, const canary_phys* phys, void* vg64, void* sg64,
+                                 int32_t* nbr, int8_t* bnd8, void* bg, void* qp, void* stream);
+int canary_stage_perturb_mixed(int dim, int N, void* rhs64, const void* qin, void* qout,
+                               const void* bg, const void* vg64, const void* sg64,
+                               const int32_t* nbr, const int8_t* bnd8, const double* D,
+                               int64_t nreal, double rka, double rkb, double dt,
+                               const canary_phys* phys, void* stream);
 
 /* ---- batched form ---------------------------------------------------------
  * Many independent solves of one equation set / dimension, each with its own
@@ -220,7 +287,7 @@ typedef struct canary_batch_solve {
   double dt;
   canary_phys phys;
   int32_t N;
-  int32_t reserved;
+  int32_t advection;           /* shallow water: keep the velocity (as updatesolution) */
 } canary_batch_solve;
 
 typedef struct canary_batch_item {
@@ -403,6 +470,15 @@ typedef struct canary_vortex {
   double beta, x0, y0, u0, v0, rho_inf, p_inf, gamma, Lx, Ly;
 } canary_vortex;
 
+/* x^y correctly rounded to double (positive finite x, finite y,
+ * |y log x| < 700; outside that, the C library's pow): the pow the
+ * exact-order kernels use for the pressure, here on the host.  The
+ * reference calls Julia's libm pow (euler3d_set2c.jl:175), which no test
+ * fixes to the last bit; this is one function of (x, y) on host and device. */
+double canary_pow_cr(double x, double y);
+/* canary_pow_cr through the kernels' fixed-exponent table of y (built on
+ * the host for y in [1, 2]); the same value, to check that path. */
+double canary_pow_cr_table(double x, double y);
 double canary_hash_jitter(uint64_t seed, double x, double y, double z);
 int canary_ic_bubble(int dim, int N, int64_t nelem, const double* vgeo,
                      const canary_bubble* b, const canary_phys* ph, double* Q);

@@ -34,10 +34,14 @@
  *     y' = z y (1, 1, 1/2, 1/6, 1/24, 1/200) and h-convergence of the
  *     isentropic vortex at order >= N + 1/2 (tests/test_convergence.py);
  *   cfl, L2energysquared: closed forms on uniform states.
- * Parity unpinned at the last ulp: libm pow (glibc here, Julia's libm in
- * the reference) in the pressure; the 1e-12 / 1e-10 tolerances cover it.
+ * pow: the reference calls Julia's libm pow, whose last bit no reference
+ * test fixes.  The pressure pow here is correctly rounded by default
+ * (oracle_pow below), so the device's correctly rounded pow gives the same bits;
+ * mode 0 uses the C library's pow instead (timing of the CPU baseline).
+ * oracle_pow is pinned against mpmath (tests/test_crpow.py).
  */
 #include <math.h>
+#include <quadmath.h>
 #include <stdint.h>
 #include <stdlib.h>
 #ifdef _OPENMP
@@ -49,6 +53,18 @@ typedef struct {
 } ophys;
 
 #define RET_BADBC 4
+
+/* x^y for the pressure.  Mode 1 (default): correctly rounded -- libquadmath's
+ * powq (error about one unit of its 113-bit significand) rounded once to
+ * double, which is the correctly rounded double unless x^y lies within
+ * about 2^-110 (relative) of a midpoint between two doubles.  Mode 0: the C
+ * library's pow (glibc, not correctly rounded; about 40x faster). */
+static int g_pow_mode = 1;
+void oracle_set_pow_mode(int mode) { g_pow_mode = mode; }
+int oracle_get_pow_mode(void) { return g_pow_mode; }
+double oracle_pow(double x, double y) {
+  return g_pow_mode ? (double)powq((__float128)x, (__float128)y) : pow(x, y);
+}
 
 static int ipow(int a, int b) {
   int r = 1;
@@ -95,7 +111,7 @@ int oracle_euler3d_volumerhs(int N, double* rhs, const double* Q, const double* 
             double zetax = vg[n + Np * ZETAX], zetay = vg[n + Np * ZETAY], zetaz = vg[n + Np * ZETAZ];
             double U = q[n + Np * U3], V = q[n + Np * V3], W = q[n + Np * W3];
             double rho = q[n + Np * R3], E = q[n + Np * E3];
-            double P = p0 * pow(R_gas * E / p0, c_p / c_v);
+            double P = p0 * oracle_pow(R_gas * E / p0, c_p / c_v);
             double rinv = 1 / rho;
             double frx = U, fUx = rinv * U * U + P, fVx = rinv * V * U, fWx = rinv * W * U,
                    fEx = E * rinv * U;
@@ -172,7 +188,7 @@ int oracle_euler3d_facerhs(int N, double* rhs, const double* Q, const double* sg
         const double* qP = Q + (size_t)Np * ns * eP;
         double rhoM = qM[vidM + Np * R3], UM = qM[vidM + Np * U3], VM = qM[vidM + Np * V3],
                WM = qM[vidM + Np * W3], EM = qM[vidM + Np * E3];
-        double PM = p0 * pow(R_gas * EM / p0, c_p / c_v);
+        double PM = p0 * oracle_pow(R_gas * EM / p0, c_p / c_v);
         double rhoP, UP, VP, WP, EP, PP;
         if (bc == 0) {
           rhoP = qP[vidP + Np * R3];
@@ -180,7 +196,7 @@ int oracle_euler3d_facerhs(int N, double* rhs, const double* Q, const double* sg
           VP = qP[vidP + Np * V3];
           WP = qP[vidP + Np * W3];
           EP = qP[vidP + Np * E3];
-          PP = p0 * pow(R_gas * EP / p0, c_p / c_v);
+          PP = p0 * oracle_pow(R_gas * EP / p0, c_p / c_v);
         } else {
           double UnM = nxM * UM + nyM * VM + nzM * WM;
           UP = UM - 2 * UnM * nxM;
@@ -269,7 +285,7 @@ int oracle_euler2d_volumerhs(int N, double* rhs, const double* Q, const double* 
           double xix = vg[n + Np * XIX2], xiy = vg[n + Np * XIY2];
           double etax = vg[n + Np * ETAX2], etay = vg[n + Np * ETAY2];
           double U = q[n + Np * U2], V = q[n + Np * V2], rho = q[n + Np * R2], E = q[n + Np * E2];
-          double P = p0 * pow(R_gas * E / p0, c_p / c_v);
+          double P = p0 * oracle_pow(R_gas * E / p0, c_p / c_v);
           double rinv = 1 / rho;
           double frx = U, fUx = rinv * U * U + P, fVx = rinv * V * U, fEx = rinv * U * E;
           double fry = V, fUy = rinv * U * V, fVy = rinv * V * V + P, fEy = rinv * V * E;
@@ -326,14 +342,14 @@ int oracle_euler2d_facerhs(int N, double* rhs, const double* Q, const double* sg
         const double* qP = Q + (size_t)Np * ns * eP;
         double rhoM = qM[vidM + Np * R2], UM = qM[vidM + Np * U2], VM = qM[vidM + Np * V2],
                EM = qM[vidM + Np * E2];
-        double PM = p0 * pow(R_gas * EM / p0, c_p / c_v);
+        double PM = p0 * oracle_pow(R_gas * EM / p0, c_p / c_v);
         double rhoP, UP, VP, EP, PP;
         if (bc == 0) {
           rhoP = qP[vidP + Np * R2];
           UP = qP[vidP + Np * U2];
           VP = qP[vidP + Np * V2];
           EP = qP[vidP + Np * E2];
-          PP = p0 * pow(R_gas * EP / p0, c_p / c_v);
+          PP = p0 * oracle_pow(R_gas * EP / p0, c_p / c_v);
         } else {
           double UnM = nxM * UM + nyM * VM;
           UP = UM - 2 * UnM * nxM;
@@ -599,7 +615,7 @@ double oracle_euler_cfl(int dim, int N, const double* vgeo, const double* Q, int
       if (dim == 3) {
         double rho = q[n + Np * R3], U = q[n + Np * U3], V = q[n + Np * V3], W = q[n + Np * W3];
         double E = q[n + Np * E3];
-        double P = p0 * pow(R_gas * E / p0, c_p / c_v);
+        double P = p0 * oracle_pow(R_gas * E / p0, c_p / c_v);
         double xix = vg[n + Np * XIX], xiy = vg[n + Np * XIY], xiz = vg[n + Np * XIZ];
         double etax = vg[n + Np * ETAX], etay = vg[n + Np * ETAY], etaz = vg[n + Np * ETAZ];
         double zetax = vg[n + Np * ZETAX], zetay = vg[n + Np * ZETAY], zetaz = vg[n + Np * ZETAZ];
@@ -612,7 +628,7 @@ double oracle_euler_cfl(int dim, int N, const double* vgeo, const double* Q, int
         loc = 2 * rho / m;
       } else {
         double rho = q[n + Np * R2], U = q[n + Np * U2], V = q[n + Np * V2], E = q[n + Np * E2];
-        double P = p0 * pow(R_gas * E / p0, c_p / c_v);
+        double P = p0 * oracle_pow(R_gas * E / p0, c_p / c_v);
         double xix = vg[n + Np * XIX2], xiy = vg[n + Np * XIY2];
         double etax = vg[n + Np * ETAX2], etay = vg[n + Np * ETAY2];
         double c = rho * sqrt(gamma * P / rho);

@@ -46,8 +46,27 @@ def lib():
         L.oracle_euler_cfl.argtypes = [INT, INT, V, V, I64, PP]
         L.oracle_euler_cfl.restype = D
         L.oracle_lsrk54_coefs.argtypes = [V, V, V]
+        L.oracle_set_pow_mode.argtypes = [INT]
+        L.oracle_get_pow_mode.restype = INT
+        L.oracle_pow.argtypes = [D, D]
+        L.oracle_pow.restype = D
         _lib = L
     return _lib
+
+
+def set_pow_mode(mode):
+    """1 (default): the pressure's pow correctly rounded (quad precision
+    rounded once); 0: the C library's pow (CPU-baseline timing)."""
+    lib().oracle_set_pow_mode(int(mode))
+
+
+def pow_mode():
+    return lib().oracle_get_pow_mode()
+
+
+def opow(x, y):
+    """The oracle's pow in its current mode."""
+    return lib().oracle_pow(float(x), float(y))
 
 
 def _p(a):

@@ -54,18 +54,18 @@ def test_device_entry_points_reject_bad_arguments_before_launch():
     assert lib.canary_updatesolution(0, 2, 3, L.FP64_FMA, None, None, None, 1, 0.0, 0.0, 0.0, 0, None) == L.ERR_UNSUPPORTED
     for eqn, dim in ((0, 3), (0, 2), (1, 2)):
         assert lib.canary_stage_fused(eqn, dim, 4, L.FP64_FMA, None, None, None, None, None, None, 0, 0,
-                                      None, None, None, 0.0, 0.0, 0.0, ctypes.byref(p), None) == 0
+                                      None, None, None, 0.0, 0.0, 0.0, 0, ctypes.byref(p), None) == 0
         assert lib.canary_stage_fused(eqn, dim, 4, L.FP64_FMA, None, None, None, None, None, None, 1, 1,
-                                      None, None, None, 0.0, 0.0, 0.0, ctypes.byref(p), None) == L.ERR_ARG
+                                      None, None, None, 0.0, 0.0, 0.0, 0, ctypes.byref(p), None) == L.ERR_ARG
     assert lib.canary_stage_fused(0, 3, 8, L.FP64_FMA, None, None, None, None, None, None, 1, 1,
-                                  None, None, None, 0.0, 0.0, 0.0, ctypes.byref(p), None) == L.ERR_UNSUPPORTED
+                                  None, None, None, 0.0, 0.0, 0.0, 0, ctypes.byref(p), None) == L.ERR_UNSUPPORTED
     for prec in (L.FP64, L.FP64_FMA):
         assert lib.canary_facerhs_update(0, 3, 4, prec, None, None, None, None, None, 0, 0, None,
-                                         None, None, 0.0, 0.0, 0.0, ctypes.byref(p), None) == 0
+                                         None, None, 0.0, 0.0, 0.0, 0, ctypes.byref(p), None) == 0
         assert lib.canary_facerhs_update(1, 2, 4, prec, None, None, None, None, None, 3, 3, None,
-                                         None, None, 0.0, 0.0, 0.0, ctypes.byref(p), None) == L.ERR_ARG
+                                         None, None, 0.0, 0.0, 0.0, 0, ctypes.byref(p), None) == L.ERR_ARG
     assert lib.canary_facerhs_update(0, 3, 4, L.FP32, None, None, None, None, None, 3, 3, None,
-                                     None, None, 0.0, 0.0, 0.0, ctypes.byref(p), None) == L.ERR_UNSUPPORTED
+                                     None, None, 0.0, 0.0, 0.0, 0, ctypes.byref(p), None) == L.ERR_UNSUPPORTED
 
 
 def test_lsrk54_tables_match_the_host_ones():
@@ -138,3 +138,23 @@ def test_no_kernel_is_instantiated_in_two_units():
     assert len(owner) > 100
     shared = {s: u for s, u in owner.items() if len(u) > 1}
     assert not shared, list(shared.items())[:3]
+
+
+def test_lsrk54_refuses_ghost_elements_before_launch():
+    """The library loop exchanges no ghosts: nelem_total != nreal is refused
+    on the host, whatever the path (nothing is launched or dereferenced)."""
+    lib = L.lib()
+    p = L.default_phys()
+    alt = ctypes.c_int(-1)
+    dummy = ctypes.c_void_p(0x1000)
+    for path in (L.PATH_KERNELS, L.PATH_FUSED, L.PATH_FACE_UPDATE):
+        rc = lib.canary_lsrk54(0, 3, 4, L.FP64, path, 1, 0.1, dummy, ctypes.c_void_p(0x2000), dummy,
+                               dummy, dummy, dummy, 8, 9, None, dummy, dummy, ctypes.byref(p), 0, 0,
+                               ctypes.byref(alt), None)
+        assert rc == L.ERR_ARG, path
+    # the shallow-water velocity freeze is refused for the Euler sets
+    rc = lib.canary_lsrk54(0, 3, 4, L.FP64, L.PATH_FUSED, 1, 0.1, dummy, ctypes.c_void_p(0x2000),
+                           dummy, dummy, dummy, dummy, 8, 8, None, dummy, dummy, ctypes.byref(p), 1,
+                           0, ctypes.byref(alt), None)
+    assert rc == L.ERR_ARG
+

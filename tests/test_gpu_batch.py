@@ -232,3 +232,85 @@ def test_bad_table_entry_is_reported(cuda):
     assert ei.value.code == ERR_ARG
     b.check()  # cleared
     assert max_ulp(b.state(ids[0]), before) == 0
+
+
+def test_swe_ensemble_of_64(cuda):
+    """The 64-solve shallow-water ensemble at its own size (periodic, Ne^2
+    quads with Ne uniform in 8..64, N in {3,4,5,7}, still depth 10, two
+    seeded Gaussian bumps of height 0.01-0.1 and width 0.05-0.2 domain
+    widths; 102,836 elements, 16.3 M DOF), elements per workgroup
+    autotuned per N, 3 LSRK54 steps:
+      * FP64 (reference evaluation order), eager and replaying step graphs:
+        every solve bit-identical to its own single-solve fused run;
+      * one solve of each N bit-identical to the oracle's lowstorageRK;
+      * contracted FP64 within 1e-13 of FP64, FP32 within 1e-5 (relative
+        L2 of h, Hu, Hv per solve)."""
+    ens = P.swe_ensemble(n=64, seed=42)
+    assert len(ens) == 64 and sorted({s.N for s, _ in ens}) == [3, 4, 5, 7]
+    assert all(8 <= int(round(np.sqrt(s.nelem))) <= 64 for s, _ in ens)
+    print("ensemble: %d elements, %d DOF" % (sum(s.nelem for s, _ in ens),
+                                            sum(s.ndof() for s, _ in ens)), flush=True)
+    states = {}
+    for name, prec, graph in (("fp64", FP64, False), ("fp64 graph", FP64, True),
+                              ("fp64_fma", FP64_FMA, False), ("fp32", FP32, False)):
+        b = Batch(SWE, 2, prec)
+        ids = [b.add(s, dt) for s, dt in ens]
+        b.finalize(autotune=True)
+        b.run(3, graph=graph)
+        b.check()
+        states[name] = [b.state(i) for i in ids]
+        print("%s: epw %s" % (name, dict(sorted(b.epw.items()))), flush=True)
+        del b
+        torch.cuda.empty_cache()
+    for k, (s, dt) in enumerate(ens):
+        assert max_ulp(states["fp64 graph"][k], states["fp64"][k]) == 0, k
+        want = single_run(s, dt, 3)
+        assert max_ulp(states["fp64"][k], want) == 0, "solve %d (N=%d, %d elements)" % (
+            k, s.N, s.nelem)
+        ref = states["fp64"][k][:, :3]
+        assert rel_l2(states["fp64_fma"][k][:, :3], ref) <= 1e-13, k
+        e32 = rel_l2(states["fp32"][k][:, :3], ref)
+        assert e32 <= 1e-5, (k, e32)
+    for N in (3, 4, 5, 7):
+        k = min((k for k, (s, _) in enumerate(ens) if s.N == N), key=lambda k: ens[k][0].nelem)
+        s, dt = ens[k]
+        Q = np.array(s.Q, order="F", copy=True)
+        assert O.lsrk(s, Q, np.zeros_like(Q, order="F"), dt, 3, nthreads=8) == 0
+        assert max_ulp(states["fp64"][k], Q) == 0, "oracle, solve %d (N=%d)" % (k, N)
+
+
+def test_batch_swe_advection_mode(cuda):
+    """Solves flagged with the shallow-water advection mode (velocity kept
+    through each update) beside unflagged ones in one batch: FP64 bit-identical
+    to single-solve fused runs with the same flag, FP32 within 1e-5."""
+    from states import smooth_swe_state
+    mix = swe_mix()[:4]
+    for s, _ in mix:
+        s.Q = np.asfortranarray(smooth_swe_state(s, seed=s.N))
+    flags = [True, False, True, True]
+    A, B, _ = P.rk_coefs()
+    out = {}
+    for prec in (FP64, FP32):
+        b = Batch(SWE, 2, prec)
+        ids = [b.add(s, dt, advection=f) for (s, dt), f in zip(mix, flags)]
+        b.finalize()
+        b.run(3, graph=False)
+        b.check()
+        out[prec] = [b.state(i) for i in ids]
+    for k, ((s, dt), f) in enumerate(zip(mix, flags)):
+        ds = DeviceSetup(s)
+        q, q2, rhs = dev(s.Q), dev(s.Q), dev(np.zeros_like(s.Q))
+        for _ in range(3):
+            for st in range(5):
+                K.stage_fused(2, s.N, rhs, q, q2, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM,
+                              ds.vmapP, ds.bndy, A[(st + 1) % 5], B[st], dt, eqn=SWE, phys=s.phys,
+                              advection=f)
+                q, q2 = q2, q
+        K.check_errors()
+        want = host(q, s.Q.shape)
+        assert max_ulp(out[FP64][k], want) == 0, k
+        assert rel_l2(out[FP32][k][:, :3], want[:, :3]) <= 1e-5, k
+        if f:
+            Qo = np.array(s.Q, order="F", copy=True)
+            assert O.lsrk(s, Qo, np.zeros_like(Qo, order="F"), dt, 3, advection=True) == 0
+            assert max_ulp(want, Qo) == 0, k

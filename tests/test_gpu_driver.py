@@ -88,15 +88,36 @@ def test_native_loop_matches_oracle_lowstoragerk(cuda):
         e = rel_l2(got, Qo)
         print("native loop (prec %d) vs oracle after %d steps: %.3e" % (prec, nsteps, e), flush=True)
         assert e <= 1e-10
+        if prec == FP64:
+            assert max_ulp(got, Qo) == 0  # exact order, correctly rounded pow
 
 
-def test_native_loop_swe_advection(cuda):
+@pytest.mark.parametrize("path,prec", [(PATH_KERNELS, FP64), (PATH_FUSED, FP64),
+                                       (PATH_FACE_UPDATE, FP64), (PATH_FUSED, FP64_FMA),
+                                       (PATH_FACE_UPDATE, FP64_FMA)])
+def test_native_loop_swe_advection(cuda, path, prec):
+    """The shallow-water example's advection mode (velocity kept through
+    every update, shallow_water.jl:274-297) on every path of the library
+    loop: bit-identical to the oracle in the exact order, within 1e-12
+    contracted."""
     s = P.swe2d(Ne=5, N=3, bumps=((0.4, 0.5, 0.05, 0.1),))
+    s.Q = np.asfortranarray(smooth_swe_state(s, seed=3))  # a moving state
     ds = DeviceSetup(s)
     Qo, ro = np.asfortranarray(s.Q.copy()), s.zeros_state()
     assert O.lsrk(s, Qo, ro, 1e-4, 7, advection=True) == 0
     q, rhs = dev(s.Q), dev(s.zeros_state())
+    qalt = None if path == PATH_KERNELS else torch.empty_like(q)
     out = K.lsrk54(2, 3, q, rhs, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM, ds.vmapP, ds.bndy,
-                   1e-4, 7, eqn=1, phys=s.phys, path=PATH_KERNELS, advection=True)
+                   1e-4, 7, eqn=1, phys=s.phys, path=path, prec=prec, Qalt=qalt, advection=True)
     K.check_errors()
-    assert max_ulp(host(out, s.Q.shape), Qo) == 0
+    got = host(out, s.Q.shape)
+    if prec == FP64:
+        assert max_ulp(got, Qo) == 0
+    else:
+        assert rel_l2(got, Qo) <= 1e-12
+    # the velocity moved with the state only through the last rounding
+    u0 = s.Q[:, 0, :] / (s.Q[:, 2, :] + s.Q[:, 3, :])
+    u1 = got[:, 0, :] / (got[:, 2, :] + got[:, 3, :])
+    assert np.max(np.abs(u0)) > 0.1
+    assert np.max(np.abs(u1 - u0)) <= 1e-13 * np.max(np.abs(u0))
+    assert not np.array_equal(got[:, 2, :], s.Q[:, 2, :])  # h did move

@@ -2,17 +2,16 @@
 N = 4: 32,768 elements, 20.48 M DOF) on the GPU, checked through what does
 not depend on size:
 
-  * one RHS evaluation against the C oracle (one evaluation takes about a
-    second on the host): on a generic state exact-order and contracted
-    within 1e-12 relative L2 (they differ only through pow and, contracted,
-    the fused multiply-adds), bit-identical on a state whose pressure needs
-    no pow; on the bubble itself the difference stays at the last bits of
-    the terms the RHS sums;
+  * one RHS evaluation against the C oracle (a few seconds on the host):
+    the exact-order form bit-identical on the bubble and on a generic state;
+    the contracted one within 1e-12 relative L2 on the generic state and at
+    the last bits of the terms the RHS sums on the bubble;
   * the paths agree bit for bit: fused exact-order stage vs the three
     kernels, vmapM = NULL vs the map, graph replay vs eager launches, the
     library loop vs the Python stage loop;
-  * two LSRK54 steps against the oracle's lowstorageRK (exact order), and
-    100 steps in both arithmetic forms within 1e-10 (the target's bar);
+  * two LSRK54 steps against the oracle's lowstorageRK (exact order:
+    bit-identical), and 100 steps in both arithmetic forms within 1e-10
+    (the target's bar);
   * mass (sum of MJ rho) and rho-theta are conserved: walls, no sources for
     them, and discrete operators whose contributions cancel in pairs.
 """
@@ -54,29 +53,36 @@ def gpu_rhs(s, ds, Q, prec, mapM="map"):
 
 
 def test_fullsize_rhs_against_oracle(big):
+    """One RHS on the benchmark's own seeded bubble and on a generic state:
+    the exact-order form bit-identical to the oracle (so within the task's
+    1e-12 relative L2 per RHS); the contracted form within 1e-12 on the
+    generic state.  On the bubble, which sits in near-hydrostatic balance,
+    the RHS is about 2000x smaller than the volume and face terms it sums,
+    so the contracted form's last-bit differences (fused multiply-adds,
+    series pow) come out near 1e-10 relative to the RHS: its figure is
+    printed and held below 2e-10, and against the size of the terms it
+    stays at the last bits."""
     s, ds, _ = big
     print("full-size RHS vs oracle", flush=True)
     res = {}
     for name, Q in (("bubble", s.Q), ("smooth", np.asfortranarray(smooth_euler_state(s, seed=6)))):
-        want = O.rhs_eval(s, Q, nthreads=8)
+        want = O.rhs_eval(s, Q, nthreads=16)
         vol = s.zeros_state()
-        O.volumerhs(s, vol, Q, nthreads=8)
+        O.volumerhs(s, vol, Q, nthreads=16)
         for prec in (FP64, FP64_FMA):
             got = gpu_rhs(s, ds, Q, prec)
-            res[(name, prec)] = (rel_l2(got, want), np.linalg.norm(got - want) / np.linalg.norm(vol))
-    print(res, flush=True)
+            res[(name, prec)] = (rel_l2(got, want), np.linalg.norm(got - want) / np.linalg.norm(vol),
+                                 max_ulp(got, want))
+    print({"%s %s" % (k[0], "fp64" if k[1] == FP64 else "fp64_fma"):
+           "rel_l2 %.3e, vs volume term %.3e, max ulp %d" % v for k, v in res.items()}, flush=True)
     Qp = np.asfortranarray(pow_free_euler_state(s, seed=5))
-    assert max_ulp(gpu_rhs(s, ds, Qp, FP64), O.rhs_eval(s, Qp, nthreads=8)) == 0
-    for prec in (FP64, FP64_FMA):
-        # a generic state: the per-RHS bar
-        assert res[("smooth", prec)][0] <= 1e-12, res
-        # the benchmark's bubble sits in near-hydrostatic balance: its RHS is
-        # about 2000x smaller than the volume and face terms it is the sum of,
-        # so a last-bit difference of pow (device vs host libm) shows up
-        # 2000x larger relative to the RHS (measured 2.9e-11 exact order,
-        # 8.2e-11 contracted); against the size of the terms it stays at the
-        # last bits
-        assert res[("bubble", prec)][1] <= 1e-15, res
+    assert max_ulp(gpu_rhs(s, ds, Qp, FP64), O.rhs_eval(s, Qp, nthreads=16)) == 0
+    for name in ("bubble", "smooth"):
+        assert res[(name, FP64)][2] == 0, res  # bit-identical
+        assert res[(name, FP64)][0] <= 1e-12, res
+    assert res[("smooth", FP64_FMA)][0] <= 1e-12, res
+    assert res[("bubble", FP64_FMA)][0] <= 2e-10, res
+    assert res[("bubble", FP64_FMA)][1] <= 1e-15, res
 
 
 def test_fullsize_paths_agree_bitwise(big):
@@ -129,13 +135,14 @@ def test_fullsize_two_steps_against_oracle(big):
     s, ds, dt = big
     print("full-size 2 LSRK54 steps vs oracle", flush=True)
     Qo, ro = s.Q.copy(order="F"), s.zeros_state()
-    O.lsrk(s, Qo, ro, dt, 2, nthreads=8)
+    O.lsrk(s, Qo, ro, dt, 2, nthreads=16)
     q, q2, r = dev(s.Q), torch.empty(s.Q.size, dtype=torch.float64, device="cuda"), \
         dev(s.zeros_state())
     fin = K.lsrk54(3, s.N, q, r, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM, ds.vmapP, ds.bndy,
                    dt, 2, phys=s.phys, path=PATH_FUSED, prec=FP64, Qalt=q2)
     K.check_errors()
-    assert rel_l2(host(fin, s.Q.shape), Qo) <= 1e-12
+    assert max_ulp(host(fin, s.Q.shape), Qo) == 0  # exact order: the oracle's bits
+    assert max_ulp(host(r, s.Q.shape), ro) == 0
     # the state moved: the increment itself is pinned, not just Q
     dQ = host(fin, s.Q.shape) - s.Q
     assert np.linalg.norm(dQ) > 0
@@ -145,11 +152,20 @@ def test_fullsize_two_steps_against_oracle(big):
 def test_fullsize_100_steps_against_oracle(big):
     """The target's own bar at the target's own size: 100 LSRK54 steps of the
     32^3 N=4 bubble, the library loop against the oracle's lowstorageRK
-    (about 20 s on 16 host threads), both arithmetic forms within 1e-10."""
+    (about 20 s on 16 host threads), both arithmetic forms within 1e-10.
+    The oracle uses the C library's pow here (its correctly rounded pow
+    would take some 20 minutes for 100 steps at this size); the exact-order
+    form's bit-identity with the correctly rounded oracle is shown per RHS
+    and over two steps at this size and over 40-100 steps on smaller
+    meshes."""
     s, ds, dt = big
     print("full-size 100 LSRK54 steps vs oracle", flush=True)
     Qo, ro = s.Q.copy(order="F"), s.zeros_state()
-    O.lsrk(s, Qo, ro, dt, 100, nthreads=16)
+    O.set_pow_mode(0)
+    try:
+        O.lsrk(s, Qo, ro, dt, 100, nthreads=16)
+    finally:
+        O.set_pow_mode(1)
     for prec in (FP64, FP64_FMA):
         q, q2, r = dev(s.Q), torch.empty(s.Q.size, dtype=torch.float64, device="cuda"), \
             dev(s.zeros_state())
@@ -193,9 +209,11 @@ def test_fullsize_conservation(big):
 
 def test_precision_form_at_48cubed(cuda):
     """The precision form's benchmark size (48^3, N = 4, 69.12 M DOF): after
-    20 LSRK54 steps its full state (FP64 background + FP32 perturbation) is
-    within 1e-5 relative L2 of the FP64 fused path, per field, and mass
-    sum(MJ rho) moves by less than the FP32 rounding of the perturbation."""
+    100 LSRK54 steps the full state (FP64 background + FP32 perturbation) of
+    both variants -- compact (FP32 geometry and LSRK register) and mixed
+    (FP64 geometry, normals and register) -- is within 1e-5 relative L2 of
+    the FP64 fused path, per field, and mass sum(MJ rho) moves by less than
+    the FP32 rounding of the perturbation."""
     from canary_amd.perturb import Perturb32
     print("precision form 48^3 N=4", flush=True)
     s = P.bubble3d(Ne=48, N=4, gravity=9.81, dtheta=2.0, jitter=1e-3, seed=1)
@@ -205,19 +223,23 @@ def test_precision_form_at_48cubed(cuda):
     q, q2, r = dev(s.Q), torch.empty(s.Q.size, dtype=torch.float64, device="cuda"), \
         dev(s.zeros_state())
     fin = K.lsrk54(3, s.N, q, r, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM, ds.vmapP, ds.bndy,
-                   dt, 20, phys=s.phys, path=PATH_FUSED, prec=FP64, Qalt=q2)
+                   dt, 100, phys=s.phys, path=PATH_FUSED, prec=FP64, Qalt=q2)
     K.check_errors()
     want = host(fin, s.Q.shape)
     del q, q2, r, fin, ds
     torch.cuda.empty_cache()
-    pf = Perturb32(s, Q0, s.Q)
-    pf.run(dt, 20)
-    pf.check()
-    got = pf.full_state().cpu().numpy().reshape(s.Q.shape, order="F")
-    errs = {k: np.linalg.norm(got[:, k] - want[:, k]) / np.linalg.norm(want[:, k])
-            for k in range(5)}
-    print("relative L2 per field:", errs, flush=True)
-    assert max(errs.values()) <= 1e-5
     MJ = s.vgeo[:, 9, :]
-    m0, m1 = float((MJ * s.Q[:, 3, :]).sum()), float((MJ * got[:, 3, :]).sum())
-    assert abs(m1 - m0) <= 1e-9 * abs(m0)
+    m0 = float((MJ * s.Q[:, 3, :]).sum())
+    for mixed in (False, True):
+        pf = Perturb32(s, Q0, s.Q, mixed=mixed)
+        pf.run(dt, 100)
+        pf.check()
+        got = pf.full_state().cpu().numpy().reshape(s.Q.shape, order="F")
+        del pf
+        torch.cuda.empty_cache()
+        errs = {k: np.linalg.norm(got[:, k] - want[:, k]) / np.linalg.norm(want[:, k])
+                for k in range(5)}
+        print("mixed=%s: relative L2 per field after 100 steps:" % mixed, errs, flush=True)
+        assert max(errs.values()) <= 1e-5
+        m1 = float((MJ * got[:, 3, :]).sum())
+        assert abs(m1 - m0) <= 1e-9 * abs(m0)

@@ -36,13 +36,16 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
+@pytest.mark.parametrize("mixed", [False, True])
 @pytest.mark.parametrize("N", [3, 4])
-def test_perturb32_tracks_fp64(cuda, N):
+def test_perturb32_tracks_fp64(cuda, N, mixed):
+    """Both variants: compact (FP32 geometry and LSRK register) and mixed
+    (FP64 geometry, normals and register)."""
     s = P.bubble3d(Ne=5, N=N, dtheta=2.0, jitter=1e-3, seed=4)
     Q0 = P.bubble_state(s, dtheta=0.0, jitter=0.0)
     dt = P.cfl(s, s.Q) / s.N ** np.sqrt(2)
     ds = DeviceSetup(s)
-    pf = Perturb32(s, Q0, s.Q)
+    pf = Perturb32(s, Q0, s.Q, mixed=mixed)
     # round trip: reconstruct(prepare(Q)) == Q up to FP32 rounding of q'
     Qr = host(pf.full_state(), s.Q.shape)
     assert np.max(np.abs(Qr - s.Q)) <= 1e-6 * np.max(np.abs(s.Q - Q0)) + 1e-300
@@ -55,7 +58,8 @@ def test_perturb32_tracks_fp64(cuda, N):
         want = fp64_run(s, ds, s.Q, dt, n)
         e = {nm: rel(got[:, k, :], want[:, k, :]) for k, nm in ((3, "rho"), (0, "rhou"), (2, "rhow"),
                                                                (4, "rhotheta"))}
-        print("N=%d steps=%d" % (N, n), {k: "%.2e" % v for k, v in e.items()}, flush=True)
+        print("N=%d mixed=%s steps=%d" % (N, mixed, n), {k: "%.2e" % v for k, v in e.items()},
+              flush=True)
         assert all(v <= 1e-5 for v in e.values()), e
 
 
@@ -73,12 +77,13 @@ def state_errors(got, want, dim):
     return {nm: rel(got[:, k, :], want[:, k, :]) for k, nm in enumerate(names)}
 
 
+@pytest.mark.parametrize("mixed", [False, True])
 @pytest.mark.parametrize("N", [4, 7])
-def test_perturb32_vortex2d_tracks_fp64(cuda, N):
+def test_perturb32_vortex2d_tracks_fp64(cuda, N, mixed):
     s = P.vortex2d(Ne=8, N=N)
     dt = P.euler_courant(s, s.Q)
     ds = DeviceSetup(s)
-    pf = Perturb32(s, s.Q0, s.Q)
+    pf = Perturb32(s, s.Q0, s.Q, mixed=mixed)
     done = 0
     for n in (1, 10, 100):
         pf.run(dt, n - done)
@@ -87,7 +92,8 @@ def test_perturb32_vortex2d_tracks_fp64(cuda, N):
         got = host(pf.full_state(), s.Q.shape)
         want = fp64_run(s, ds, s.Q, dt, n)
         e = state_errors(got, want, 2)
-        print("vortex N=%d steps=%d" % (N, n), {k: "%.2e" % v for k, v in e.items()}, flush=True)
+        print("vortex N=%d mixed=%s steps=%d" % (N, mixed, n),
+              {k: "%.2e" % v for k, v in e.items()}, flush=True)
         assert all(v <= 1e-5 for v in e.values()), e
 
 

@@ -116,3 +116,44 @@ def test_device_preparation_matches_host(cuda, kind):
     Q, Q0 = host(s.Qd, h.Q.shape), host(s.Q0d, h.Q.shape)
     assert rel_l2(Q, h.Q) <= 1e-15 and rel_l2(Q0, h.Q0) <= 1e-15
     assert max_ulp(np.array([dt]), np.array([dth])) <= 8
+
+
+def test_stream_of_64_under_a_binding_budget(cuda):
+    """The 64-request Poisson stream at its own sizes (mean 200 requests/s,
+    seed 2024; 2D isentropic vortices of 16^2-64^2 quads with N 3..7 and 3D
+    rising bubbles of 8^3-24^3 hexes with N 3..5, 50-500 LSRK54 steps
+    each), FP32 perturbation form, under an HBM budget of a quarter of the
+    stream's summed footprint:
+      * the resident footprint never exceeds the budget, and requests wait
+        for room (admission by footprint binds);
+      * every result is bit-identical to the same request served alone;
+      * every result is within 1e-5 relative L2 of the request's plain FP64
+        solve."""
+    reqs = poisson_stream(n=64, rate=200.0, seed=2024)
+    hosts = {r.rid: make_setup_host(r) for r in reqs}
+    probe = Server("fp32")
+    fps = {rid: probe.footprint(s) for rid, s in hosts.items()}
+    del probe
+    budget = max(max(fps.values()), sum(fps.values()) // 4)
+    srv = Server("fp32", hbm_budget=budget)
+    out = srv.serve(reqs, setups=hosts)
+    st = summary(out, srv.makespan)
+    waited = [r for r in out if r.admitted - r.prepared > 5e-3]
+    print({k: round(v, 4) if isinstance(v, float) else v for k, v in st.items()},
+          "budget %.0f MB of %.0f MB, %d requests waited for room"
+          % (budget / 1e6, sum(fps.values()) / 1e6, len(waited)), flush=True)
+    assert all(r.done is not None and r.latency > 0 for r in out)
+    assert max(res for _, _, res in srv.trace) <= budget
+    assert len(waited) >= 8
+    for r in out:
+        solo = Request(rid=r.rid, kind=r.kind, Ne=r.Ne, N=r.N, nsteps=r.nsteps, arrival=0.0,
+                       seed=r.seed)
+        got = Server("fp32", epw=srv.epw).serve([solo], setups=hosts)[0]
+        assert max_ulp(got.result, r.result) == 0, "request %d (%s N=%d)" % (r.rid, r.kind, r.N)
+        s, dt = make_setup_device(r, hosts[r.rid])
+        assert dt == r.dt
+        ref = fp64_alone(s, dt, r.nsteps)
+        s.dev = s.Qd = s.Q0d = None
+        err = rel_l2(r.result, ref)
+        assert err <= 1e-5, (r.rid, r.kind, r.N, err)
+    print("max FP32 vs FP64 error checked for %d requests" % len(out), flush=True)

@@ -13,7 +13,7 @@ from canary_amd._lib import FP64, FP64_FMA
 from oracle import oracle as O
 
 from gpu_util import DeviceSetup, dev, host
-from states import rel_l2
+from states import max_ulp, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -43,12 +43,14 @@ def test_config1_vortex_matches_oracle(cuda, fused):
     s = P.vortex2d(Ne=10, N=4)
     dt = 0.5 * P.cfl(s, s.Q) / s.N ** np.sqrt(2)
     want = np.array(s.Q, order="F", copy=True)
-    O.lsrk(s, want, np.zeros_like(want, order="F"), dt, 100)
+    O.lsrk(s, want, np.zeros_like(want, order="F"), dt, 100, nthreads=8)
     got = gpu_run(s, dt, 100, fused)
     e = rel_l2(got, want)
     kind = {True: "fused", False: "3 kernels", "fma": "fused, contracted"}[fused]
     print("config 1 GPU (%s) vs oracle after 100 steps: %.3e" % (kind, e), flush=True)
     assert e <= 1e-10
+    if fused != "fma":
+        assert max_ulp(got, want) == 0  # exact order, correctly rounded pow
 
 
 def test_vortex_h_convergence_gpu(cuda):

@@ -45,3 +45,27 @@ def test_summary_percentiles():
 def test_graph_replay_in_serving_loop_is_off():
     with pytest.raises(NotImplementedError):
         Server("fp32", graph=True)
+
+
+def test_collected_streams_are_queued_not_destroyed():
+    """A finalizer runs in whichever thread triggers the collection, so an
+    OwnedStream's finalizer makes no HIP call: it queues the stream for
+    reap(), which the runtimes call from their launch thread."""
+    import threading
+    import canary_amd.streams as S
+    before = list(S._graveyard)
+    objs = []
+    for p in (0x1234, 0x5678):
+        o = object.__new__(S.OwnedStream)
+        o.ptr = p
+        objs.append(o)
+    del o
+
+    def drop():  # collected on another thread
+        objs.clear()
+
+    t = threading.Thread(target=drop)
+    t.start()
+    t.join()
+    assert sorted(S._graveyard[len(before):]) == [0x1234, 0x5678]
+    del S._graveyard[len(before):]  # never destroyed: they are not real streams
