@@ -49,6 +49,15 @@ def algorithmic_bytes(dim, N, nstate=None, word=8):
     return word * (4 * ns * Np + nvol * Np + nface * Nfp * nsurf) + 2 * nface * Nfp * 8 + nface * 8
 
 
+def prepared_bytes(dim, N, word=8):
+    """Bytes one element moves per stage over the prepared arrays: Q r+w,
+    dQ r+w, the metric terms + MJ, normals + sMJ, one int32 face code per
+    face (MJI, vmapM, vmapP and elemtobndy are not read)."""
+    Nq = N + 1
+    Np, Nfp, nface, ns = Nq ** dim, Nq ** (dim - 1), 2 * dim, dim + 2
+    return word * (4 * ns * Np + (dim * dim + 1) * Np + nface * Nfp * (dim + 1)) + 4 * nface
+
+
 def swe_bytes(N):
     """Shallow-water bytes per element-stage by the same rule: Q read with b
     (4 fields), written without it (3), dQ r+w (3 fields), 4 metric terms +
@@ -202,7 +211,22 @@ def main():
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also run one device-to-device copy of this many GiB (untimed), the "
                          "known-size reference for HBM byte counters")
+    ap.add_argument("--calib", action="store_true",
+                    help="also run two more known-size kernels for the byte counters (untimed): "
+                         "a 1 GiB read-only sum and a gather of 2^20 random 128-byte rows")
+    ap.add_argument("--summarize-trace", metavar="DIR", default=None,
+                    help="print the kernel table of a rocprofv3 --kernel-trace csv run of this "
+                         "script (with the same --steps / --ne / --N) and exit")
+    ap.add_argument("--summarize-pmc", metavar="DIR", default=None,
+                    help="print the per-kernel counter means of a rocprofv3 --pmc csv run of "
+                         "this script and exit")
     args = ap.parse_args()
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace, args.steps, args.ne, args.N)
+        return
+    if args.summarize_pmc:
+        summarize_pmc(args.summarize_pmc, args.ne, args.N)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU kernel; "
               "multi-GPU sharding is described, not run (got --gpus %d)" % args.gpus,
@@ -280,6 +304,24 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    if args.calib:
+        # known-size references for the byte counters: a streaming copy made
+        # by an elementwise kernel of its own (1 GiB negated into another:
+        # 1 GiB read, 1 GiB written), a read-only stream (1 GiB summed:
+        # 1 GiB read) and a gather of whole 128-byte rows in random order
+        # (2^20 rows of a 128 MiB array, each once, plus the 8 MiB index:
+        # 136 MiB read, 128 MiB written)
+        x = torch.ones((1 << 30) // 8, dtype=torch.float64, device=dev)
+        y = torch.empty_like(x)
+        torch.neg(x, out=y)
+        float(x.sum())
+        del x, y
+        rows = torch.ones(1 << 20, 16, dtype=torch.float64, device=dev)
+        perm = torch.randperm(1 << 20, device=dev)
+        g = rows.index_select(0, perm)
+        torch.cuda.synchronize()
+        del rows, perm, g
+
     # measured copy bandwidth of the box (1 GiB device-to-device, read + write)
     cw = (1 << 30) // 8
     csrc = torch.ones(cw, dtype=torch.float64, device=dev)
@@ -289,13 +331,23 @@ def main():
     copy_gbs = 2 * (1 << 30) * 10 / t_copy / 1e9
     del csrc, cdst
 
-    # the measured value: the library's LSRK54 time loop (canary_lsrk54, the
-    # reference's lowstorageRK) over the contracted fused stage, launched from
-    # C on a stream of its own: `warmup` untimed steps, then exactly `steps`
-    # timed steps
+    # the measured value: the library's LSRK54 time loop (the reference's
+    # lowstorageRK) over the contracted fused stage reading the prepared
+    # arrays (canary_prepare_geometry, built once from the reference arrays
+    # before the timed region), launched from C on a stream of its own:
+    # `warmup` untimed steps, then exactly `steps` timed steps
     from canary_amd.streams import OwnedStream
     from canary_amd._lib import PATH_FUSED
     own = OwnedStream(dev)
+    prep = K.prepare_geometry(3, s.N, vgeo, sgeo, vM, vP, bnd, s.nelem)
+    K.check_errors()
+
+    def pnative(qa, qb, r, nsteps, prec=FP64_FMA, graph=False):
+        """nsteps steps over the prepared arrays from qa; returns (state, spare)."""
+        with torch.cuda.stream(own.stream):
+            out = K.lsrk54_prepared(3, s.N, qa, qb, r, prep, D, s.nelem, dt, nsteps,
+                                    phys=s.phys, prec=prec, graph=graph)
+        return (out, qb) if out is qa else (out, qa)
 
     def native(qa, qb, r, nsteps, mapM=vM, graph=False):
         """nsteps steps from qa; returns (state, spare buffer)."""
@@ -315,8 +367,20 @@ def main():
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / 1e3, time.perf_counter() - t_wall
 
+    stages = 5 * args.steps
+    dof = s.nelem * s.Np * 5
+    bytes_launch = algorithmic_bytes(3, s.N) * s.nelem
+    own_bytes = prepared_bytes(3, s.N)
+    extra = {}
+    # The loops below run in this order: the reference-array loop, the
+    # exact-order prepared loop, then the measured one.  The first second of
+    # sustained load on a box runs through a clock / power transient (stage
+    # times rise by up to a third for some 20 launches, then settle); the
+    # measured loop's own warmup and timed steps come after it.
+    # The same loop over the reference arrays themselves (canary_lsrk54,
+    # contracted fused stage): the call surface the Julia shim binds
     q, q2, rhs = Q0.clone(), torch.empty_like(Q0), torch.zeros_like(Q0)
-    torch.cuda.synchronize()  # made on the current stream, used on the loop's own
+    torch.cuda.synchronize()
     q, q2 = native(q, q2, rhs, args.warmup)
     K.check_errors()
     holder = {}
@@ -324,19 +388,53 @@ def main():
     def go():
         holder["q"] = native(q, q2, rhs, args.steps)
 
-    t_fused, wall_fused = timed_own(go)
+    t_ref, _ = timed_own(go)
     K.check_errors()
     qf = holder["q"][0]
-    if not bool(torch.isfinite(qf).all()):
+    extra["reference_arrays_loop_gdof_stage_s"] = round(dof * stages / t_ref / 1e9, 3)
+    extra["reference_arrays_loop_us_per_stage"] = round(t_ref / stages * 1e6, 2)
+    extra["reference_arrays_loop_frac"] = round(bytes_launch / (t_ref / stages) / 1e9
+                                                / HBM_PEAK_GBS, 4)
+    # the exact-order loop over the prepared arrays (bit-identical to the
+    # oracle), from the same initial state
+    qx_, qx2_, rx_ = Q0.clone(), torch.empty_like(Q0), torch.zeros_like(Q0)
+    torch.cuda.synchronize()
+    qx_, qx2_ = pnative(qx_, qx2_, rx_, args.warmup, prec=FP64)
+    hold_x = {}
+
+    def go_x():
+        hold_x["q"] = pnative(qx_, qx2_, rx_, args.steps, prec=FP64)
+
+    t_px, _ = timed_own(go_x)
+    K.check_errors()
+    extra["prepared_exact_order_gdof_stage_s"] = round(dof * stages / t_px / 1e9, 3)
+    extra["prepared_exact_order_us_per_stage"] = round(t_px / stages * 1e6, 2)
+    extra["prepared_exact_order_frac"] = round(bytes_launch / (t_px / stages) / 1e9
+                                               / HBM_PEAK_GBS, 4)
+    del qx_, qx2_, rx_, hold_x
+    # the measured value
+    qp_, qp2_, rp_ = Q0.clone(), torch.empty_like(Q0), torch.zeros_like(Q0)
+    torch.cuda.synchronize()  # made on the current stream, used on the loop's own
+    qp_, qp2_ = pnative(qp_, qp2_, rp_, args.warmup)
+    K.check_errors()
+    hold_p = {}
+
+    def go_p():
+        hold_p["q"] = pnative(qp_, qp2_, rp_, args.steps)
+
+    t_fused, wall_fused = timed_own(go_p)
+    K.check_errors()
+    qpf = hold_p["q"][0]
+    if not bool(torch.isfinite(qpf).all()):
         raise SystemExit("non-finite state after the timed steps")
-    stages = 5 * args.steps
-    dof = s.nelem * s.Np * 5
     value = dof * stages / t_fused / 1e9
     launch_s = t_fused / stages
-    bytes_launch = algorithmic_bytes(3, s.N) * s.nelem
     achieved = bytes_launch / launch_s / 1e9
 
-    extra = {}
+    extra["prepared_vs_reference_arrays_rel_l2"] = float(
+        "%.3e" % (torch.linalg.vector_norm(qpf - qf) / torch.linalg.vector_norm(qf)).item())
+    del qp_, qp2_, rp_
+
     # the same loop replaying two captured steps per hipGraph launch (capture
     # and instantiation inside the timed call)
     qg, qg2, rg = Q0.clone(), torch.empty_like(Q0), torch.zeros_like(Q0)
@@ -384,25 +482,6 @@ def main():
     extra["fma_vs_exact_rel_l2_after_%d_steps" % (args.warmup + args.steps)] = float(
         "%.3e" % (torch.linalg.vector_norm(qf - qx) / torch.linalg.vector_norm(qx)).item())
     del qe, qe2, re, hold_e, qx
-    # the library loop over the prepared arrays (canary_prepare_geometry, once),
-    # both arithmetic forms, from the same initial state
-    prep = K.prepare_geometry(3, s.N, vgeo, sgeo, vM, vP, bnd, s.nelem)
-    K.check_errors()
-    for name, prec in (("prepared_exact_order", FP64), ("prepared_contracted", FP64_FMA)):
-        qa_, qb_, r_ = Q0.clone(), torch.empty_like(Q0), torch.zeros_like(Q0)
-        torch.cuda.synchronize()
-
-        def prun(n, qa_=qa_, qb_=qb_, r_=r_, prec=prec):
-            with torch.cuda.stream(own.stream):
-                return K.lsrk54_prepared(3, s.N, qa_, qb_, r_, prep, D, s.nelem, dt, n,
-                                         phys=s.phys, prec=prec)
-        prun(args.warmup)
-        t_p, _ = timed_own(lambda: prun(args.steps))
-        K.check_errors()
-        extra[name + "_gdof_stage_s"] = round(dof * stages / t_p / 1e9, 3)
-        extra[name + "_us_per_stage"] = round(t_p / stages * 1e6, 2)
-        del qa_, qb_, r_
-    del prep
     # the measured loop with vmapM = NULL (M-side ids from the canonical face
     # mask, the map not read); same bits as with the map
     qc, qc2, rc_ = Q0.clone(), torch.empty_like(Q0), torch.zeros_like(Q0)
@@ -459,7 +538,7 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(s, args.cpu_sample_elems)
     if args.precision_ne > 0:
-        del q, q2, rhs, vgeo, sgeo, vM, vP, bnd, Q0, holder, qf
+        del q, q2, rhs, vgeo, sgeo, vM, vP, bnd, Q0, holder, qf, prep, hold_p, qpf
         torch.cuda.empty_cache()
         extra["precision_form"] = precision_form(args.precision_ne, s.N, args.precision_steps)
     torch.cuda.empty_cache()
@@ -472,10 +551,10 @@ def main():
         extra["served_form"] = form_child("served", ["--served-requests", str(args.served_requests),
                                                      "--served-rate", str(args.served_rate)])
 
-    traffic = measured_traffic("euler3d_N%d_%d" % (s.N, args.ne))
+    traffic = measured_traffic("prepared_euler3d_N%d_%d" % (s.N, args.ne))
     line = {
-        "metric": "GDOF-stage/s 3D Euler N=4 FP64 (LSRK54 stages, fused single-pass kernel, "
-                  "multiply-adds fused, library time loop)",
+        "metric": "GDOF-stage/s 3D Euler N=4 FP64 (LSRK54 stages, fused single-pass kernel over "
+                  "prepared geometry, multiply-adds fused, library time loop)",
         "value": round(value, 3),
         "unit": "GDOF-stage/s",
         "n_gpus": 1,
@@ -495,10 +574,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "stage_kernel<Euler3DFma,5,1> (fused volume+face+update)",
+                     "kernel": "stage_prep_kernel<Euler3DFma,5,1,1> (fused volume+face+update "
+                               "over the prepared arrays)",
                      "bytes_per_element_stage": algorithmic_bytes(3, s.N),
-                     "bytes_per_element_stage_without_vmapM": algorithmic_bytes(3, s.N)
-                     - 6 * (s.N + 1) ** 2 * 8,
+                     "bytes_per_element_stage_own": own_bytes,
+                     "frac_on_own_bytes": round(own_bytes * s.nelem / launch_s / 1e9
+                                                / HBM_PEAK_GBS, 4),
                      "avg_launch_us": round(launch_s * 1e6, 2),
                      "measured_copy_gbs": round(copy_gbs, 1),
                      "frac_of_measured_copy": round(achieved / copy_gbs, 4)},
@@ -736,7 +817,8 @@ def batched_form(nsolves, steps, seed=42):
     uniform in 8..64, N in {3,4,5,7}, seeded bumps) advanced together:
     FP64 (reference evaluation order), contracted FP64 and FP32 batches
     (autotuned elements per workgroup, one graph per step) against the same
-    solves run one after another with the single-solve fused stage."""
+    solves run one after another with the single-solve fused stage; each
+    precision also with every stage one launch over all orders."""
     import numpy as np
     import torch
     import canary_amd.kernels as K
@@ -772,6 +854,30 @@ def batched_form(nsolves, steps, seed=42):
         print("batched form %s: %.3f GDOF-stage/s" % (name, out[name]["gdof_stage_s"]),
               file=sys.stderr, flush=True)
         del b
+        torch.cuda.empty_cache()
+        # the same batch with every stage ONE launch over all orders
+        # (canary_batch_stage_all), same number of steps, step graphs
+        b = Batch(SWE, 2, prec)
+        ids = [b.add(s, dt) for s, dt in ens]
+        b.finalize(single_launch=True)
+        b.run(2)
+        b.check()
+        t = timed(lambda: b.run(steps))
+        b.check()
+        got = [b.state(i) for i in ids]
+        if prec == FP64_FMA:
+            same = max(float(np.linalg.norm(a - c) / np.linalg.norm(c))
+                       for a, c in zip(got, states[name]))
+        else:
+            same = all(np.array_equal(a, c) for a, c in zip(got, states[name]))
+        out[name]["single_launch"] = {
+            "gdof_stage_s": round(dof * 5 * steps / t / 1e9, 3),
+            "ms_per_step": round(t / steps * 1e3, 4),
+            "epw": {str(k): v for k, v in sorted(b.epw.items())},
+            ("rel_l2_vs_per_order" if prec == FP64_FMA else "same_bits_as_per_order"): same}
+        print("batched form %s, single launch: %.3f GDOF-stage/s"
+              % (name, out[name]["single_launch"]["gdof_stage_s"]), file=sys.stderr, flush=True)
+        del b, got
         torch.cuda.empty_cache()
     errs = [np.linalg.norm(a[:, :3] - c[:, :3]) / np.linalg.norm(c[:, :3])
             for a, c in zip(states["fp32"], states["fp64"])]
@@ -829,10 +935,17 @@ def served_form(nreq, rate, seed=2024):
     # each new order inside the stream
     Server("fp32").serve(poisson_stream(n=8, rate=1000.0, seed=1, vortex_ne=(2, 3),
                                         bubble_ne=(2, 2), steps=(2, 3)))
+    # admission binds: an HBM budget of a quarter of the stream's summed
+    # footprint (but at least its largest request), so requests wait for room
+    from canary_amd.serve import make_setup_host
+    probe = Server("fp32")
+    fps = [probe.footprint(make_setup_host(r)) for r in reqs]
+    del probe
+    budget = max(max(fps), sum(fps) // 4)
     # work tiers: requests of at most 5e8 DOF-steps in a batch of their own,
     # the larger ones advanced every 8th iteration while small ones are
     # resident (swept on this stream: 2e8 / 5e8 x 4 / 8)
-    srv = Server("fp32", keep_setups=True, tier_work=5e8, tier_ratio=8)
+    srv = Server("fp32", keep_setups=True, tier_work=5e8, tier_ratio=8, hbm_budget=budget)
     t0 = time.time()
     out_reqs = srv.serve(reqs)
     wall = time.time() - t0
@@ -840,7 +953,7 @@ def served_form(nreq, rate, seed=2024):
     print("served %d requests in %.1f s" % (nreq, wall), file=sys.stderr, flush=True)
     # the same stream in lock step (one batch per dimension); results must
     # not change
-    srv_l = Server("fp32")
+    srv_l = Server("fp32", hbm_budget=budget)
     out_l = srv_l.serve(poisson_stream(n=nreq, rate=rate, seed=seed))
     st_l = summary(out_l, srv_l.makespan)
     by_rid = {r.rid: r.result for r in out_l}
@@ -888,15 +1001,121 @@ def served_form(nreq, rate, seed=2024):
     return {"workload": "%d requests, Poisson arrivals (mean %g/s, seed %d): %d 2D isentropic "
                         "vortices (16^2-64^2 quads, N 3..7), %d 3D rising bubbles (8^3-24^3 "
                         "hexes, N 3..5), 50-500 LSRK54 steps each; FP32 perturbation form, "
-                        "work tiers"
+                        "work tiers, HBM budget a quarter of the summed footprint"
                         % (nreq, rate, seed, nk["vortex2d"], nreq - nk["vortex2d"]),
             **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()},
             "work_tiers": {"tier_work_dof_steps": 5e8, "tier_ratio": 8},
+            "hbm_budget_bytes": budget, "stream_footprint_bytes": sum(fps),
+            "max_resident_bytes": max(res for _, _, res in srv.trace),
+            "requests_that_waited_for_room": sum(1 for r in out_reqs
+                                                 if r.admitted - r.prepared > 5e-3),
             "lock_step": lock,
             "sequential_fp64_gpu_s": round(t_seq, 4),
             "sequential_fp64_gdof_stage_s": round(st["dof_stages"] / t_seq / 1e9, 3),
             "fp32_rel_l2_vs_fp64": {"max": float("%.3e" % max(errs)),
                                     "median": float("%.3e" % float(np.median(errs)))}}
+
+
+HEADLINE_KERNEL = "stage_prep_kernel<canary::Euler3DFma, 5, 1, 1>"
+
+
+def _csv_rows(d, pattern):
+    import csv
+    import glob
+    out = []
+    for f in sorted(glob.glob(os.path.join(d, "**", pattern), recursive=True)):
+        with open(f) as fh:
+            out += list(csv.DictReader(fh))
+    return out
+
+
+def _short(name, n=96):
+    return name if len(name) <= n else name[:n - 3] + "..."
+
+
+def summarize_trace(d, steps, ne, N):
+    """Per-kernel table of a `rocprofv3 --kernel-trace` run of this script
+    (csv output under d), and the headline kernel's average over its last
+    5 * steps dispatches -- the timed launches of the measured loop, which
+    is the only loop that launches that kernel at that grid -- with the
+    roofline fraction recomputed from it."""
+    groups = {}
+    for r in _csv_rows(d, "*kernel_trace.csv"):
+        key = (r["Kernel_Name"], int(r.get("Grid_Size", r.get("Grid_Size_X"))),
+               int(r.get("Workgroup_Size", r.get("Workgroup_Size_X"))))
+        t = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        groups.setdefault(key, []).append((int(r["Dispatch_Id"]), t))
+    print("%-96s %9s %4s %6s %9s %9s %9s" % ("kernel", "grid", "wg", "calls", "avg_us", "min_us",
+                                             "max_us"))
+    for (name, grid, wg), v in sorted(groups.items(), key=lambda kv: -sum(t for _, t in kv[1]))[:30]:
+        ts = [t for _, t in v]
+        print("%-96s %9d %4d %6d %9.2f %9.2f %9.2f" % (_short(name), grid, wg, len(ts),
+                                                       sum(ts) / len(ts), min(ts), max(ts)))
+    nelem = ne ** 3
+    grid = nelem * 128  # one 128-lane workgroup per element
+    sel = [v for (name, g, _), v in groups.items() if HEADLINE_KERNEL in name and g == grid]
+    if len(sel) != 1:
+        print("headline kernel %s at grid %d: not found once" % (HEADLINE_KERNEL, grid))
+        return
+    ts = [t for _, t in sorted(sel[0])][-5 * steps:]
+    avg = sum(ts) / len(ts)
+    nbytes = algorithmic_bytes(3, N) * nelem
+    print("timed launches of %s (last %d of %d dispatches at grid %d): avg %.2f us, "
+          "min %.2f, max %.2f" % (HEADLINE_KERNEL, len(ts), len(sel[0]), grid, avg, min(ts),
+                                  max(ts)))
+    print("recomputed: achieved %.1f GB/s, frac %.4f (%d B/element-stage x %d elements), "
+          "5 x avg = %.4f ms per step"
+          % (nbytes / avg / 1e3, nbytes / avg / 1e3 / HBM_PEAK_GBS, algorithmic_bytes(3, N), nelem,
+             5 * avg / 1e3))
+
+
+# request sizes of the TCC->memory counters (bytes per request)
+_READ_SIZES = {"TCC_EA0_RDREQ_32B_sum": 32, "TCC_EA0_RDREQ_64B_sum": 64,
+               "TCC_EA0_RDREQ_128B_sum": 128}
+
+
+def summarize_pmc(d, ne, N):
+    """Per-kernel means of every counter of a `rocprofv3 --pmc` run of this
+    script (csv output under d).  With the request counters
+    TCC_EA0_RDREQ_{32B,64B,128B}_sum the HBM read bytes are their sizes
+    summed; with TCC_EA0_WRREQ_sum and TCC_EA0_WRREQ_64B_sum the write bytes
+    are 64 per 64-byte request and 32 per other one.  The known-size kernels
+    of --calib are listed with their expected bytes."""
+    vals = {}
+    for r in _csv_rows(d, "*counter_collection.csv"):
+        key = (r["Kernel_Name"], int(r["Grid_Size"]))
+        vals.setdefault(key, {}).setdefault(r["Counter_Name"], []).append(
+            float(r["Counter_Value"]))
+    rows = []
+    for (name, grid), cs in vals.items():
+        mean = {c: sum(v) / len(v) for c, v in cs.items()}
+        calls = max(len(v) for v in cs.values())
+        rd = wr = None
+        if all(c in mean for c in _READ_SIZES):
+            rd = sum(mean[c] * b for c, b in _READ_SIZES.items())
+        if "TCC_EA0_WRREQ_sum" in mean and "TCC_EA0_WRREQ_64B_sum" in mean:
+            w64 = mean["TCC_EA0_WRREQ_64B_sum"]
+            wr = 64 * w64 + 32 * (mean["TCC_EA0_WRREQ_sum"] - w64)
+        rows.append((name, grid, calls, mean, rd, wr))
+    rows.sort(key=lambda r: -r[2] * max([v for v in r[3].values()] + [0]))
+    counters = sorted({c for r in rows for c in r[3]})
+    print("%-80s %9s %6s %s %14s %14s" % ("kernel", "grid", "calls",
+                                         " ".join("%16s" % c[-16:] for c in counters),
+                                         "read_MB", "write_MB"))
+    for name, grid, calls, mean, rd, wr in rows[:30]:
+        print("%-80s %9d %6d %s %14s %14s" % (
+            _short(name, 80), grid, calls,
+            " ".join("%16.0f" % mean.get(c, float("nan")) for c in counters),
+            "-" if rd is None else "%.3f" % (rd / 1e6), "-" if wr is None else "%.3f" % (wr / 1e6)))
+    nelem = ne ** 3
+    head = [r for r in rows if HEADLINE_KERNEL in r[0] and r[1] == nelem * 128]
+    if head:
+        name, grid, calls, mean, rd, wr = head[0]
+        own = prepared_bytes(3, N) * nelem
+        print("headline %s: %d dispatches; read %s, write %s bytes per launch; own algorithmic "
+              "bytes %d (read %d, write %d)" % (HEADLINE_KERNEL, calls, rd, wr, own,
+                                               own - 2 * 5 * (N + 1) ** 3 * 8 * nelem,
+                                               2 * 5 * (N + 1) ** 3 * 8 * nelem))
 
 
 def measured_traffic(tag):

@@ -76,7 +76,7 @@ class BatchClass(C.Structure):
                 ("epw", C.c_int32)]
 
 
-BATCH_DROPIN, BATCH_P32 = 0, 1
+BATCH_DROPIN, BATCH_P32, BATCH_DROPIN_ALL = 0, 1, 2
 
 
 _lib = None
@@ -121,6 +121,8 @@ _SIGS = {
     "canary_p32_batch_stage": (INT, [INT, INT, INT, P, P, I64, INT, D, D, P]),
     "canary_batch_variant": (INT, [INT, INT, INT, INT, INT, C.POINTER(INT), C.POINTER(INT)]),
     "canary_batch_stage": (INT, [INT, INT, INT, INT, INT, P, P, I64, INT, D, D, P]),
+    "canary_batch_all_variant": (INT, [INT, INT, INT, INT, C.POINTER(INT), C.POINTER(INT)]),
+    "canary_batch_stage_all": (INT, [INT, INT, INT, P, P, I64, INT, D, D, P]),
     "canary_device_count": (INT, []),
     "canary_stream_create": (INT, [C.POINTER(P)]),
     "canary_stream_destroy": (INT, [P]),

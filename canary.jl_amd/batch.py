@@ -14,7 +14,9 @@ Solves may differ in N, mesh, element count, dt and physical constants.  A
 stage launches one kernel per (N, elements-per-workgroup) class over that
 class's slice of the device work table; inside a captured step the classes
 run on forked streams, so a step is a graph of 5 stages x (classes in
-parallel).  All solves share one stage parity: buffer [parity] holds the
+parallel).  finalize(single_launch=True) (drop-in form) makes each stage
+one launch over the whole table instead, the kernel switching on each
+entry's N.  All solves share one stage parity: buffer [parity] holds the
 current state and a stage writes buffer [1 - parity].
 """
 import ctypes as C
@@ -89,9 +91,11 @@ class _BatchBase:
     batched drop-in and precision forms."""
 
     _rowtype = None
+    _has_single_launch = False
 
     def __init__(self, device="cuda"):
         self.device = torch.device(device)
+        self.single_launch = False
         self.solves = {}          # id -> _Solve
         self._next_id = 0
         self.parity = 0
@@ -139,14 +143,31 @@ class _BatchBase:
         return Q
 
     # --------------------------------------------------------- work table
-    def finalize(self, epw=None, autotune=False):
+    def finalize(self, epw=None, autotune=False, single_launch=None):
         """Build the solve table and the work table (entries grouped by N).
         epw: {N: elements per workgroup} overrides; autotune: time every
-        candidate per N on this batch and keep the fastest."""
+        candidate per N on this batch and keep the fastest.  single_launch
+        (drop-in form): every stage is ONE launch over the entries of all
+        orders (canary_batch_stage_all, each order at its fullest 256-lane
+        variant) instead of one launch per order; None keeps the current
+        choice."""
         if not self.solves:
             raise ValueError("empty batch")
+        if single_launch is not None:
+            if single_launch and not self._has_single_launch:
+                raise ValueError("this batched form has no single-launch kernel")
+            if bool(single_launch) != self.single_launch:
+                self.single_launch = bool(single_launch)
+                self.dirty = True
         epw = dict(epw or {})
         Ns = sorted({v.N for v in self.solves.values()})
+        if self.single_launch:
+            if epw or autotune:
+                raise ValueError("single_launch fixes elements per workgroup per order")
+            for N in Ns:
+                self.epw[N] = self._all_variant(N)
+            self._build_tables()
+            return self
         for N in Ns:
             cands = [e for e, _ in self._variants(N)]
             if N in epw:
@@ -200,6 +221,8 @@ class _BatchBase:
             self.groups.append((N, self.epw[N], off, it.size))
             off += it.size
         items = np.concatenate(parts)
+        if self.single_launch:  # one class of every entry (N = epw = 0 marks it)
+            self.groups = [(0, 0, 0, off)]
         self._items = torch.from_numpy(items.view(np.uint8).copy()).to(self.device)
         self._graphs = {}
         self._version += 1
@@ -361,6 +384,7 @@ class Batch(_BatchBase):
     FP32 copies of Q, rhs, vgeo and sgeo (prec = FP32, shallow water)."""
 
     _rowtype = _lib.BatchSolve
+    _has_single_launch = True
 
     def __init__(self, eqn, dim, prec=FP64, device="cuda"):
         if (eqn, dim) not in _NS:
@@ -375,11 +399,23 @@ class Batch(_BatchBase):
         return variants(self.eqn, self.dim, self.prec, N)
 
     def _launch_raw(self, N, epw, table, items_ptr, count, parity, rka, rkb, stream):
+        if N == 0:  # the single-launch class
+            return lib().canary_batch_stage_all(self.eqn, self.dim, self.prec, _ptr(table),
+                                                items_ptr, count, parity, rka, rkb, stream)
         return lib().canary_batch_stage(self.eqn, self.dim, self.prec, N, epw, _ptr(table),
                                         items_ptr, count, parity, rka, rkb, stream)
 
+    def _all_variant(self, N):
+        e, t = C.c_int(), C.c_int()
+        rc = lib().canary_batch_all_variant(self.eqn, self.dim, self.prec, N, C.byref(e),
+                                            C.byref(t))
+        if rc:
+            raise ValueError("order N=%d has no single-launch variant (error %d)" % (N, rc))
+        return e.value
+
     def _run_kind(self):
-        return _lib.BATCH_DROPIN, self.eqn, self.dim, self.prec
+        kind = _lib.BATCH_DROPIN_ALL if self.single_launch else _lib.BATCH_DROPIN
+        return kind, self.eqn, self.dim, self.prec
 
     def _word(self):
         return 4 if self.prec == FP32 else 8

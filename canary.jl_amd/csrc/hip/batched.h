@@ -100,6 +100,106 @@ __global__ __launch_bounds__((BCfg<Phys, NQ, EPW>::NT)) void batched_kernel(
       sv.vmapM, sv.vmapP, sv.elemtobndy, rka, rkb, sv.dt, p, sv.advection);
 }
 
+// ---------------------------------------------- every class in one launch
+// All entries of a batch whose order fits 256 lanes with candidate 0 (the
+// most elements that fit 256 lanes: every 2D order, 3D N <= 5) run in ONE
+// launch per stage: 256-lane workgroups, one work-table entry each, a
+// switch on the entry's N into that order's stage body.
+template <class Phys, int NQ>
+constexpr bool fits_all() {
+  return epw_candidate<Phys::DIM, NQ, 0>() * ipow(NQ, Phys::DIM) <= MAXT;
+}
+
+template <class Phys, int NQ>
+constexpr int all_lds_nq() {
+  if constexpr (fits_all<Phys, NQ>())
+    return FusedCfg<Phys, NQ, epw_candidate<Phys::DIM, NQ, 0>(), false>::LDS;
+  else
+    return 0;
+}
+
+template <class Phys>
+constexpr int all_lds() {
+  constexpr int a[7] = {all_lds_nq<Phys, 2>(), all_lds_nq<Phys, 3>(), all_lds_nq<Phys, 4>(),
+                        all_lds_nq<Phys, 5>(), all_lds_nq<Phys, 6>(), all_lds_nq<Phys, 7>(),
+                        all_lds_nq<Phys, 8>()};
+  int m = 16;
+  for (int i = 0; i < 7; ++i) m = a[i] > m ? a[i] : m;
+  return m;
+}
+
+template <class Phys, int NQ>
+__device__ __forceinline__ void all_item(char* lds, const canary_batch_solve& sv,
+                                         const canary_batch_item& it, int parity, double rka,
+                                         double rkb, const PhysParams& p) {
+  if constexpr (fits_all<Phys, NQ>()) {
+    using Real = typename Phys::Real;
+    constexpr int E = epw_candidate<Phys::DIM, NQ, 0>();
+    stage_body<Phys, NQ, E, MAXT, 1, true>(
+        lds, (Real*)sv.rhs, (const Real*)sv.Q[parity], (Real*)sv.Q[1 - parity],
+        (const Real*)sv.vgeo, (const Real*)sv.sgeo, sv.D, it.e0, sv.nreal, sv.nelem_total,
+        sv.vmapM, sv.vmapP, sv.elemtobndy, rka, rkb, sv.dt, p, sv.advection);
+  }
+}
+
+template <class Phys>
+__global__ __launch_bounds__(MAXT) void batched_all_kernel(
+    const canary_batch_solve* __restrict__ solves, const canary_batch_item* __restrict__ items,
+    int64_t nitems, int64_t chunk, int parity, double rka, double rkb,
+    const crpow::KappaTable* __restrict__ ktabs, int nktab) {
+  __shared__ __attribute__((aligned(16))) char lds[all_lds<Phys>()];
+  const int64_t w = stage_block(blockIdx.x, chunk, parity);
+  if (w >= nitems) return;
+  const canary_batch_item it = items[w];
+  const canary_batch_solve& sv = solves[it.solve];
+  int want = 0;
+  switch (it.N) {
+    case 1: want = fits_all<Phys, 2>() ? epw_candidate<Phys::DIM, 2, 0>() : 0; break;
+    case 2: want = fits_all<Phys, 3>() ? epw_candidate<Phys::DIM, 3, 0>() : 0; break;
+    case 3: want = fits_all<Phys, 4>() ? epw_candidate<Phys::DIM, 4, 0>() : 0; break;
+    case 4: want = fits_all<Phys, 5>() ? epw_candidate<Phys::DIM, 5, 0>() : 0; break;
+    case 5: want = fits_all<Phys, 6>() ? epw_candidate<Phys::DIM, 6, 0>() : 0; break;
+    case 6: want = fits_all<Phys, 7>() ? epw_candidate<Phys::DIM, 7, 0>() : 0; break;
+    case 7: want = fits_all<Phys, 8>() ? epw_candidate<Phys::DIM, 8, 0>() : 0; break;
+  }
+  if (want == 0 || it.epw != want || sv.N != it.N) {
+    if (threadIdx.x == 0) record_error(1 /*CANARY_ERR_ARG*/, it.solve, it.e0 + 1, it.N);
+    return;
+  }
+  if (it.e0 < 0 || it.e0 >= sv.nreal) {
+    if (threadIdx.x == 0) record_error(1 /*CANARY_ERR_ARG*/, it.solve, it.e0 + 1, sv.nreal);
+    return;
+  }
+  PhysParams p = params_of(sv.phys);
+  if (Phys::HAS_P)
+    for (int i = 0; i < nktab; ++i)
+      if (ktabs[i].kappa == p.kappa) {
+        p.ktab = ktabs + i;
+        break;
+      }
+  switch (it.N) {
+    case 1: all_item<Phys, 2>(lds, sv, it, parity, rka, rkb, p); break;
+    case 2: all_item<Phys, 3>(lds, sv, it, parity, rka, rkb, p); break;
+    case 3: all_item<Phys, 4>(lds, sv, it, parity, rka, rkb, p); break;
+    case 4: all_item<Phys, 5>(lds, sv, it, parity, rka, rkb, p); break;
+    case 5: all_item<Phys, 6>(lds, sv, it, parity, rka, rkb, p); break;
+    case 6: all_item<Phys, 7>(lds, sv, it, parity, rka, rkb, p); break;
+    case 7: all_item<Phys, 8>(lds, sv, it, parity, rka, rkb, p); break;
+  }
+}
+
+template <class Phys>
+int run_all(const canary_batch_solve* solves, const canary_batch_item* items, int64_t nitems,
+            int parity, double rka, double rkb, hipStream_t st) {
+  const int64_t chunk = (nitems + 7) / 8;
+  if (chunk > ((int64_t)1 << 27)) return CANARY_ERR_ARG;
+  int nktab = 0;
+  const crpow::KappaTable* ktabs = Phys::HAS_P ? kappa_tables(&nktab) : nullptr;
+  batched_all_kernel<Phys><<<dim3((unsigned)(8 * chunk)), dim3(MAXT), 0, st>>>(
+      solves, items, nitems, chunk, parity, rka, rkb, ktabs, nktab);
+  return hipGetLastError() == hipSuccess ? 0 : CANARY_ERR_LAUNCH;
+}
+
 template <class Phys, int NQ, int WHICH>
 int run_which(int* epw, int* threads, const canary_batch_solve* solves,
               const canary_batch_item* items, int64_t nitems, int parity, double rka, double rkb,

@@ -7,6 +7,9 @@ extern "C" int canary_k_batch_fma(int eqn, int dim, int N, int which, int want, 
                                   int* threads, const canary_batch_solve* solves,
                                   const canary_batch_item* items, int64_t nitems, int parity,
                                   double rka, double rkb, void* stream);
+extern "C" int canary_k_batch_fma_all(int eqn, int dim, const canary_batch_solve* solves,
+                                      const canary_batch_item* items, int64_t nitems, int parity,
+                                      double rka, double rkb, void* stream);
 
 namespace canary {
 namespace batch {
@@ -35,8 +38,51 @@ int route(int eqn, int dim, int prec, int N, int which, int want, int* epw, int*
   return CANARY_ERR_UNSUPPORTED;
 }
 
+int route_all(int eqn, int dim, int prec, const canary_batch_solve* solves,
+              const canary_batch_item* items, int64_t nitems, int parity, double rka, double rkb,
+              hipStream_t st) {
+  if (prec == CANARY_FP64) {
+    if (eqn == CANARY_EULER && dim == 3)
+      return run_all<Euler3D>(solves, items, nitems, parity, rka, rkb, st);
+    if (eqn == CANARY_EULER && dim == 2)
+      return run_all<Euler2D>(solves, items, nitems, parity, rka, rkb, st);
+    if (eqn == CANARY_SWE && dim == 2)
+      return run_all<SWE2D>(solves, items, nitems, parity, rka, rkb, st);
+  } else if (prec == CANARY_FP32) {
+    if (eqn == CANARY_SWE && dim == 2)
+      return run_all<SWE2DF32>(solves, items, nitems, parity, rka, rkb, st);
+  } else if (prec == CANARY_FP64_FMA) {
+    return canary_k_batch_fma_all(eqn, dim, solves, items, nitems, parity, rka, rkb, st);
+  }
+  return CANARY_ERR_UNSUPPORTED;
+}
+
 }  // namespace batch
 }  // namespace canary
+
+extern "C" int canary_batch_all_variant(int eqn, int dim, int prec, int N, int* epw,
+                                        int* threads) {
+  if (!epw || !threads) return CANARY_ERR_ARG;
+  int e = 0, t = 0;
+  const int rc = canary::batch::route(eqn, dim, prec, N, 0, 0, &e, &t, nullptr, nullptr, 0, 0,
+                                      0.0, 0.0, nullptr);
+  if (rc) return rc;
+  if (t > canary::batch::MAXT) return CANARY_ERR_UNSUPPORTED;
+  *epw = e;
+  *threads = canary::batch::MAXT;
+  return 0;
+}
+
+extern "C" int canary_batch_stage_all(int eqn, int dim, int prec,
+                                      const canary_batch_solve* solves,
+                                      const canary_batch_item* items, int64_t nitems, int parity,
+                                      double rka, double rkb, void* stream) {
+  if (nitems < 0 || (parity != 0 && parity != 1)) return CANARY_ERR_ARG;
+  if (nitems == 0) return 0;
+  if (!solves || !items) return CANARY_ERR_ARG;
+  return canary::batch::route_all(eqn, dim, prec, solves, items, nitems, parity, rka, rkb,
+                                  (hipStream_t)stream);
+}
 
 extern "C" int canary_batch_variant(int eqn, int dim, int prec, int N, int which, int* epw,
                                     int* threads) {

@@ -22,6 +22,21 @@ extern "C" int canary_k_batch_fma(int eqn, int dim, int N, int which, int want, 
   return CANARY_ERR_UNSUPPORTED;
 }
 
+extern "C" int canary_k_batch_fma_all(int eqn, int dim, const canary_batch_solve* solves,
+                                      const canary_batch_item* items, int64_t nitems, int parity,
+                                      double rka, double rkb, void* stream) {
+  using namespace canary;
+  using namespace canary::batch;
+  hipStream_t st = (hipStream_t)stream;
+  if (eqn == CANARY_EULER && dim == 3)
+    return run_all<Euler3DFma>(solves, items, nitems, parity, rka, rkb, st);
+  if (eqn == CANARY_EULER && dim == 2)
+    return run_all<Euler2DFma>(solves, items, nitems, parity, rka, rkb, st);
+  if (eqn == CANARY_SWE && dim == 2)
+    return run_all<SWE2DFma>(solves, items, nitems, parity, rka, rkb, st);
+  return CANARY_ERR_UNSUPPORTED;
+}
+
 extern "C" int canary_k_batch_fma_status(int64_t* info, int clear) {
   unsigned long long h[4] = {0, 0, 0, 0};
   if (hipMemcpyFromSymbol(h, HIP_SYMBOL(canary::g_status), sizeof(h), 0, hipMemcpyDeviceToHost) !=

@@ -42,8 +42,16 @@ __device__ __forceinline__ double jmax(double a, double b) {
 //     log x = 2 atanh(s) = 2 (s + s^3/3 + s^5/5 + ...), s = (x - 1)/(x + 1),
 //     for 1/sqrt2 < x < sqrt2 (|s| < 0.1716: ten terms leave < 2^-55);
 //     e^y as its Taylor series to y^14/14! for |y| <= 1/2 (the next term is
-//     < 2^-55 of e^y).  Outside those ranges the device log / exp are used.
+//     < 2^-55 of e^y).  Outside those ranges, exp(k log x) by the device
+//     library, out of line (pow_by_exp_log).
 //     Evaluated by Horner's rule with fused multiply-adds; a few ulp in all.
+// exp(k log x) by the device library, out of line: the contracted pow's
+// path outside the series ranges (rare for an atmosphere), kept out of the
+// stage kernels' instruction stream.
+__device__ __attribute__((noinline)) inline double pow_by_exp_log(double x, double k) {
+  return exp(k * log(x));
+}
+
 struct FastOps {
   __device__ static double recip(double x) {
     double r = __builtin_amdgcn_rcp(x);
@@ -54,26 +62,22 @@ struct FastOps {
   }
   __device__ static double powk(double x, const PhysParams& p) {
     const double k = p.kappa;
-    double lx;
-    if (x > 0.70710678118654752 && x < 1.4142135623730950) {
-      const double s = (x - 1) * recip(x + 1);
-      const double z = s * s;
-      double a = 1.0 / 19;
-      a = fma(a, z, 1.0 / 17);
-      a = fma(a, z, 1.0 / 15);
-      a = fma(a, z, 1.0 / 13);
-      a = fma(a, z, 1.0 / 11);
-      a = fma(a, z, 1.0 / 9);
-      a = fma(a, z, 1.0 / 7);
-      a = fma(a, z, 1.0 / 5);
-      a = fma(a, z, 1.0 / 3);
-      a = fma(a, z, 1.0);
-      lx = (2 * s) * a;
-    } else {
-      lx = log(x);
-    }
+    if (!(x > 0.70710678118654752 && x < 1.4142135623730950)) return pow_by_exp_log(x, k);
+    const double s = (x - 1) * recip(x + 1);
+    const double z = s * s;
+    double a = 1.0 / 19;
+    a = fma(a, z, 1.0 / 17);
+    a = fma(a, z, 1.0 / 15);
+    a = fma(a, z, 1.0 / 13);
+    a = fma(a, z, 1.0 / 11);
+    a = fma(a, z, 1.0 / 9);
+    a = fma(a, z, 1.0 / 7);
+    a = fma(a, z, 1.0 / 5);
+    a = fma(a, z, 1.0 / 3);
+    a = fma(a, z, 1.0);
+    const double lx = (2 * s) * a;
     const double y = k * lx;
-    if (!(fabs(y) <= 0.5)) return exp(y);
+    if (!(fabs(y) <= 0.5)) return pow_by_exp_log(x, k);
     double b = 1.0 / 87178291200.0;  // 1/14!
     b = fma(b, y, 1.0 / 6227020800.0);
     b = fma(b, y, 1.0 / 479001600.0);

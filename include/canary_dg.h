@@ -310,6 +310,21 @@ int canary_batch_stage(int eqn, int dim, int prec, int N, int epw,
                        const canary_batch_solve* solves, const canary_batch_item* items,
                        int64_t nitems, int parity, double rka, double rkb, void* stream);
 
+/* The batched stage in ONE launch for entries of every order: each
+ * workgroup (256 lanes) takes one work-table entry and runs the stage body
+ * of that entry's N.  Every entry's epw must be the one
+ * canary_batch_all_variant gives for its N (the most elements that fit 256
+ * lanes); an entry of another epw, of an order that does not fit (3D
+ * N = 6, 7) or outside its solve is skipped and recorded as CANARY_ERR_ARG.
+ * The kernel is sized by its largest order (registers and LDS), so it runs
+ * fewer waves per CU than the per-order kernels of canary_batch_stage; it
+ * takes the stage in one launch instead of one per order.  No reference
+ * counterpart (the reference runs one solve per process). */
+int canary_batch_all_variant(int eqn, int dim, int prec, int N, int* epw, int* threads);
+int canary_batch_stage_all(int eqn, int dim, int prec, const canary_batch_solve* solves,
+                           const canary_batch_item* items, int64_t nitems, int parity,
+                           double rka, double rkb, void* stream);
+
 /* Batched precision form: one solve of the table is the set of arrays of
  * the precision form above (q[parity] -> q[1-parity]); items as for
  * canary_batch_stage. */
@@ -352,6 +367,7 @@ typedef struct canary_batch_class {
 } canary_batch_class;
 #define CANARY_BATCH_DROPIN 0
 #define CANARY_BATCH_P32 1
+#define CANARY_BATCH_DROPIN_ALL 2 /* canary_batch_stage_all per class (N, epw ignored) */
 int canary_batch_run(int kind, int eqn, int dim, int prec, const void* solves,
                      const canary_batch_class* classes, int nclasses, int64_t nsteps, int parity,
                      void* stream);

@@ -107,6 +107,37 @@ def test_batch_run_rejects_bad_arguments_before_launch():
     cls = (L.BatchClass * 1)()
     cls[0].items, cls[0].nitems, cls[0].N, cls[0].epw = None, 5, 4, 1
     assert lib.canary_batch_run(L.BATCH_DROPIN, 0, 3, 0, solves, cls, 1, 1, 0, None) == L.ERR_ARG
+    # the single-launch kind: an (eqn, dim, prec) without the kernel is refused
+    cls[0].items = 64
+    assert lib.canary_batch_run(L.BATCH_DROPIN_ALL, 1, 3, 0, solves, cls, 1, 1, 0,
+                                None) == L.ERR_UNSUPPORTED  # 3D shallow water
+
+
+def test_single_launch_variants():
+    """Each order's single-launch variant is its fullest 256-lane one; 3D
+    orders 6 and 7 (343 and 512 nodes) have none."""
+    lib = L.lib()
+    e, t = ctypes.c_int(), ctypes.c_int()
+    want2 = {1: 64, 2: 28, 3: 16, 4: 10, 5: 7, 6: 5, 7: 4}
+    want3 = {1: 32, 2: 9, 3: 4, 4: 2, 5: 1}
+    for prec in (L.FP64, L.FP64_FMA):
+        for N in range(1, 8):
+            assert lib.canary_batch_all_variant(L.SWE, 2, prec, N, ctypes.byref(e),
+                                                ctypes.byref(t)) == 0
+            assert (e.value, t.value) == (want2[N], 256)
+            rc = lib.canary_batch_all_variant(L.EULER, 3, prec, N, ctypes.byref(e),
+                                              ctypes.byref(t))
+            if N <= 5:
+                assert rc == 0 and (e.value, t.value) == (want3[N], 256)
+            else:
+                assert rc == L.ERR_UNSUPPORTED
+    assert lib.canary_batch_all_variant(L.EULER, 3, L.FP32, 4, ctypes.byref(e),
+                                        ctypes.byref(t)) == L.ERR_UNSUPPORTED
+    assert lib.canary_batch_stage_all(L.SWE, 2, L.FP64, None, None, 0, 0, 0.0, 0.0, None) == 0
+    assert lib.canary_batch_stage_all(L.SWE, 2, L.FP64, None, None, 3, 0, 0.0, 0.0,
+                                      None) == L.ERR_ARG
+    assert lib.canary_batch_stage_all(L.SWE, 2, L.FP64, None, None, 1, 2, 0.0, 0.0,
+                                      None) == L.ERR_ARG
 
 
 def test_phys_defaults_are_the_reference_rationals():

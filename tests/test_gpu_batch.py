@@ -314,3 +314,61 @@ def test_batch_swe_advection_mode(cuda):
             Qo = np.array(s.Q, order="F", copy=True)
             assert O.lsrk(s, Qo, np.zeros_like(Qo, order="F"), dt, 3, advection=True) == 0
             assert max_ulp(want, Qo) == 0, k
+
+
+@pytest.mark.parametrize("eqn,dim,mixf", [(SWE, 2, swe_mix), (EULER, 2, euler2d_mix),
+                                          (EULER, 3, euler3d_mix)])
+def test_single_launch_same_bits(cuda, eqn, dim, mixf):
+    """One launch per stage over the entries of every order
+    (canary_batch_stage_all) gives the per-order launches' bits: in the
+    exact order (FP64, and FP32 for shallow water) the same bits, from the
+    library loop and from replayed step graphs; contracted FP64 within 1e-14
+    (the two kernels may fuse different products).  3D orders 6 and 7 have
+    no single-launch variant and are refused."""
+    mix = mixf()
+    if dim == 3:
+        b = Batch(eqn, dim)
+        for s, dt in mix:
+            b.add(s, dt)
+        with pytest.raises(ValueError):
+            b.finalize(single_launch=True)
+        mix = [(s, dt) for s, dt in mix if s.N <= 5]
+    print("single launch eqn=%d dim=%d N=%s" % (eqn, dim, [s.N for s, _ in mix]), flush=True)
+    precs = (FP64, FP64_FMA, FP32) if eqn == SWE else (FP64, FP64_FMA)
+    for prec in precs:
+        ref = Batch(eqn, dim, prec)
+        rids = [ref.add(s, dt) for s, dt in mix]
+        ref.finalize()
+        ref.run(3, graph=False)
+        ref.check()
+        for graph in (False, True):
+            b = Batch(eqn, dim, prec)
+            ids = [b.add(s, dt) for s, dt in mix]
+            b.finalize(single_launch=True)
+            assert len(b.groups) == 1
+            b.run(3, graph=graph)
+            b.check()
+            for (s, _), i, j in zip(mix, ids, rids):
+                got, want = b.state(i), ref.state(j)
+                if prec == FP64_FMA:
+                    assert rel_l2(got, want) <= 1e-14, (s.N, graph)
+                else:
+                    assert max_ulp(got, want) == 0, (prec, s.N, graph)
+
+
+def test_single_launch_bad_entry_is_reported(cuda):
+    """An entry whose epw is not its order's single-launch variant is skipped
+    and reported."""
+    s, dt = swe_mix()[0]
+    b = Batch(SWE, 2)
+    i = b.add(s, dt)
+    b.finalize(single_launch=True)
+    items = np.zeros(1, dtype=ITEM_DTYPE)
+    items["solve"], items["N"], items["epw"], items["e0"] = 0, s.N, b.epw[s.N] - 1, 0
+    t = torch.from_numpy(items.view(np.uint8).copy()).cuda()
+    before = b.state(i)
+    b._launch(0, 0, t.data_ptr(), 1, b.parity, 0.0, 1.0, b._st())
+    with pytest.raises(CanaryError) as ei:
+        b.check()
+    assert ei.value.code == ERR_ARG
+    assert max_ulp(b.state(i), before) == 0
