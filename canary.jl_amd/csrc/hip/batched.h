@@ -5,9 +5,10 @@
 // element of a block; the kernel for one (N, elements-per-workgroup) class
 // runs the fused stage body of dg_kernels.h on every entry of its slice of
 // the table.  One kernel per class keeps each variant's register allocation
-// its own (a single kernel switching over every N is sized by its worst
-// variant and drops to one wave per SIMD); the host launches the classes
-// back to back, or on forked streams inside one graph.
+// its own; the host launches the classes back to back, or on forked streams
+// inside one graph.  batched_all_kernel instead runs the entries of every
+// order in one launch (each order at its fullest 256-lane variant), sized by
+// its largest order.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
