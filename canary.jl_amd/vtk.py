@@ -129,19 +129,38 @@ def read_vtu(fname):
     return arrays.pop("Points"), conn, arrays.pop("types"), arrays
 
 
-def write_state(base_name, s, Q, theta_ref=300.0):
-    """The drivers' plot output (euler3d_set2c.jl:870-882): convert set2c to
+def plot_name(eqn_tag, dim, arrtype, rank, step):
+    """The drivers' file name: viz/euler3D_set2c_<ArrType>_rank_0000_step_00010
+    (euler3d_set2c.jl:879, euler2d_set2c.jl:763) or viz/swe2D_<ArrType>_rank_..
+    (shallow_water.jl:738); eqn_tag is "euler" or "swe"."""
+    if eqn_tag == "swe":
+        return "viz/swe%dD_%s_rank_%04d_step_%05d" % (dim, arrtype, rank, step)
+    return "viz/euler%dD_set2c_%s_rank_%04d_step_%05d" % (dim, arrtype, rank, step)
+
+
+def write_state(base_name, s, Q, theta_ref=300.0, realelems=None):
+    """The drivers' plot output on the element nodes.  Euler
+    (euler3d_set2c.jl:869-882, euler2d_set2c.jl:753-766): convert set2c to
     set2nc (u = U/rho, ..., theta = E/rho), subtract theta_ref from E, and
-    write rho, U, V, (W,) E on the element nodes."""
+    write rho, U, V, (W,) E.  Shallow water (shallow_water.jl:730-741): h, b
+    and the velocities U/(h+b), V/(h+b).  realelems: 1-based element numbers
+    whose sub-cells are written (the points of every element are)."""
+    from ._lib import SWE
     Q = np.asarray(Q)
     dim, Nq = s.dim, s.N + 1
     shp = (Nq,) * dim + (s.nelem,)
     ix = 11 if dim == 3 else 6
     X = tuple(s.vgeo[:, ix + c, :].reshape(shp, order="F") for c in range(dim))
+    if s.eqn == SWE:
+        h, b = Q[:, 2, :], Q[:, 3, :]
+        fields = [("h", h.reshape(shp, order="F")), ("b", b.reshape(shp, order="F")),
+                  ("U", (Q[:, 0, :] / (h + b)).reshape(shp, order="F")),
+                  ("V", (Q[:, 1, :] / (h + b)).reshape(shp, order="F"))]
+        return writemesh(base_name, *X, fields=fields, realelems=realelems)
     rho = Q[:, dim, :]
     names = ["U", "V", "W"][:dim]
     fields = [("ρ", rho.reshape(shp, order="F"))]
     for c, nm in enumerate(names):
         fields.append((nm, (Q[:, c, :] / rho).reshape(shp, order="F")))
     fields.append(("E", (Q[:, dim + 1, :] / rho - theta_ref).reshape(shp, order="F")))
-    return writemesh(base_name, *X, fields=fields)
+    return writemesh(base_name, *X, fields=fields, realelems=realelems)
