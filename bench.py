@@ -75,6 +75,7 @@ def two_d_form(ne, N, steps, warmup):
     import canary_amd.kernels as K
     import canary_amd.problems as P
     from canary_amd._lib import FP64, FP64_FMA, PATH_FUSED, SWE, EULER
+    from canary_amd.batch import bytes_per_element_stage
     from canary_amd.streams import OwnedStream
 
     dev = torch.device("cuda:0")
@@ -96,12 +97,20 @@ def two_d_form(ne, N, steps, warmup):
         nbytes = (swe_bytes(N) if eqn == SWE else algorithmic_bytes(2, N)) * s.nelem
         res = {"workload": "%s, %d^2 quads, N=%d, periodic" % (s.label, ne, N), "dof": dof,
                "bytes_per_element_stage": nbytes // s.nelem}
-        for tag, prec in (("fp64_fma", FP64_FMA), ("fp64", FP64)):
+        prep = K.prepare_geometry(2, N, vgeo, sgeo, vM, vP, bnd, s.nelem)
+        K.check_errors()
+        pbytes = bytes_per_element_stage(eqn, 2, N, prepared=True)
+        for tag, prec, prepared in (("fp64_fma", FP64_FMA, False), ("fp64", FP64, False),
+                                    ("fp64_fma_prepared", FP64_FMA, True),
+                                    ("fp64_prepared", FP64, True)):
             q, q2, r = Q0.clone(), torch.empty_like(Q0), torch.zeros_like(Q0)
             torch.cuda.synchronize()
 
             def loop(n):
                 with torch.cuda.stream(own.stream):
+                    if prepared:
+                        return K.lsrk54_prepared(2, N, q, q2, r, prep, D, s.nelem, dt, n, eqn=eqn,
+                                                 phys=s.phys, prec=prec)
                     return K.lsrk54(2, N, q, r, vgeo, sgeo, D, s.nelem, vM, vP, bnd, dt, n,
                                     eqn=eqn, phys=s.phys, path=PATH_FUSED, prec=prec, Qalt=q2)
             loop(warmup)
@@ -118,7 +127,12 @@ def two_d_form(ne, N, steps, warmup):
             res[tag] = {"us_per_stage": round(t * 1e6, 2),
                         "gdof_stage_s": round(dof / t / 1e9, 3),
                         "hbm_frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4)}
+            if prepared:
+                res[tag]["hbm_frac_on_own_bytes"] = round(pbytes * s.nelem / t / 1e9
+                                                          / HBM_PEAK_GBS, 4)
             del q, q2, r, fin
+        res["bytes_per_element_stage_prepared"] = pbytes
+        del prep
         out[name] = res
         print("2D form %s: %s" % (name, res["fp64_fma"]), file=sys.stderr, flush=True)
         del vgeo, sgeo, D, vM, vP, bnd, Q0, s
@@ -690,8 +704,9 @@ def precision_form(ne, N, steps_csv, timed_steps=10):
 
     def time_fma():
         # the contracted FP64 stage on a scratch copy (the error reference
-        # state st stays on the exact-order path)
+        # state st stays on the exact-order path), after one untimed step
         saved = {k: v.clone() for k, v in st.items()}
+        run64(1, FP64_FMA)
         t = timed(lambda: run64(timed_steps, FP64_FMA)) / (5 * timed_steps)
         for k in st:
             st[k].copy_(saved[k])
@@ -710,6 +725,7 @@ def precision_form(ne, N, steps_csv, timed_steps=10):
         return e0.elapsed_time(e1) / 1e3
 
     marks = sorted(int(v) for v in steps_csv.split(","))
+    tp = {}
     errors, errors_m = {}, {}
     done = 0
     t64 = t32 = t32m = None
@@ -720,11 +736,27 @@ def precision_form(ne, N, steps_csv, timed_steps=10):
         return {nm: float("%.3e" % (np.linalg.norm(a[:, k] - b[:, k]) / np.linalg.norm(b[:, k])))
                 for k, nm in ((3, "rho"), (0, "rhou"), (2, "rhow"), (4, "rhotheta"))}
 
+    prep = K.prepare_geometry(3, N, vgeo, sgeo, vM, vP, bnd, s.nelem)
+    K.check_errors()
+
+    def time_prepared(prec):
+        # the library loop over the prepared arrays (the single-solve
+        # benchmark's measured path) on scratch copies of the current state
+        qa, qb, r = st["q"].clone(), torch.empty_like(st["q"]), st["rhs"].clone()
+        K.lsrk54_prepared(3, N, qa, qb, r, prep, D, s.nelem, dt, 1, phys=s.phys, prec=prec)
+        t = timed(lambda: K.lsrk54_prepared(3, N, qa, qb, r, prep, D, s.nelem, dt, timed_steps,
+                                            phys=s.phys, prec=prec)) / (5 * timed_steps)
+        K.check_errors()
+        del qa, qb, r
+        return t
+
     def time_all():
         t64f = time_fma()
         t64 = timed(lambda: run64(timed_steps)) / (5 * timed_steps)
         t32 = timed(lambda: pf.run(dt, timed_steps)) / (5 * timed_steps)
         t32m = timed(lambda: pfm.run(dt, timed_steps)) / (5 * timed_steps)
+        tp["fma"] = time_prepared(FP64_FMA)
+        tp["exact"] = time_prepared(FP64)
         return t64f, t64, t32, t32m
 
     for m in marks:
@@ -756,13 +788,17 @@ def precision_form(ne, N, steps_csv, timed_steps=10):
             "fp64_fma_ms_per_stage": round(t64f * 1e3, 4),
             "speedup_fp32_vs_fp64": round(t64 / t32, 3),
             "speedup_fp32_vs_fp64_fma": round(t64f / t32, 3),
+            "fp64_prepared_ms_per_stage": {"fma": round(tp["fma"] * 1e3, 4),
+                                           "exact": round(tp["exact"] * 1e3, 4)},
+            "speedup_fp32_vs_fp64_prepared": {"fma": round(tp["fma"] / t32, 3),
+                                              "exact": round(tp["exact"] / t32, 3)},
             "fp32_gdof_stage_s": round(dof / t32 / 1e9, 3),
             "fp64_gdof_stage_s": round(dof / t64 / 1e9, 3),
             "fp32_bytes_per_element_stage": b32,
             "fp32_hbm_frac": round(b32 * s.nelem / t32 / 1e9 / HBM_PEAK_GBS, 4),
             "rel_l2_vs_fp64": errors,
-            "mixed": {"what": "FP32 state q' and RHS arithmetic; metric terms, MJ, MJI, normals, "
-                              "sMJ and the LSRK register dq in FP64",
+            "mixed": {"what": "FP32 state q' and RHS arithmetic; metric terms, MJ, normals, "
+                              "sMJ and the LSRK register dq in FP64 (MJI = 1/MJ in FP64)",
                       "ms_per_stage": round(t32m * 1e3, 4),
                       "speedup_vs_fp64": round(t64 / t32m, 3),
                       "gdof_stage_s": round(dof / t32m / 1e9, 3),
@@ -770,10 +806,11 @@ def precision_form(ne, N, steps_csv, timed_steps=10):
                       "byte_ratio_fp64_over_mixed": round(b64 / b32m, 3),
                       "hbm_frac": round(b32m * s.nelem / t32m / 1e9 / HBM_PEAK_GBS, 4),
                       "rel_l2_vs_fp64": errors_m},
-            "carries_the_speedup": "compact (FP32 metric terms, normals and LSRK register: "
-                                   "%d B per element-stage against %d B for the FP64 stage); the "
+            "carries_the_speedup": "compact (FP32 metric terms, normals, LSRK register and "
+                                   "FP32 copy of the background: %d B per element-stage against "
+                                   "%d B for the FP64 stage, %d B over the prepared arrays); the "
                                    "mixed variant moves %d B, so bytes cap it at %.2fx"
-                                   % (b32, b64, b32m, b64 / b32m),
+                                   % (b32, b64, prepared_bytes(3, N), b32m, b64 / b32m),
             "dt": dt}
 
 
@@ -818,13 +855,16 @@ def batched_form(nsolves, steps, seed=42):
     FP64 (reference evaluation order), contracted FP64 and FP32 batches
     (autotuned elements per workgroup, one graph per step) against the same
     solves run one after another with the single-solve fused stage; each
-    precision also with every stage one launch over all orders."""
+    precision also with every stage one launch over all orders, and both
+    launch forms over the prepared arrays (hbm_frac on the reference-array
+    byte rule throughout, hbm_frac_on_own_bytes on the bytes each form
+    moves)."""
     import numpy as np
     import torch
     import canary_amd.kernels as K
     import canary_amd.problems as P
     from canary_amd._lib import SWE, FP32, FP64, FP64_FMA
-    from canary_amd.batch import Batch
+    from canary_amd.batch import Batch, bytes_per_element_stage
 
     t0 = time.time()
     ens = P.swe_ensemble(n=nsolves, seed=seed)
@@ -855,30 +895,45 @@ def batched_form(nsolves, steps, seed=42):
               file=sys.stderr, flush=True)
         del b
         torch.cuda.empty_cache()
-        # the same batch with every stage ONE launch over all orders
-        # (canary_batch_stage_all), same number of steps, step graphs
-        b = Batch(SWE, 2, prec)
-        ids = [b.add(s, dt) for s, dt in ens]
-        b.finalize(single_launch=True)
-        b.run(2)
-        b.check()
-        t = timed(lambda: b.run(steps))
-        b.check()
-        got = [b.state(i) for i in ids]
-        if prec == FP64_FMA:
-            same = max(float(np.linalg.norm(a - c) / np.linalg.norm(c))
-                       for a, c in zip(got, states[name]))
-        else:
-            same = all(np.array_equal(a, c) for a, c in zip(got, states[name]))
-        out[name]["single_launch"] = {
-            "gdof_stage_s": round(dof * 5 * steps / t / 1e9, 3),
-            "ms_per_step": round(t / steps * 1e3, 4),
-            "epw": {str(k): v for k, v in sorted(b.epw.items())},
-            ("rel_l2_vs_per_order" if prec == FP64_FMA else "same_bits_as_per_order"): same}
-        print("batched form %s, single launch: %.3f GDOF-stage/s"
-              % (name, out[name]["single_launch"]["gdof_stage_s"]), file=sys.stderr, flush=True)
-        del b, got
-        torch.cuda.empty_cache()
+        # the same batch with every stage ONE launch over all orders, and
+        # both launch forms over the prepared arrays; same steps, step graphs
+        ref_bytes = None
+        for tag, prepared, single in (("single_launch", False, True),
+                                      ("prepared", True, False),
+                                      ("prepared_single_launch", True, True)):
+            b = Batch(SWE, 2, prec, prepared=prepared)
+            ids = [b.add(s, dt) for s, dt in ens]
+            if single:
+                b.finalize(single_launch=True)
+            else:
+                b.finalize(autotune=True)
+            b.run(2)
+            b.check()
+            t = timed(lambda: b.run(steps))
+            b.check()
+            got = [b.state(i) for i in ids]
+            if prec == FP64:
+                same = {"same_bits_as_reference_arrays_per_order":
+                        all(np.array_equal(a, c) for a, c in zip(got, states[name]))}
+            else:
+                same = {"rel_l2_vs_reference_arrays_per_order": float("%.3e" % max(
+                    float(np.linalg.norm(a[:, :3] - c[:, :3]) / np.linalg.norm(c[:, :3]))
+                    for a, c in zip(got, states[name])))}
+            if ref_bytes is None:
+                ref_bytes = sum(v.nelem * bytes_per_element_stage(SWE, 2, v.N, 4 if prec == FP32
+                                                                  else 8)
+                                for v in b.solves.values())
+            out[name][tag] = {
+                "gdof_stage_s": round(dof * 5 * steps / t / 1e9, 3),
+                "ms_per_step": round(t / steps * 1e3, 4),
+                "hbm_frac": round(ref_bytes * 5 * steps / t / 1e9 / HBM_PEAK_GBS, 4),
+                "hbm_frac_on_own_bytes": round(b.bytes_per_stage() * 5 * steps / t / 1e9
+                                               / HBM_PEAK_GBS, 4),
+                "epw": {str(k): v for k, v in sorted(b.epw.items())}, **same}
+            print("batched form %s, %s: %.3f GDOF-stage/s"
+                  % (name, tag, out[name][tag]["gdof_stage_s"]), file=sys.stderr, flush=True)
+            del b, got
+            torch.cuda.empty_cache()
     errs = [np.linalg.norm(a[:, :3] - c[:, :3]) / np.linalg.norm(c[:, :3])
             for a, c in zip(states["fp32"], states["fp64"])]
     out["fp32_rel_l2_vs_fp64_max"] = float("%.3e" % max(errs))

@@ -57,12 +57,12 @@ class BatchSolve(C.Structure):
                 ("sgeo", C.c_void_p), ("D", C.c_void_p), ("vmapM", C.c_void_p),
                 ("vmapP", C.c_void_p), ("elemtobndy", C.c_void_p), ("nreal", C.c_int64),
                 ("nelem_total", C.c_int64), ("dt", C.c_double), ("phys", Phys),
-                ("N", C.c_int32), ("advection", C.c_int32)]
+                ("N", C.c_int32), ("advection", C.c_int32), ("pface", C.c_void_p)]
 
 
 class P32Solve(C.Structure):
-    _fields_ = [("q", C.c_void_p * 2), ("rhs", C.c_void_p), ("bg", C.c_void_p),
-                ("vg", C.c_void_p), ("sg", C.c_void_p), ("nbr", C.c_void_p), ("bnd", C.c_void_p),
+    _fields_ = [("q", C.c_void_p * 2), ("rhs", C.c_void_p), ("bg32", C.c_void_p),
+                ("vg", C.c_void_p), ("sg", C.c_void_p), ("nbr", C.c_void_p),
                 ("D", C.c_void_p), ("nreal", C.c_int64), ("dt", C.c_double), ("phys", Phys),
                 ("N", C.c_int32), ("dim", C.c_int32)]
 
@@ -76,7 +76,7 @@ class BatchClass(C.Structure):
                 ("epw", C.c_int32)]
 
 
-BATCH_DROPIN, BATCH_P32, BATCH_DROPIN_ALL = 0, 1, 2
+BATCH_DROPIN, BATCH_P32, BATCH_DROPIN_ALL, BATCH_PREPARED, BATCH_PREPARED_ALL = 0, 1, 2, 3, 4
 
 
 _lib = None
@@ -110,12 +110,12 @@ _SIGS = {
     "canary_check_errors": (INT, [P, P]),
     "canary_perturb32_prepare": (INT, [INT, INT, I64, I64, P, P, P, P, P, P, P, C.POINTER(Phys),
                                        P, P, P, P, P, P, P]),
-    "canary_stage_perturb32": (INT, [INT, INT, P, P, P, P, P, P, P, P, P, I64, D, D, D,
+    "canary_stage_perturb32": (INT, [INT, INT, P, P, P, P, P, P, P, P, I64, D, D, D,
                                      C.POINTER(Phys), P]),
     "canary_perturb32_reconstruct": (INT, [INT, INT, I64, P, P, P, P]),
     "canary_perturb_mixed_prepare": (INT, [INT, INT, I64, I64, P, P, P, P, P, P, P,
                                            C.POINTER(Phys), P, P, P, P, P, P, P]),
-    "canary_stage_perturb_mixed": (INT, [INT, INT, P, P, P, P, P, P, P, P, P, I64, D, D, D,
+    "canary_stage_perturb_mixed": (INT, [INT, INT, P, P, P, P, P, P, P, P, I64, D, D, D,
                                          C.POINTER(Phys), P]),
     "canary_p32_batch_variant": (INT, [INT, INT, INT, C.POINTER(INT), C.POINTER(INT)]),
     "canary_p32_batch_stage": (INT, [INT, INT, INT, P, P, I64, INT, D, D, P]),
@@ -123,6 +123,8 @@ _SIGS = {
     "canary_batch_stage": (INT, [INT, INT, INT, INT, INT, P, P, I64, INT, D, D, P]),
     "canary_batch_all_variant": (INT, [INT, INT, INT, INT, C.POINTER(INT), C.POINTER(INT)]),
     "canary_batch_stage_all": (INT, [INT, INT, INT, P, P, I64, INT, D, D, P]),
+    "canary_batch_stage_prepared": (INT, [INT, INT, INT, INT, INT, P, P, I64, INT, D, D, P]),
+    "canary_batch_stage_all_prepared": (INT, [INT, INT, INT, P, P, I64, INT, D, D, P]),
     "canary_device_count": (INT, []),
     "canary_stream_create": (INT, [C.POINTER(P)]),
     "canary_stream_destroy": (INT, [P]),

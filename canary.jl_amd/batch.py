@@ -69,15 +69,20 @@ def variants32(dim, N):
     return _query(lib().canary_p32_batch_variant, dim, N)
 
 
-def bytes_per_element_stage(eqn, dim, N, word=8):
+def bytes_per_element_stage(eqn, dim, N, word=8, prepared=False):
     """Bytes one element moves per fused stage, every array counted once:
     Q read + write (all states), the LSRK register read + write (evolved
     states), metric terms + MJ + MJI, normals + sMJ at `word` bytes, Int64
-    vmapM + vmapP and elemtobndy."""
+    vmapM + vmapP and elemtobndy; over the prepared arrays: metric terms +
+    MJ, normals + sMJ and one int32 face code per face."""
     Nq = N + 1
     Np, Nfp, nface = Nq ** dim, Nq ** (dim - 1), 2 * dim
     ns, nsr = _NS[(eqn, dim)], _NSR[(eqn, dim)]
-    nvol, nsurf = dim * dim + 2, dim + 1
+    nsurf = dim + 1
+    if prepared:
+        return (word * (2 * ns * Np + 2 * nsr * Np + (dim * dim + 1) * Np + nface * Nfp * nsurf)
+                + 4 * nface)
+    nvol = dim * dim + 2
     return (word * (2 * ns * Np + 2 * nsr * Np + nvol * Np + nface * Nfp * nsurf)
             + 2 * nface * Nfp * 8 + nface * 8)
 
@@ -386,13 +391,18 @@ class Batch(_BatchBase):
     _rowtype = _lib.BatchSolve
     _has_single_launch = True
 
-    def __init__(self, eqn, dim, prec=FP64, device="cuda"):
+    def __init__(self, eqn, dim, prec=FP64, device="cuda", prepared=False):
+        """prepared: every solve runs over the prepared arrays of
+        canary_prepare_geometry (metric terms and MJ, normal and sMJ, one
+        face code per face) instead of the reference arrays: fewer bytes per
+        element-stage, and in the exact order the same bits."""
         if (eqn, dim) not in _NS:
             raise ValueError("unknown (eqn, dim) = (%r, %r)" % (eqn, dim))
         if prec not in (FP64, FP64_FMA, FP32):
             raise ValueError("prec must be FP64, FP64_FMA or FP32")
         super().__init__(device)
         self.eqn, self.dim, self.prec = eqn, dim, prec
+        self.prepared = bool(prepared)
         self.rdtype = torch.float32 if prec == FP32 else torch.float64
 
     def _variants(self, N):
@@ -400,10 +410,13 @@ class Batch(_BatchBase):
 
     def _launch_raw(self, N, epw, table, items_ptr, count, parity, rka, rkb, stream):
         if N == 0:  # the single-launch class
-            return lib().canary_batch_stage_all(self.eqn, self.dim, self.prec, _ptr(table),
-                                                items_ptr, count, parity, rka, rkb, stream)
-        return lib().canary_batch_stage(self.eqn, self.dim, self.prec, N, epw, _ptr(table),
-                                        items_ptr, count, parity, rka, rkb, stream)
+            fn = (lib().canary_batch_stage_all_prepared if self.prepared
+                  else lib().canary_batch_stage_all)
+            return fn(self.eqn, self.dim, self.prec, _ptr(table), items_ptr, count, parity, rka,
+                      rkb, stream)
+        fn = lib().canary_batch_stage_prepared if self.prepared else lib().canary_batch_stage
+        return fn(self.eqn, self.dim, self.prec, N, epw, _ptr(table), items_ptr, count, parity,
+                  rka, rkb, stream)
 
     def _all_variant(self, N):
         e, t = C.c_int(), C.c_int()
@@ -414,14 +427,18 @@ class Batch(_BatchBase):
         return e.value
 
     def _run_kind(self):
-        kind = _lib.BATCH_DROPIN_ALL if self.single_launch else _lib.BATCH_DROPIN
+        if self.prepared:
+            kind = _lib.BATCH_PREPARED_ALL if self.single_launch else _lib.BATCH_PREPARED
+        else:
+            kind = _lib.BATCH_DROPIN_ALL if self.single_launch else _lib.BATCH_DROPIN
         return kind, self.eqn, self.dim, self.prec
 
     def _word(self):
         return 4 if self.prec == FP32 else 8
 
     def _bytes_per_element_stage(self, v):
-        return bytes_per_element_stage(self.eqn, self.dim, v.N, self._word())
+        return bytes_per_element_stage(self.eqn, self.dim, v.N, self._word(),
+                                       prepared=self.prepared)
 
     def add(self, s, dt, Q=None, advection=False):
         """Upload one solve (a problems.Setup; Q defaults to s.Q).  Returns its
@@ -447,9 +464,21 @@ class Batch(_BatchBase):
         v.Q = [q, torch.empty_like(q)] if self.parity == 0 else [torch.empty_like(q), q]
         v.rhs = torch.zeros_like(q)
         arr = device_arrays(s, dv)
-        v.vgeo = arr["vgeo"].to(rd)
-        v.sgeo = arr["sgeo"].to(rd)
-        v.D, v.vmapM, v.vmapP, v.bndy = arr["D"], arr["vmapM"], arr["vmapP"], arr["bndy"]
+        v.D = arr["D"]
+        if self.prepared:
+            from . import kernels as K
+            prep = K.prepare_geometry(s.dim, s.N, arr["vgeo"], arr["sgeo"], arr["vmapM"],
+                                      arr["vmapP"], arr["bndy"], s.nelem)
+            K.check_errors()
+            v.vgeo, v.sgeo = prep["pvgeo"].to(rd), prep["psgeo"].to(rd)
+            v.pface = prep["pface"]
+            v.vmapM = v.vmapP = v.bndy = None
+            del prep
+        else:
+            v.vgeo = arr["vgeo"].to(rd)
+            v.sgeo = arr["sgeo"].to(rd)
+            v.vmapM, v.vmapP, v.bndy = arr["vmapM"], arr["vmapP"], arr["bndy"]
+            v.pface = None
         v.phys = s.phys
         v.advection = bool(advection)
         if s.eqn == _lib.EULER:  # the pressure's pow table, before any capture
@@ -460,8 +489,13 @@ class Batch(_BatchBase):
         t.Q[0], t.Q[1] = v.Q[0].data_ptr(), v.Q[1].data_ptr()
         t.rhs = v.rhs.data_ptr()
         t.vgeo, t.sgeo, t.D = v.vgeo.data_ptr(), v.sgeo.data_ptr(), v.D.data_ptr()
-        t.vmapM, t.vmapP = v.vmapM.data_ptr(), v.vmapP.data_ptr()
-        t.elemtobndy = v.bndy.data_ptr()
+        if v.pface is not None:
+            t.vmapM = t.vmapP = t.elemtobndy = None
+            t.pface = v.pface.data_ptr()
+        else:
+            t.vmapM, t.vmapP = v.vmapM.data_ptr(), v.vmapP.data_ptr()
+            t.elemtobndy = v.bndy.data_ptr()
+            t.pface = None
         t.nreal = t.nelem_total = v.nelem
         t.dt = v.dt
         t.phys = v.phys
@@ -474,6 +508,9 @@ class Batch(_BatchBase):
         d = s.dim
         Np, Nfp, nf, ne = s.Np, (s.N + 1) ** (d - 1), 2 * d, s.nelem
         nq = Np * s.nstate * ne
+        if self.prepared:
+            return (w * (3 * nq + (d * d + 1) * Np * ne + (d + 1) * Nfp * nf * ne)
+                    + 4 * nf * ne + 8 * (s.N + 1) ** 2)
         nvg, nsg = (14, 5) if d == 3 else (8, 4)
         return (w * (3 * nq + nvg * Np * ne + nsg * Nfp * nf * ne)
                 + 8 * (2 * Nfp * nf * ne + nf * ne + (s.N + 1) ** 2))
@@ -543,8 +580,8 @@ class Batch32(_BatchBase):
         pf = v.pf
         t.q[0], t.q[1] = v.Q[0].data_ptr(), v.Q[1].data_ptr()
         t.rhs = v.rhs.data_ptr()
-        t.bg, t.vg, t.sg = pf.bg.data_ptr(), pf.vg.data_ptr(), pf.sg.data_ptr()
-        t.nbr, t.bnd, t.D = pf.nbr.data_ptr(), pf.bnd.data_ptr(), pf.D.data_ptr()
+        t.bg32, t.vg, t.sg = pf.bg32.data_ptr(), pf.vg.data_ptr(), pf.sg.data_ptr()
+        t.nbr, t.D = pf.nbr.data_ptr(), pf.D.data_ptr()
         t.nreal = v.nelem
         t.dt = v.dt
         t.phys = v.phys
@@ -552,11 +589,13 @@ class Batch32(_BatchBase):
         t.dim = self.dim
 
     def footprint(self, s):
-        """Device bytes a solve of this setup holds in the batch."""
+        """Device bytes a solve of this setup holds in the batch: q', its
+        ping-pong twin and dq, the metric terms and MJ, the face records
+        (FP32), the background in FP64 and FP32, the face codes and D."""
         d = s.dim
         Np, Nfp, nf, ns, ne = s.Np, (s.N + 1) ** (d - 1), 2 * d, d + 2, s.nelem
-        return ne * (4 * (3 * ns * Np + (d * d + 2) * Np + 4 * nf * Nfp + nf * Nfp)
-                     + 8 * 3 * Np + nf) + 8 * s.D.size
+        return ne * (4 * (3 * ns * Np + (d * d + 1) * Np + 4 * nf * Nfp + 3 * Np + nf)
+                     + 8 * 3 * Np) + 8 * s.D.size
 
     def state_tensor(self, sid):
         v = self.solves[sid]

@@ -61,24 +61,43 @@ struct BCfg {
   static constexpr int NT = C::NT;
 };
 
+// One stage of the block at e0 of solve sv, over the reference arrays or,
+// PREP, over the prepared ones (pvgeo / psgeo in vgeo / sgeo, the face codes
+// in pface; the Prepared layout of dg_kernels.h).
+template <class Phys, int NQ, int EPW, int NT, int PF, bool PREP>
+__device__ __forceinline__ void solve_stage(char* lds, const canary_batch_solve& sv, int64_t e0,
+                                            int parity, double rka, double rkb,
+                                            const PhysParams& p) {
+  using Real = typename Phys::Real;
+  if constexpr (PREP)
+    stage_body<Phys, NQ, EPW, NT, PF, true, true, true, true>(
+        lds, (Real*)sv.rhs, (const Real*)sv.Q[parity], (Real*)sv.Q[1 - parity],
+        (const Real*)sv.vgeo, (const Real*)sv.sgeo, sv.D, e0, sv.nreal, sv.nelem_total, nullptr,
+        reinterpret_cast<const int64_t*>(sv.pface), nullptr, rka, rkb, sv.dt, p, sv.advection);
+  else
+    stage_body<Phys, NQ, EPW, NT, PF, true>(
+        lds, (Real*)sv.rhs, (const Real*)sv.Q[parity], (Real*)sv.Q[1 - parity],
+        (const Real*)sv.vgeo, (const Real*)sv.sgeo, sv.D, e0, sv.nreal, sv.nelem_total,
+        sv.vmapM, sv.vmapP, sv.elemtobndy, rka, rkb, sv.dt, p, sv.advection);
+}
+
 // One workgroup per work-table entry; workgroup b takes entry
 // (b & 7) * chunk + (b >> 3) (grid = 8 * chunk), so XCD b & 7 gets a
 // contiguous run of the table.  (A grid-stride loop over entries keeps the
 // per-thread face and LDS addressing live across iterations and costs
 // about 100 VGPRs in 3D; one entry per workgroup does not.)
-template <class Phys, int NQ, int EPW>
+template <class Phys, int NQ, int EPW, bool PREP>
 __global__ __launch_bounds__((BCfg<Phys, NQ, EPW>::NT)) void batched_kernel(
     const canary_batch_solve* __restrict__ solves, const canary_batch_item* __restrict__ items,
     int64_t nitems, int64_t chunk, int parity, double rka, double rkb,
     const crpow::KappaTable* __restrict__ ktabs, int nktab) {
   using C = FusedCfg<Phys, NQ, EPW, false>;
-  using Real = typename Phys::Real;
   __shared__ __attribute__((aligned(16))) char lds[C::LDS];
   const int64_t w = stage_block(blockIdx.x, chunk, parity);  // direction alternates per stage
   if (w >= nitems) return;
   const canary_batch_item it = items[w];
   const canary_batch_solve& sv = solves[it.solve];
-  if (it.N != NQ - 1 || it.epw != EPW || sv.N != NQ - 1) {
+  if (it.N != NQ - 1 || it.epw != EPW || sv.N != NQ - 1 || (sv.pface != nullptr) != PREP) {
     if (threadIdx.x == 0) record_error(1 /*CANARY_ERR_ARG*/, it.solve, it.e0 + 1, it.N);
     return;
   }
@@ -95,10 +114,8 @@ __global__ __launch_bounds__((BCfg<Phys, NQ, EPW>::NT)) void batched_kernel(
       }
   // 3D N=6 prefetches no face item (fewer VGPRs: a third 384-lane
   // workgroup per CU), as the single-solve stage does
-  stage_body<Phys, NQ, EPW, C::NT, (Phys::DIM == 3 && NQ == 7 && EPW == 1) ? 0 : 1, true>(
-      lds, (Real*)sv.rhs, (const Real*)sv.Q[parity], (Real*)sv.Q[1 - parity],
-      (const Real*)sv.vgeo, (const Real*)sv.sgeo, sv.D, it.e0, sv.nreal, sv.nelem_total,
-      sv.vmapM, sv.vmapP, sv.elemtobndy, rka, rkb, sv.dt, p, sv.advection);
+  solve_stage<Phys, NQ, EPW, C::NT, (Phys::DIM == 3 && NQ == 7 && EPW == 1) ? 0 : 1, PREP>(
+      lds, sv, it.e0, parity, rka, rkb, p);
 }
 
 // ---------------------------------------------- every class in one launch
@@ -129,21 +146,17 @@ constexpr int all_lds() {
   return m;
 }
 
-template <class Phys, int NQ>
+template <class Phys, int NQ, bool PREP>
 __device__ __forceinline__ void all_item(char* lds, const canary_batch_solve& sv,
                                          const canary_batch_item& it, int parity, double rka,
                                          double rkb, const PhysParams& p) {
   if constexpr (fits_all<Phys, NQ>()) {
-    using Real = typename Phys::Real;
     constexpr int E = epw_candidate<Phys::DIM, NQ, 0>();
-    stage_body<Phys, NQ, E, MAXT, 1, true>(
-        lds, (Real*)sv.rhs, (const Real*)sv.Q[parity], (Real*)sv.Q[1 - parity],
-        (const Real*)sv.vgeo, (const Real*)sv.sgeo, sv.D, it.e0, sv.nreal, sv.nelem_total,
-        sv.vmapM, sv.vmapP, sv.elemtobndy, rka, rkb, sv.dt, p, sv.advection);
+    solve_stage<Phys, NQ, E, MAXT, 1, PREP>(lds, sv, it.e0, parity, rka, rkb, p);
   }
 }
 
-template <class Phys>
+template <class Phys, bool PREP>
 __global__ __launch_bounds__(MAXT) void batched_all_kernel(
     const canary_batch_solve* __restrict__ solves, const canary_batch_item* __restrict__ items,
     int64_t nitems, int64_t chunk, int parity, double rka, double rkb,
@@ -163,7 +176,7 @@ __global__ __launch_bounds__(MAXT) void batched_all_kernel(
     case 6: want = fits_all<Phys, 7>() ? epw_candidate<Phys::DIM, 7, 0>() : 0; break;
     case 7: want = fits_all<Phys, 8>() ? epw_candidate<Phys::DIM, 8, 0>() : 0; break;
   }
-  if (want == 0 || it.epw != want || sv.N != it.N) {
+  if (want == 0 || it.epw != want || sv.N != it.N || (sv.pface != nullptr) != PREP) {
     if (threadIdx.x == 0) record_error(1 /*CANARY_ERR_ARG*/, it.solve, it.e0 + 1, it.N);
     return;
   }
@@ -179,29 +192,29 @@ __global__ __launch_bounds__(MAXT) void batched_all_kernel(
         break;
       }
   switch (it.N) {
-    case 1: all_item<Phys, 2>(lds, sv, it, parity, rka, rkb, p); break;
-    case 2: all_item<Phys, 3>(lds, sv, it, parity, rka, rkb, p); break;
-    case 3: all_item<Phys, 4>(lds, sv, it, parity, rka, rkb, p); break;
-    case 4: all_item<Phys, 5>(lds, sv, it, parity, rka, rkb, p); break;
-    case 5: all_item<Phys, 6>(lds, sv, it, parity, rka, rkb, p); break;
-    case 6: all_item<Phys, 7>(lds, sv, it, parity, rka, rkb, p); break;
-    case 7: all_item<Phys, 8>(lds, sv, it, parity, rka, rkb, p); break;
+    case 1: all_item<Phys, 2, PREP>(lds, sv, it, parity, rka, rkb, p); break;
+    case 2: all_item<Phys, 3, PREP>(lds, sv, it, parity, rka, rkb, p); break;
+    case 3: all_item<Phys, 4, PREP>(lds, sv, it, parity, rka, rkb, p); break;
+    case 4: all_item<Phys, 5, PREP>(lds, sv, it, parity, rka, rkb, p); break;
+    case 5: all_item<Phys, 6, PREP>(lds, sv, it, parity, rka, rkb, p); break;
+    case 6: all_item<Phys, 7, PREP>(lds, sv, it, parity, rka, rkb, p); break;
+    case 7: all_item<Phys, 8, PREP>(lds, sv, it, parity, rka, rkb, p); break;
   }
 }
 
-template <class Phys>
+template <class Phys, bool PREP>
 int run_all(const canary_batch_solve* solves, const canary_batch_item* items, int64_t nitems,
             int parity, double rka, double rkb, hipStream_t st) {
   const int64_t chunk = (nitems + 7) / 8;
   if (chunk > ((int64_t)1 << 27)) return CANARY_ERR_ARG;
   int nktab = 0;
   const crpow::KappaTable* ktabs = Phys::HAS_P ? kappa_tables(&nktab) : nullptr;
-  batched_all_kernel<Phys><<<dim3((unsigned)(8 * chunk)), dim3(MAXT), 0, st>>>(
+  batched_all_kernel<Phys, PREP><<<dim3((unsigned)(8 * chunk)), dim3(MAXT), 0, st>>>(
       solves, items, nitems, chunk, parity, rka, rkb, ktabs, nktab);
   return hipGetLastError() == hipSuccess ? 0 : CANARY_ERR_LAUNCH;
 }
 
-template <class Phys, int NQ, int WHICH>
+template <class Phys, int NQ, int WHICH, bool PREP>
 int run_which(int* epw, int* threads, const canary_batch_solve* solves,
               const canary_batch_item* items, int64_t nitems, int parity, double rka, double rkb,
               hipStream_t st) {
@@ -219,7 +232,7 @@ int run_which(int* epw, int* threads, const canary_batch_solve* solves,
     if (chunk > ((int64_t)1 << 27)) return CANARY_ERR_ARG;
     int nktab = 0;
     const crpow::KappaTable* ktabs = Phys::HAS_P ? kappa_tables(&nktab) : nullptr;
-    batched_kernel<Phys, NQ, E><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, st>>>(
+    batched_kernel<Phys, NQ, E, PREP><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, st>>>(
         solves, items, nitems, chunk, parity, rka, rkb, ktabs, nktab);
     return hipGetLastError() == hipSuccess ? 0 : CANARY_ERR_LAUNCH;
   }
@@ -230,33 +243,36 @@ int run_which(int* epw, int* threads, const canary_batch_solve* solves,
 template <class Phys, int NQ>
 int run_nq(int which, int want, int* epw, int* threads, const canary_batch_solve* solves,
            const canary_batch_item* items, int64_t nitems, int parity, double rka, double rkb,
-           hipStream_t st) {
+           bool prep, hipStream_t st) {
   if (which >= 0) {
-    if (which == 0) return run_which<Phys, NQ, 0>(epw, threads, 0, 0, 0, 0, 0, 0, 0);
-    if (which == 1) return run_which<Phys, NQ, 1>(epw, threads, 0, 0, 0, 0, 0, 0, 0);
-    if (which == 2) return run_which<Phys, NQ, 2>(epw, threads, 0, 0, 0, 0, 0, 0, 0);
+    if (which == 0) return run_which<Phys, NQ, 0, false>(epw, threads, 0, 0, 0, 0, 0, 0, 0);
+    if (which == 1) return run_which<Phys, NQ, 1, false>(epw, threads, 0, 0, 0, 0, 0, 0, 0);
+    if (which == 2) return run_which<Phys, NQ, 2, false>(epw, threads, 0, 0, 0, 0, 0, 0, 0);
     return CANARY_ERR_UNSUPPORTED;
   }
   constexpr int E0 = epw_candidate<Phys::DIM, NQ, 0>();
   constexpr int E1 = epw_candidate<Phys::DIM, NQ, 1>();
   constexpr int E2 = epw_candidate<Phys::DIM, NQ, 2>();
-  if (want == E0 && E0 > 0)
-    return run_which<Phys, NQ, 0>(nullptr, nullptr, solves, items, nitems, parity, rka, rkb, st);
-  if (want == E1 && E1 > 0)
-    return run_which<Phys, NQ, 1>(nullptr, nullptr, solves, items, nitems, parity, rka, rkb, st);
-  if (want == E2 && E2 > 0)
-    return run_which<Phys, NQ, 2>(nullptr, nullptr, solves, items, nitems, parity, rka, rkb, st);
+#define CANARY_RUN_W(W)                                                                         \
+  return prep ? run_which<Phys, NQ, W, true>(nullptr, nullptr, solves, items, nitems, parity,   \
+                                             rka, rkb, st)                                      \
+              : run_which<Phys, NQ, W, false>(nullptr, nullptr, solves, items, nitems, parity,  \
+                                              rka, rkb, st);
+  if (want == E0 && E0 > 0) CANARY_RUN_W(0)
+  if (want == E1 && E1 > 0) CANARY_RUN_W(1)
+  if (want == E2 && E2 > 0) CANARY_RUN_W(2)
+#undef CANARY_RUN_W
   return CANARY_ERR_UNSUPPORTED;
 }
 
 template <class Phys>
 int run(int N, int which, int want, int* epw, int* threads, const canary_batch_solve* solves,
         const canary_batch_item* items, int64_t nitems, int parity, double rka, double rkb,
-        hipStream_t st) {
+        bool prep, hipStream_t st) {
 #define CANARY_BATCH_N(NN)                                                                    \
   case NN:                                                                                    \
     return run_nq<Phys, NN + 1>(which, want, epw, threads, solves, items, nitems, parity, rka, \
-                                rkb, st);
+                                rkb, prep, st);
   switch (N) {
     CANARY_BATCH_N(1)
     CANARY_BATCH_N(2)

@@ -6,34 +6,36 @@
 extern "C" int canary_k_batch_fma(int eqn, int dim, int N, int which, int want, int* epw,
                                   int* threads, const canary_batch_solve* solves,
                                   const canary_batch_item* items, int64_t nitems, int parity,
-                                  double rka, double rkb, void* stream) {
+                                  double rka, double rkb, int prep, void* stream) {
   using namespace canary;
   using namespace canary::batch;
   hipStream_t st = (hipStream_t)stream;
+  const bool pr = prep != 0;
   if (eqn == CANARY_EULER && dim == 3)
     return run<Euler3DFma>(N, which, want, epw, threads, solves, items, nitems, parity, rka, rkb,
-                           st);
+                           pr, st);
   if (eqn == CANARY_EULER && dim == 2)
     return run<Euler2DFma>(N, which, want, epw, threads, solves, items, nitems, parity, rka, rkb,
-                           st);
+                           pr, st);
   if (eqn == CANARY_SWE && dim == 2)
     return run<SWE2DFma>(N, which, want, epw, threads, solves, items, nitems, parity, rka, rkb,
-                         st);
+                         pr, st);
   return CANARY_ERR_UNSUPPORTED;
 }
 
 extern "C" int canary_k_batch_fma_all(int eqn, int dim, const canary_batch_solve* solves,
                                       const canary_batch_item* items, int64_t nitems, int parity,
-                                      double rka, double rkb, void* stream) {
+                                      double rka, double rkb, int prep, void* stream) {
   using namespace canary;
   using namespace canary::batch;
   hipStream_t st = (hipStream_t)stream;
-  if (eqn == CANARY_EULER && dim == 3)
-    return run_all<Euler3DFma>(solves, items, nitems, parity, rka, rkb, st);
-  if (eqn == CANARY_EULER && dim == 2)
-    return run_all<Euler2DFma>(solves, items, nitems, parity, rka, rkb, st);
-  if (eqn == CANARY_SWE && dim == 2)
-    return run_all<SWE2DFma>(solves, items, nitems, parity, rka, rkb, st);
+#define CANARY_ALL(P)                                                                   \
+  return prep ? run_all<P, true>(solves, items, nitems, parity, rka, rkb, st)           \
+              : run_all<P, false>(solves, items, nitems, parity, rka, rkb, st);
+  if (eqn == CANARY_EULER && dim == 3) CANARY_ALL(Euler3DFma)
+  if (eqn == CANARY_EULER && dim == 2) CANARY_ALL(Euler2DFma)
+  if (eqn == CANARY_SWE && dim == 2) CANARY_ALL(SWE2DFma)
+#undef CANARY_ALL
   return CANARY_ERR_UNSUPPORTED;
 }
 

@@ -112,8 +112,8 @@ void register_kappa(int eqn, const canary_phys* phys, void* stream) {
 extern "C" int canary_batch_run(int kind, int eqn, int dim, int prec, const void* solves,
                                 const canary_batch_class* classes, int nclasses, int64_t nsteps,
                                 int parity, void* stream) {
-  if (kind != CANARY_BATCH_DROPIN && kind != CANARY_BATCH_P32 && kind != CANARY_BATCH_DROPIN_ALL)
-    return CANARY_ERR_ARG;
+  if (kind < CANARY_BATCH_DROPIN || kind > CANARY_BATCH_PREPARED_ALL) return CANARY_ERR_ARG;
+  const bool all = kind == CANARY_BATCH_DROPIN_ALL || kind == CANARY_BATCH_PREPARED_ALL;
   if (nclasses < 0 || nsteps < 0 || (parity != 0 && parity != 1)) return CANARY_ERR_ARG;
   if (nsteps == 0 || nclasses == 0) return 0;
   if (!solves || !classes) return CANARY_ERR_ARG;
@@ -123,7 +123,7 @@ extern "C" int canary_batch_run(int kind, int eqn, int dim, int prec, const void
     if (cl.nitems < 0) return CANARY_ERR_ARG;
     if (cl.nitems == 0) continue;
     if (!cl.items) return CANARY_ERR_ARG;
-    if (kind == CANARY_BATCH_DROPIN_ALL) {  // entries of any order, checked by the kernel
+    if (all) {  // entries of any order, checked by the kernel
       int e = 0, t = 0;
       const int rc = canary_batch_all_variant(eqn, dim, prec, 1, &e, &t);
       if (rc) return rc;
@@ -132,9 +132,9 @@ extern "C" int canary_batch_run(int kind, int eqn, int dim, int prec, const void
     bool found = false;
     for (int which = 0; which < 3 && !found; ++which) {
       int e = 0, t = 0;
-      const int rc = kind == CANARY_BATCH_DROPIN
-                         ? canary_batch_variant(eqn, dim, prec, cl.N, which, &e, &t)
-                         : canary_p32_batch_variant(dim, cl.N, which, &e, &t);
+      const int rc = kind == CANARY_BATCH_P32
+                         ? canary_p32_batch_variant(dim, cl.N, which, &e, &t)
+                         : canary_batch_variant(eqn, dim, prec, cl.N, which, &e, &t);
       found = rc == 0 && e == cl.epw;
     }
     if (!found) return CANARY_ERR_UNSUPPORTED;
@@ -154,6 +154,14 @@ extern "C" int canary_batch_run(int kind, int eqn, int dim, int prec, const void
         else if (kind == CANARY_BATCH_DROPIN_ALL)
           rc = canary_batch_stage_all(eqn, dim, prec, (const canary_batch_solve*)solves, cl.items,
                                       cl.nitems, par, A[(s + 1) % 5], B[s], stream);
+        else if (kind == CANARY_BATCH_PREPARED)
+          rc = canary_batch_stage_prepared(eqn, dim, prec, cl.N, cl.epw,
+                                           (const canary_batch_solve*)solves, cl.items, cl.nitems,
+                                           par, A[(s + 1) % 5], B[s], stream);
+        else if (kind == CANARY_BATCH_PREPARED_ALL)
+          rc = canary_batch_stage_all_prepared(eqn, dim, prec, (const canary_batch_solve*)solves,
+                                               cl.items, cl.nitems, par, A[(s + 1) % 5], B[s],
+                                               stream);
         else
           rc = canary_p32_batch_stage(dim, cl.N, cl.epw, (const canary_p32_solve*)solves,
                                       cl.items, cl.nitems, par, A[(s + 1) % 5], B[s], stream);

@@ -13,18 +13,23 @@
 // For a hydrostatic (or uniform) background L(Q0) is roundoff (measured
 // below 2e-12 of the buoyancy scale in 3D, exactly 0 for a uniform state
 // without gravity), so it is not carried.  Metric terms and normals are
-// computed in FP64 and rounded to FP32; the background is read in FP64; the
-// LSRK update q' + ((rkb*dt) dq) MJI is evaluated in FP64 and rounded once.
+// computed in FP64 and rounded to FP32; the background is kept in FP64 and
+// its FP32 rounding (the values the FP32 arithmetic uses) is what the stage
+// reads; the LSRK update q' + ((rkb*dt) dq) MJI is evaluated in FP64, with
+// MJI = 1 / MJ, and rounded once.
 //
 // Data (per element contiguous, reference-style column-major; NS = dim + 2,
 // NF = 2 dim faces):
 //   qp, rhs  float (Np, NS, nelem)          perturbation state / LSRK register
-//   bg       double (Np, 3, nelem)          rho0, rhotheta0, P0
-//   vg       float (Np, dim*dim + 2, nelem) metric terms, MJ, MJI
+//   bg       double (Np, 3, nelem)          rho0, rhotheta0, P0 (reconstruction)
+//   bg32     float (Np, 3, nelem)           the same rounded to FP32 (the stage)
+//   vg       float (Np, dim*dim + 1, nelem) metric terms, MJ
 //   sg       float4 (Nfp, NF, nelem)        (nx, ny, nz, sMJ), nz = 0 in 2D
-//   nbr      int32 (Nfp, NF, nelem)         0-based P-side global node (vmapP - 1)
-//   bnd      int8  (NF, nelem)              elemtobndy
-// vmapM must be the canonical face mask of `mappings` (checked by prepare).
+//   nbr      int32 (NF, nelem)              face code: -1 wall, else NF eP + fP
+// prepare checks what the compact arrays leave out: vmapM is the canonical
+// face mask of `mappings` (src/mesh.jl:994), each vmapP column is the
+// canonical mask of one neighbour face (src/mesh.jl:995), elemtobndy is 0
+// or 1, and MJI is 1 ./ MJ.
 // Beside the single-solve stage there is a batched stage over a work table
 // of (solve, first element) entries, as in batched.hip.
 //
@@ -67,7 +72,7 @@ template <int DIM>
 struct Dims {
   static constexpr int NS = DIM + 2;            // U, V, (W), rho, rhotheta
   static constexpr int RHO = DIM, E = DIM + 1;
-  static constexpr int NV = DIM * DIM + 2;      // FP32 vg fields
+  static constexpr int NV = DIM * DIM + 1;      // vg fields: metric terms, MJ
   static constexpr int NVG = DIM == 3 ? 14 : 8; // FP64 vgeo fields
   static constexpr int NSG = DIM == 3 ? 5 : 4;  // FP64 sgeo fields
   static constexpr int NF = 2 * DIM;
@@ -97,7 +102,7 @@ __global__ __launch_bounds__(256) void prepare_kernel(
     const int64_t* __restrict__ vmapM, const int64_t* __restrict__ vmapP,
     const int64_t* __restrict__ bndy, const double* __restrict__ Q0, const double* __restrict__ Q,
     double kappa_d, double p0, double R, GT* __restrict__ vg, GT* __restrict__ sg,
-    int32_t* __restrict__ nbr, int8_t* __restrict__ bnd, double* __restrict__ bg,
+    int32_t* __restrict__ nbr, double* __restrict__ bg, float* __restrict__ bg32,
     float* __restrict__ qp) {
   using G = Dims<DIM>;
   constexpr int Np = ipow(NQ, DIM), Nfp = ipow(NQ, DIM - 1), NFN = G::NF * Nfp, NS = G::NS;
@@ -110,13 +115,20 @@ __global__ __launch_bounds__(256) void prepare_kernel(
     GT* o = vg + (size_t)G::NV * Np * e;
 #pragma unroll
     for (int c = 0; c < G::NV; ++c) o[n + Np * c] = (GT)v[n + Np * c];
+    const double MJ = v[n + Np * (DIM * DIM)], MJI = v[n + Np * (DIM * DIM + 1)];
+    if (!(MJI == 1.0 / MJ)) p32_error(1 /*CANARY_ERR_ARG*/, 0, e + 1, n + 1);
     const double* q0 = Q0 + (size_t)NS * Np * e;
     const double* q = Q + (size_t)NS * Np * e;
     const double rho0 = q0[n + G::RHO * Np], rt0 = q0[n + G::E * Np];
+    const double P0 = p0 * crpow::pow_cr(R * rt0 / p0, kappa_d);
     double* b = bg + (size_t)3 * Np * e;
     b[n] = rho0;
     b[n + Np] = rt0;
-    b[n + 2 * Np] = p0 * crpow::pow_cr(R * rt0 / p0, kappa_d);
+    b[n + 2 * Np] = P0;
+    float* b32 = bg32 + (size_t)3 * Np * e;
+    b32[n] = (float)rho0;
+    b32[n + Np] = (float)rt0;
+    b32[n + 2 * Np] = (float)P0;
     float* qo = qp + (size_t)NS * Np * e;
 #pragma unroll
     for (int s = 0; s < NS; ++s) qo[n + Np * s] = (float)(q[n + Np * s] - q0[n + Np * s]);
@@ -131,19 +143,40 @@ __global__ __launch_bounds__(256) void prepare_kernel(
     so[1] = (GT)s[1];
     so[2] = DIM == 3 ? (GT)s[2] : (GT)0;
     so[3] = (GT)s[DIM];
-    const int64_t idM = vmapM[g], idP = vmapP[g];
+    const int64_t idM = vmapM[g];
     if (idM != (int64_t)Np * e + fmask<DIM, NQ>(f, fn) + 1) p32_error(5, f + 1, e + 1, idM);
-    int64_t pn = idP - 1;
-    if (idP < 1 || idP > ntotal * Np || pn > 0x7fffffffll) {
-      p32_error(5, f + 1, e + 1, idP);
-      pn = idM - 1;
+  }
+  // faces: wall (-1) or the neighbour element and face of the whole vmapP column
+  for (int64_t g = tid; g < nelem * G::NF; g += stride) {
+    const int64_t e = g / G::NF;
+    const int f = (int)(g - e * G::NF);
+    const int64_t bc = bndy[g];
+    int32_t code = -1;
+    if (bc == 0) {
+      const int64_t* vp = vmapP + (size_t)Nfp * g;
+      const int64_t id0 = vp[0];
+      if (id0 < 1 || id0 > ntotal * Np) {
+        p32_error(5 /*CANARY_ERR_VMAP*/, f + 1, e + 1, id0);
+      } else {
+        const int64_t eP = (id0 - 1) / Np;
+        int fP = -1;
+        for (int c = 0; c < G::NF && fP < 0; ++c) {
+          bool all = true;
+          for (int k = 0; k < Nfp && all; ++k)
+            all = vp[k] == (int64_t)Np * eP + fmask<DIM, NQ>(c, k) + 1;
+          if (all) fP = c;
+        }
+        if (fP < 0)
+          p32_error(9 /*CANARY_ERR_ORIENTATION*/, f + 1, e + 1, id0);
+        else if (eP > (int64_t)(0x7fffffff / G::NF) - 1)
+          p32_error(11 /*CANARY_ERR_CAPACITY*/, f + 1, e + 1, eP + 1);
+        else
+          code = (int32_t)(G::NF * eP + fP);
+      }
+    } else if (bc != 1) {
+      p32_error(4 /*CANARY_ERR_BC*/, f + 1, e + 1, bc);
     }
-    nbr[g] = (int32_t)pn;
-    if (fn == 0) {
-      const int64_t bc = bndy[f + G::NF * e];
-      if (bc != 0 && bc != 1) p32_error(4, f + 1, e + 1, bc);
-      bnd[f + G::NF * e] = (int8_t)(bc == 0 ? 0 : 1);
-    }
+    nbr[g] = code;
   }
 }
 
@@ -210,11 +243,10 @@ __device__ __forceinline__ void load_face(const double* sg, size_t fo, float nn[
 template <int DIM, int NQ, int EPW, class GT = float, class RT = float>
 __device__ __forceinline__ void body(Lds32<DIM, NQ, EPW>& L, RT* __restrict__ rhs,
                                      const float* __restrict__ qin, float* __restrict__ qout,
-                                     const double* __restrict__ bg, const GT* __restrict__ vg,
+                                     const float* __restrict__ bg, const GT* __restrict__ vg,
                                      const GT* __restrict__ sg, const int32_t* __restrict__ nbr,
-                                     const int8_t* __restrict__ bnd, const double* __restrict__ D,
-                                     int64_t e0, int64_t nreal, double rka, double bdt,
-                                     const P32Params& p) {
+                                     const double* __restrict__ D, int64_t e0, int64_t nreal,
+                                     double rka, double bdt, const P32Params& p) {
   constexpr bool REG64 = sizeof(RT) == 8;
   using C = Cfg32<DIM, NQ, EPW>;
   using G = Dims<DIM>;
@@ -235,8 +267,7 @@ __device__ __forceinline__ void body(Lds32<DIM, NQ, EPW>& L, RT* __restrict__ rh
   const size_t fo = fvalid ? (size_t)ffi + (size_t)NFN * (size_t)(e0 + fsl) : (size_t)NFN * (size_t)e0;
   float fn4[4];
   load_face(sg, fo, fn4);
-  const int32_t pnode_l = nbr[fo];
-  const int fbc_l = bnd[(fvalid ? ffi / Nfp : 0) + NF * (fvalid ? e0 + fsl : e0)];
+  const int32_t code_l = nbr[(fvalid ? ffi / Nfp : 0) + NF * (fvalid ? e0 + fsl : e0)];
   const double dval = D[t < NQ * NQ ? t : 0];
   const int64_t el = valid ? e : e0;
   const int nl = valid ? n : 0;
@@ -247,10 +278,10 @@ __device__ __forceinline__ void body(Lds32<DIM, NQ, EPW>& L, RT* __restrict__ rh
   const GT* v = vg + (size_t)G::NV * Np * el;
 #pragma unroll
   for (int c = 0; c < DIM * DIM; ++c) met[c] = (float)v[nl + Np * c];
-  const float MJ = (float)v[nl + Np * DIM * DIM];
-  const double MJI = (double)v[nl + Np * (DIM * DIM + 1)];
-  const double* b = bg + (size_t)3 * Np * el;
-  const double b0 = b[nl], b1 = b[nl + Np], b2 = b[nl + 2 * Np];
+  const GT MJg = v[nl + Np * DIM * DIM];
+  const float MJ = (float)MJg;
+  const float* b = bg + (size_t)3 * Np * el;
+  const float b0 = b[nl], b1 = b[nl + Np], b2 = b[nl + 2 * Np];
   const RT* re = rhs + (size_t)NS * Np * el;
   // compact: the register is the FP32 accumulator; mixed: the FP64
   // register waits in rreg and the stage's RHS is summed from zero
@@ -259,14 +290,15 @@ __device__ __forceinline__ void body(Lds32<DIM, NQ, EPW>& L, RT* __restrict__ rh
   for (int s = 0; s < NS; ++s) rreg[s] = re[nl + Np * s];
 #pragma unroll
   for (int s = 0; s < NS; ++s) r[s] = REG64 ? 0.f : (float)rreg[s];
-  // the P-side gather of the first face item, as soon as its map is here
-  const int fbc = fvalid ? fbc_l : 1;
-  const bool gat = fvalid && fbc == 0;
-  const int32_t pnode = gat ? pnode_l : (int32_t)(e0 * Np);
+  // the P-side gather of the first face item, as soon as its face code is
+  // here: node fmask(fP, k) of element eP for face node k (code NF eP + fP)
+  const int fbc = fvalid && code_l >= 0 ? 0 : 1;
+  const bool gat = fbc == 0;
+  const int64_t eP = gat ? (int64_t)(code_l / NF) : e0;
+  const int pv = gat ? fmask<DIM, NQ>(code_l - NF * (code_l / NF), ffi % Nfp) : 0;
   float qP[NS];
   {
-    const float* qg = qin + (size_t)NS * Np * (size_t)(pnode / Np);
-    const int pv = pnode % Np;
+    const float* qg = qin + (size_t)NS * Np * (size_t)eP;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       const float g = qg[pv + Np * s];
@@ -275,7 +307,7 @@ __device__ __forceinline__ void body(Lds32<DIM, NQ, EPW>& L, RT* __restrict__ rh
   }
   if (t < NQ * NQ) L.D[t] = (float)dval;
   if (valid) {
-    const float rho0 = (float)b0, rt0 = (float)b1, P0 = (float)b2;
+    const float rho0 = b0, rt0 = b1, P0 = b2;
     const float Pp = pprime(P0, rt0, q[E], p.kappa);
 #pragma unroll
     for (int s = 0; s < NS; ++s) L.Q[slot][s * Np + n] = q[s];
@@ -372,15 +404,17 @@ __device__ __forceinline__ void body(Lds32<DIM, NQ, EPW>& L, RT* __restrict__ rh
     const size_t fo = (size_t)fi + (size_t)NFN * (size_t)(e0 + sl);
     float nn[4];
     load_face(sg, fo, nn);
-    const int bc = bnd[fi / Nfp + NF * (e0 + sl)];
+    const int32_t code = nbr[fi / Nfp + NF * (e0 + sl)];
+    const int bc = code >= 0 ? 0 : 1;
     float ql[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) ql[s] = 0.f;
     if (bc == 0) {
-      const int32_t pn = nbr[fo];
-      const float* qe = qin + (size_t)NS * Np * (size_t)(pn / Np);
+      const int32_t ep = code / NF;
+      const float* qe = qin + (size_t)NS * Np * (size_t)ep;
+      const int pn = fmask<DIM, NQ>(code - NF * ep, fi % Nfp);
 #pragma unroll
-      for (int s = 0; s < NS; ++s) ql[s] = qe[pn % Np + Np * s];
+      for (int s = 0; s < NS; ++s) ql[s] = qe[pn + Np * s];
     }
     face(sl, fi, nn, ql, bc);
   }
@@ -399,6 +433,7 @@ __device__ __forceinline__ void body(Lds32<DIM, NQ, EPW>& L, RT* __restrict__ rh
     lds_barrier();
   }
   if (valid) {  // non-temporal stores: not read again in this stage
+    const double MJI = 1.0 / (double)MJg;  // the reference's MJI is 1 ./ MJ
     float* qo = qout + (size_t)NS * Np * e;
     RT* ro = rhs + (size_t)NS * Np * e;
 #pragma unroll
@@ -425,29 +460,29 @@ constexpr int single_epw() {
 template <int DIM, int NQ>
 __global__ __launch_bounds__((Cfg32<DIM, NQ, single_epw<DIM, NQ>()>::NT)) void stage32_kernel(
     float* __restrict__ rhs, const float* __restrict__ qin, float* __restrict__ qout,
-    const double* __restrict__ bg, const float* __restrict__ vg, const float* __restrict__ sg,
-    const int32_t* __restrict__ nbr, const int8_t* __restrict__ bnd, const double* __restrict__ D,
-    int64_t nreal, float rka, double bdt, int64_t chunk, int reverse, P32Params p) {
+    const float* __restrict__ bg, const float* __restrict__ vg, const float* __restrict__ sg,
+    const int32_t* __restrict__ nbr, const double* __restrict__ D, int64_t nreal, float rka,
+    double bdt, int64_t chunk, int reverse, P32Params p) {
   constexpr int EPW = single_epw<DIM, NQ>();
   __shared__ Lds32<DIM, NQ, EPW> L;
   const int64_t e0 = stage_block(blockIdx.x, chunk, reverse) * EPW;
   if (e0 >= nreal) return;
-  body<DIM, NQ, EPW>(L, rhs, qin, qout, bg, vg, sg, nbr, bnd, D, e0, nreal, (double)rka, bdt, p);
+  body<DIM, NQ, EPW>(L, rhs, qin, qout, bg, vg, sg, nbr, D, e0, nreal, (double)rka, bdt, p);
 }
 
 // the mixed variant: FP64 geometry and LSRK register
 template <int DIM, int NQ>
 __global__ __launch_bounds__((Cfg32<DIM, NQ, single_epw<DIM, NQ>()>::NT)) void stage_mixed_kernel(
     double* __restrict__ rhs, const float* __restrict__ qin, float* __restrict__ qout,
-    const double* __restrict__ bg, const double* __restrict__ vg, const double* __restrict__ sg,
-    const int32_t* __restrict__ nbr, const int8_t* __restrict__ bnd, const double* __restrict__ D,
-    int64_t nreal, double rka, double bdt, int64_t chunk, int reverse, P32Params p) {
+    const float* __restrict__ bg, const double* __restrict__ vg, const double* __restrict__ sg,
+    const int32_t* __restrict__ nbr, const double* __restrict__ D, int64_t nreal, double rka,
+    double bdt, int64_t chunk, int reverse, P32Params p) {
   constexpr int EPW = single_epw<DIM, NQ>();
   __shared__ Lds32<DIM, NQ, EPW> L;
   const int64_t e0 = stage_block(blockIdx.x, chunk, reverse) * EPW;
   if (e0 >= nreal) return;
-  body<DIM, NQ, EPW, double, double>(L, rhs, qin, qout, bg, vg, sg, nbr, bnd, D, e0, nreal, rka,
-                                     bdt, p);
+  body<DIM, NQ, EPW, double, double>(L, rhs, qin, qout, bg, vg, sg, nbr, D, e0, nreal, rka, bdt,
+                                     p);
 }
 
 // batched: one workgroup per work-table entry, entries of one XCD contiguous
@@ -470,8 +505,8 @@ __global__ __launch_bounds__((Cfg32<DIM, NQ, EPW>::NT)) void batch32_kernel(
   }
   const P32Params p = params_of(sv.phys);
   body<DIM, NQ, EPW>(L, (float*)sv.rhs, (const float*)sv.q[parity], (float*)sv.q[1 - parity],
-                     (const double*)sv.bg, (const float*)sv.vg, (const float*)sv.sg, sv.nbr,
-                     sv.bnd, sv.D, it.e0, sv.nreal, rka, rkb * sv.dt, p);
+                     (const float*)sv.bg32, (const float*)sv.vg, (const float*)sv.sg, sv.nbr,
+                     sv.D, it.e0, sv.nreal, rka, rkb * sv.dt, p);
 }
 
 int launch_status() { return hipGetLastError() == hipSuccess ? 0 : CANARY_ERR_LAUNCH; }
@@ -479,15 +514,15 @@ int launch_status() { return hipGetLastError() == hipSuccess ? 0 : CANARY_ERR_LA
 template <int DIM, int NQ, class GT = float>
 int prepare(int64_t nelem, int64_t ntotal, const void* vgeo, const void* sgeo, const int64_t* vmapM,
             const int64_t* vmapP, const int64_t* bndy, const void* Q0, const void* Q,
-            const canary_phys* ph, void* vg, void* sg, int32_t* nbr, int8_t* bnd, void* bg, void* qp,
-            hipStream_t st) {
+            const canary_phys* ph, void* vg, void* sg, int32_t* nbr, void* bg, void* bg32,
+            void* qp, hipStream_t st) {
   const int64_t work = nelem * ipow(NQ, DIM);
   int64_t blocks = (work + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   prepare_kernel<DIM, NQ, GT><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
       nelem, ntotal, (const double*)vgeo, (const double*)sgeo, vmapM, vmapP, bndy,
       (const double*)Q0, (const double*)Q, ph->c_p / ph->c_v, ph->p0, ph->R_gas, (GT*)vg,
-      (GT*)sg, nbr, bnd, (double*)bg, (float*)qp);
+      (GT*)sg, nbr, (double*)bg, (float*)bg32, (float*)qp);
   return launch_status();
 }
 
@@ -495,40 +530,39 @@ template <int DIM, int NQ>
 int prepare_mixed(int64_t nelem, int64_t ntotal, const void* vgeo, const void* sgeo,
                   const int64_t* vmapM, const int64_t* vmapP, const int64_t* bndy, const void* Q0,
                   const void* Q, const canary_phys* ph, void* vg, void* sg, int32_t* nbr,
-                  int8_t* bnd, void* bg, void* qp, hipStream_t st) {
+                  void* bg, void* bg32, void* qp, hipStream_t st) {
   return prepare<DIM, NQ, double>(nelem, ntotal, vgeo, sgeo, vmapM, vmapP, bndy, Q0, Q, ph, vg, sg,
-                                  nbr, bnd, bg, qp, st);
+                                  nbr, bg, bg32, qp, st);
 }
 
 template <int DIM, int NQ>
 int stage_mixed(void* rhs, const void* qin, void* qout, const void* bg, const void* vg,
-                const void* sg, const int32_t* nbr, const int8_t* bnd, const double* D,
-                int64_t nreal, double rka, double rkb, double dt, const canary_phys* ph,
-                hipStream_t st) {
+                const void* sg, const int32_t* nbr, const double* D, int64_t nreal, double rka,
+                double rkb, double dt, const canary_phys* ph, hipStream_t st) {
   constexpr int EPW = single_epw<DIM, NQ>();
   using C = Cfg32<DIM, NQ, EPW>;
   const int64_t nb = (nreal + EPW - 1) / EPW;
   const int64_t chunk = (nb + 7) / 8;
   P32Params p{(float)ph->gamma, (float)(ph->c_p / ph->c_v), (float)ph->gravity};
   stage_mixed_kernel<DIM, NQ><<<dim3((unsigned)(8 * chunk)), dim3(C::NT), 0, st>>>(
-      (double*)rhs, (const float*)qin, (float*)qout, (const double*)bg, (const double*)vg,
-      (const double*)sg, nbr, bnd, D, nreal, rka, rkb * dt, chunk,
+      (double*)rhs, (const float*)qin, (float*)qout, (const float*)bg, (const double*)vg,
+      (const double*)sg, nbr, D, nreal, rka, rkb * dt, chunk,
       (uintptr_t)qin > (uintptr_t)qout ? 1 : 0, p);
   return launch_status();
 }
 
 template <int DIM, int NQ>
 int stage(void* rhs, const void* qin, void* qout, const void* bg, const void* vg, const void* sg,
-          const int32_t* nbr, const int8_t* bnd, const double* D, int64_t nreal, double rka,
-          double rkb, double dt, const canary_phys* ph, hipStream_t st) {
+          const int32_t* nbr, const double* D, int64_t nreal, double rka, double rkb, double dt,
+          const canary_phys* ph, hipStream_t st) {
   constexpr int EPW = single_epw<DIM, NQ>();
   using C = Cfg32<DIM, NQ, EPW>;
   const int64_t nb = (nreal + EPW - 1) / EPW;
   const int64_t chunk = (nb + 7) / 8;
   P32Params p{(float)ph->gamma, (float)(ph->c_p / ph->c_v), (float)ph->gravity};
   stage32_kernel<DIM, NQ><<<dim3((unsigned)(8 * chunk)), dim3(C::NT), 0, st>>>(
-      (float*)rhs, (const float*)qin, (float*)qout, (const double*)bg, (const float*)vg,
-      (const float*)sg, nbr, bnd, D, nreal, (float)rka, rkb * dt, chunk,
+      (float*)rhs, (const float*)qin, (float*)qout, (const float*)bg, (const float*)vg,
+      (const float*)sg, nbr, D, nreal, (float)rka, rkb * dt, chunk,
       (uintptr_t)qin > (uintptr_t)qout ? 1 : 0,  // q ping-pong: direction alternates
       p);
   return launch_status();
@@ -635,12 +669,12 @@ extern "C" int canary_perturb32_prepare(int dim, int N, int64_t nelem, int64_t n
                                         const void* vgeo, const void* sgeo, const int64_t* vmapM,
                                         const int64_t* vmapP, const int64_t* elemtobndy,
                                         const void* Q0, const void* Q, const canary_phys* phys,
-                                        void* vg32, void* sg32, int32_t* nbr, int8_t* bnd8,
-                                        void* bg, void* qp, void* stream) {
+                                        void* vg32, void* sg32, int32_t* nbr, void* bg,
+                                        void* bg32, void* qp, void* stream) {
   if (nelem < 0 || nelem_total < nelem) return CANARY_ERR_ARG;
   if (nelem == 0) return 0;
   if (!vgeo || !sgeo || !vmapM || !vmapP || !elemtobndy || !Q0 || !Q || !vg32 || !sg32 || !nbr ||
-      !bnd8 || !bg || !qp)
+      !bg || !bg32 || !qp)
     return CANARY_ERR_ARG;
   canary_phys d;
   if (!phys) {
@@ -649,17 +683,17 @@ extern "C" int canary_perturb32_prepare(int dim, int N, int64_t nelem, int64_t n
   }
   hipStream_t st = (hipStream_t)stream;
   P32_DISPATCH(dim, N, prepare, (nelem, nelem_total, vgeo, sgeo, vmapM, vmapP, elemtobndy, Q0, Q,
-                                 phys, vg32, sg32, nbr, bnd8, bg, qp, st))
+                                 phys, vg32, sg32, nbr, bg, bg32, qp, st))
 }
 
 extern "C" int canary_stage_perturb32(int dim, int N, void* rhs, const void* qin, void* qout,
-                                      const void* bg, const void* vg32, const void* sg32,
-                                      const int32_t* nbr, const int8_t* bnd8, const double* D,
-                                      int64_t nreal, double rka, double rkb, double dt,
-                                      const canary_phys* phys, void* stream) {
+                                      const void* bg32, const void* vg32, const void* sg32,
+                                      const int32_t* nbr, const double* D, int64_t nreal,
+                                      double rka, double rkb, double dt, const canary_phys* phys,
+                                      void* stream) {
   if (nreal < 0) return CANARY_ERR_ARG;
   if (nreal == 0) return 0;
-  if (!rhs || !qin || !qout || !bg || !vg32 || !sg32 || !nbr || !bnd8 || !D || qin == qout)
+  if (!rhs || !qin || !qout || !bg32 || !vg32 || !sg32 || !nbr || !D || qin == qout)
     return CANARY_ERR_ARG;
   if (((uintptr_t)sg32 & 15) != 0) return CANARY_ERR_ARG;  // read as float4 records
   canary_phys d;
@@ -669,7 +703,7 @@ extern "C" int canary_stage_perturb32(int dim, int N, void* rhs, const void* qin
   }
   hipStream_t st = (hipStream_t)stream;
   P32_DISPATCH(dim, N, stage,
-               (rhs, qin, qout, bg, vg32, sg32, nbr, bnd8, D, nreal, rka, rkb, dt, phys, st))
+               (rhs, qin, qout, bg32, vg32, sg32, nbr, D, nreal, rka, rkb, dt, phys, st))
 }
 
 extern "C" int canary_perturb_mixed_prepare(int dim, int N, int64_t nelem, int64_t nelem_total,
@@ -677,12 +711,12 @@ extern "C" int canary_perturb_mixed_prepare(int dim, int N, int64_t nelem, int64
                                            const int64_t* vmapM, const int64_t* vmapP,
                                            const int64_t* elemtobndy, const void* Q0,
                                            const void* Q, const canary_phys* phys, void* vg64,
-                                           void* sg64, int32_t* nbr, int8_t* bnd8, void* bg,
+                                           void* sg64, int32_t* nbr, void* bg, void* bg32,
                                            void* qp, void* stream) {
   if (nelem < 0 || nelem_total < nelem) return CANARY_ERR_ARG;
   if (nelem == 0) return 0;
   if (!vgeo || !sgeo || !vmapM || !vmapP || !elemtobndy || !Q0 || !Q || !vg64 || !sg64 || !nbr ||
-      !bnd8 || !bg || !qp)
+      !bg || !bg32 || !qp)
     return CANARY_ERR_ARG;
   canary_phys d;
   if (!phys) {
@@ -691,18 +725,17 @@ extern "C" int canary_perturb_mixed_prepare(int dim, int N, int64_t nelem, int64
   }
   hipStream_t st = (hipStream_t)stream;
   P32_DISPATCH(dim, N, prepare_mixed, (nelem, nelem_total, vgeo, sgeo, vmapM, vmapP, elemtobndy,
-                                       Q0, Q, phys, vg64, sg64, nbr, bnd8, bg, qp, st))
+                                       Q0, Q, phys, vg64, sg64, nbr, bg, bg32, qp, st))
 }
 
 extern "C" int canary_stage_perturb_mixed(int dim, int N, void* rhs64, const void* qin,
-                                          void* qout, const void* bg, const void* vg64,
-                                          const void* sg64, const int32_t* nbr,
-                                          const int8_t* bnd8, const double* D, int64_t nreal,
-                                          double rka, double rkb, double dt,
+                                          void* qout, const void* bg32, const void* vg64,
+                                          const void* sg64, const int32_t* nbr, const double* D,
+                                          int64_t nreal, double rka, double rkb, double dt,
                                           const canary_phys* phys, void* stream) {
   if (nreal < 0) return CANARY_ERR_ARG;
   if (nreal == 0) return 0;
-  if (!rhs64 || !qin || !qout || !bg || !vg64 || !sg64 || !nbr || !bnd8 || !D || qin == qout)
+  if (!rhs64 || !qin || !qout || !bg32 || !vg64 || !sg64 || !nbr || !D || qin == qout)
     return CANARY_ERR_ARG;
   if (((uintptr_t)sg64 & 15) != 0) return CANARY_ERR_ARG;  // read as double2 pairs
   canary_phys d;
@@ -712,7 +745,7 @@ extern "C" int canary_stage_perturb_mixed(int dim, int N, void* rhs64, const voi
   }
   hipStream_t st = (hipStream_t)stream;
   P32_DISPATCH(dim, N, stage_mixed,
-               (rhs64, qin, qout, bg, vg64, sg64, nbr, bnd8, D, nreal, rka, rkb, dt, phys, st))
+               (rhs64, qin, qout, bg32, vg64, sg64, nbr, D, nreal, rka, rkb, dt, phys, st))
 }
 
 extern "C" int canary_perturb32_reconstruct(int dim, int N, int64_t nelem, const void* qp,

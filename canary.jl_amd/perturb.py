@@ -46,11 +46,11 @@ class Perturb32:
         f32, f64 = torch.float32, torch.float64
         gw = f64 if self.mixed else f32  # geometry and LSRK register word
         dv = self.device
-        self.vg = torch.empty(Np * (d * d + 2) * ne, dtype=gw, device=dv)
+        self.vg = torch.empty(Np * (d * d + 1) * ne, dtype=gw, device=dv)
         self.sg = torch.empty(4 * Nfp * nf * ne, dtype=gw, device=dv)
-        self.nbr = torch.empty(Nfp * nf * ne, dtype=torch.int32, device=dv)
-        self.bnd = torch.empty(nf * ne, dtype=torch.int8, device=dv)
+        self.nbr = torch.empty(nf * ne, dtype=torch.int32, device=dv)
         self.bg = torch.empty(Np * 3 * ne, dtype=f64, device=dv)
+        self.bg32 = torch.empty(Np * 3 * ne, dtype=f32, device=dv)
         self.q = torch.empty(Np * ns * ne, dtype=f32, device=dv)
         self.q2 = torch.empty_like(self.q)
         self.rhs = torch.zeros(Np * ns * ne, dtype=gw, device=dv)
@@ -63,7 +63,7 @@ class Perturb32:
         prep = lib().canary_perturb_mixed_prepare if self.mixed else lib().canary_perturb32_prepare
         rc = prep(d, s.N, ne, ne, _ptr(vgeo), _ptr(sgeo), _ptr(vM), _ptr(vP), _ptr(bd), _ptr(q0),
                   _ptr(qf), C.byref(s.phys), _ptr(self.vg), _ptr(self.sg), _ptr(self.nbr),
-                  _ptr(self.bnd), _ptr(self.bg), _ptr(self.q), self._st())
+                  _ptr(self.bg), _ptr(self.bg32), _ptr(self.q), self._st())
         check(rc, "perturb prepare")
         self.check()
         del vgeo, sgeo, vM, vP, bd, q0, qf
@@ -82,8 +82,8 @@ class Perturb32:
     def stage(self, rka, rkb, dt):
         s = self.s
         fn = lib().canary_stage_perturb_mixed if self.mixed else lib().canary_stage_perturb32
-        rc = fn(s.dim, s.N, _ptr(self.rhs), _ptr(self.q), _ptr(self.q2), _ptr(self.bg),
-                _ptr(self.vg), _ptr(self.sg), _ptr(self.nbr), _ptr(self.bnd), _ptr(self.D),
+        rc = fn(s.dim, s.N, _ptr(self.rhs), _ptr(self.q), _ptr(self.q2), _ptr(self.bg32),
+                _ptr(self.vg), _ptr(self.sg), _ptr(self.nbr), _ptr(self.D),
                 s.nelem, float(rka), float(rkb), float(dt), C.byref(s.phys), self._st())
         check(rc, "perturb stage")
         self.q, self.q2 = self.q2, self.q
@@ -104,11 +104,11 @@ class Perturb32:
     @staticmethod
     def bytes_per_element_stage(N, dim=3, mixed=False):
         """Bytes one element moves per stage, every array counted once: q' r+w
-        (FP32), dq r+w, dim^2 + 2 metric fields and 4 face fields (FP32, or
-        FP64 in the mixed variant), the FP64 background (3 fields), the int32
-        P-side map and the int8 bc."""
+        (FP32), dq r+w, dim^2 + 1 metric fields (metric terms, MJ) and 4 face
+        fields (FP32, or FP64 in the mixed variant), the background's 3
+        fields in FP32 and one int32 face code per face."""
         Nq = N + 1
         Np, Nfp, nf, ns = Nq ** dim, Nq ** (dim - 1), 2 * dim, dim + 2
         g = 8 if mixed else 4
-        return (4 * 2 * ns * Np + g * (2 * ns * Np + (dim * dim + 2) * Np + nf * Nfp * 4)
-                + 8 * 3 * Np + 4 * nf * Nfp + nf)
+        return (4 * 2 * ns * Np + g * (2 * ns * Np + (dim * dim + 1) * Np + nf * Nfp * 4)
+                + 4 * 3 * Np + 4 * nf)

@@ -229,41 +229,48 @@ void canary_lsrk54_coefs(double* rka, double* rkb, double* rkc);
  * reference evolves Q in one precision (euler3d_set2c.jl:754-891,
  * euler2d_set2c.jl:639-775).  Arrays (column-major, per element; NS = dim+2,
  * NF = 2 dim):
- *   vg32 float (Np,dim*dim+2,nelem) metric, MJ, MJI   sg32 float (4,Nfp,NF,nelem) nx ny nz sMJ
- *   nbr  int32 (Nfp,NF,nelem) vmapP-1                  bnd8 int8 (NF,nelem)
- *   bg   double (Np,3,nelem) rho0, rhotheta0, P0       qp, rhs float (Np,NS,nelem)
+ *   vg32 float (Np,dim*dim+1,nelem) metric, MJ       sg32 float (4,Nfp,NF,nelem) nx ny nz sMJ
+ *   nbr  int32 (NF,nelem) face code: -1 wall, else NF*eP + fP (0-based
+ *        neighbour element eP and face fP)
+ *   bg   double (Np,3,nelem) rho0, rhotheta0, P0       bg32 float (Np,3,nelem) the same in FP32
+ *   qp, rhs float (Np,NS,nelem)
  * prepare: builds them from the FP64 drop-in arrays, the background Q0 and a
- * full state Q (qp = Q - Q0); requires the canonical vmapM of `mappings`. */
+ * full state Q (qp = Q - Q0).  It checks what the compact arrays leave out
+ * (vmapM the canonical face mask, each vmapP column the canonical mask of
+ * one neighbour face, elemtobndy 0 or 1, MJI == 1 ./ MJ) and records the
+ * first violation (CANARY_ERR_VMAP, _ORIENTATION, _BC, _ARG) for
+ * canary_check_errors.  The stage reads bg32 (the FP32 values its
+ * arithmetic uses); reconstruct reads bg. */
 int canary_perturb32_prepare(int dim, int N, int64_t nelem, int64_t nelem_total,
                              const void* vgeo, const void* sgeo, const int64_t* vmapM,
                              const int64_t* vmapP, const int64_t* elemtobndy, const void* Q0,
                              const void* Q, const canary_phys* phys, void* vg32, void* sg32,
-                             int32_t* nbr, int8_t* bnd8, void* bg, void* qp, void* stream);
-/* one fused LSRK stage of the perturbation (qin -> qout, rhs in place) */
+                             int32_t* nbr, void* bg, void* bg32, void* qp, void* stream);
+/* one fused LSRK stage of the perturbation (qin -> qout, rhs in place);
+ * MJI = 1 / MJ in the FP64 update */
 int canary_stage_perturb32(int dim, int N, void* rhs, const void* qin, void* qout,
-                           const void* bg, const void* vg32, const void* sg32,
-                           const int32_t* nbr, const int8_t* bnd8, const double* D,
-                           int64_t nreal, double rka, double rkb, double dt,
-                           const canary_phys* phys, void* stream);
-/* Q (FP64, (Np,NS,nelem)) = Q0 + qp (both variants) */
+                           const void* bg32, const void* vg32, const void* sg32,
+                           const int32_t* nbr, const double* D, int64_t nreal, double rka,
+                           double rkb, double dt, const canary_phys* phys, void* stream);
+/* Q (FP64, (Np,NS,nelem)) = Q0 + qp (both variants; bg the FP64 background) */
 int canary_perturb32_reconstruct(int dim, int N, int64_t nelem, const void* qp, const void* bg,
                                  void* Q, void* stream);
 /* Mixed variant: the same FP32 state q' and FP32 RHS arithmetic, with FP64
- * kept where the compact arrays round: metric terms, MJ, MJI
- *   vg64 double (Np,dim*dim+2,nelem),  sg64 double (4,Nfp,NF,nelem) nx ny nz sMJ,
+ * kept where the compact arrays round: metric terms and MJ
+ *   vg64 double (Np,dim*dim+1,nelem),  sg64 double (4,Nfp,NF,nelem) nx ny nz sMJ,
  * and the LSRK register rhs64 double (Np,NS,nelem) (dq = dq + L(q') and the
- * update q' + ((rkb*dt) dq) MJI in FP64, q' rounded to FP32 once).  No
- * reference counterpart (see the precision form above). */
+ * update q' + ((rkb*dt) dq) MJI in FP64 with MJI = 1 / MJ, the reference's
+ * value, q' rounded to FP32 once).  No reference counterpart (see the
+ * precision form above). */
 int canary_perturb_mixed_prepare(int dim, int N, int64_t nelem, int64_t nelem_total,
                                  const void* vgeo, const void* sgeo, const int64_t* vmapM,
                                  const int64_t* vmapP, const int64_t* elemtobndy, const void* Q0,
                                  const void* Q, const canary_phys* phys, void* vg64, void* sg64,
-                                 int32_t* nbr, int8_t* bnd8, void* bg, void* qp, void* stream);
+                                 int32_t* nbr, void* bg, void* bg32, void* qp, void* stream);
 int canary_stage_perturb_mixed(int dim, int N, void* rhs64, const void* qin, void* qout,
-                               const void* bg, const void* vg64, const void* sg64,
-                               const int32_t* nbr, const int8_t* bnd8, const double* D,
-                               int64_t nreal, double rka, double rkb, double dt,
-                               const canary_phys* phys, void* stream);
+                               const void* bg32, const void* vg64, const void* sg64,
+                               const int32_t* nbr, const double* D, int64_t nreal, double rka,
+                               double rkb, double dt, const canary_phys* phys, void* stream);
 
 /* ---- batched form ---------------------------------------------------------
  * Many independent solves of one equation set / dimension, each with its own
@@ -273,7 +280,10 @@ int canary_stage_perturb_mixed(int dim, int N, void* rhs64, const void* qin, voi
  * drop-in ones (FP64: prec = CANARY_FP64 in the reference evaluation order
  * or CANARY_FP64_FMA contracted, as for canary_stage_fused), or FP32 copies
  * of Q, rhs, vgeo and sgeo for prec = CANARY_FP32 (shallow water only).
- * Both tables live in device memory. */
+ * A solve in the prepared layout (canary_prepare_geometry's arrays) has
+ * pface set, pvgeo / psgeo in vgeo / sgeo (FP32 copies for CANARY_FP32),
+ * and vmapM, vmapP, elemtobndy unused; it runs through the *_prepared
+ * entries.  Both tables live in device memory. */
 typedef struct canary_batch_solve {
   void* Q[2];                  /* ping-pong states; stage reads Q[parity], writes Q[1-parity] */
   void* rhs;                   /* LSRK register                                  */
@@ -288,6 +298,7 @@ typedef struct canary_batch_solve {
   canary_phys phys;
   int32_t N;
   int32_t advection;           /* shallow water: keep the velocity (as updatesolution) */
+  const int32_t* pface;        /* prepared layout: face codes; NULL: reference arrays */
 } canary_batch_solve;
 
 typedef struct canary_batch_item {
@@ -325,17 +336,27 @@ int canary_batch_stage_all(int eqn, int dim, int prec, const canary_batch_solve*
                            const canary_batch_item* items, int64_t nitems, int parity,
                            double rka, double rkb, void* stream);
 
+/* The same two launch forms over solves in the prepared layout (every
+ * solve of the table with pface set; an entry of a solve without it is
+ * skipped and recorded as CANARY_ERR_ARG, as is a prepared solve's entry
+ * in canary_batch_stage / canary_batch_stage_all). */
+int canary_batch_stage_prepared(int eqn, int dim, int prec, int N, int epw,
+                                const canary_batch_solve* solves, const canary_batch_item* items,
+                                int64_t nitems, int parity, double rka, double rkb, void* stream);
+int canary_batch_stage_all_prepared(int eqn, int dim, int prec, const canary_batch_solve* solves,
+                                    const canary_batch_item* items, int64_t nitems, int parity,
+                                    double rka, double rkb, void* stream);
+
 /* Batched precision form: one solve of the table is the set of arrays of
  * the precision form above (q[parity] -> q[1-parity]); items as for
  * canary_batch_stage. */
 typedef struct canary_p32_solve {
   void* q[2];
   void* rhs;
-  const void* bg;
+  const void* bg32;
   const void* vg;
   const void* sg;
   const int32_t* nbr;
-  const int8_t* bnd;
   const double* D;
   int64_t nreal;
   double dt;
@@ -368,6 +389,8 @@ typedef struct canary_batch_class {
 #define CANARY_BATCH_DROPIN 0
 #define CANARY_BATCH_P32 1
 #define CANARY_BATCH_DROPIN_ALL 2 /* canary_batch_stage_all per class (N, epw ignored) */
+#define CANARY_BATCH_PREPARED 3   /* canary_batch_stage_prepared */
+#define CANARY_BATCH_PREPARED_ALL 4 /* canary_batch_stage_all_prepared (N, epw ignored) */
 int canary_batch_run(int kind, int eqn, int dim, int prec, const void* solves,
                      const canary_batch_class* classes, int nclasses, int64_t nsteps, int parity,
                      void* stream);

@@ -372,3 +372,72 @@ def test_single_launch_bad_entry_is_reported(cuda):
         b.check()
     assert ei.value.code == ERR_ARG
     assert max_ulp(b.state(i), before) == 0
+
+
+@pytest.mark.parametrize("eqn,dim,mixf", [(SWE, 2, swe_mix), (EULER, 2, euler2d_mix),
+                                          (EULER, 3, euler3d_mix)])
+def test_prepared_batch_same_bits(cuda, eqn, dim, mixf):
+    """Solves in the prepared layout (canary_prepare_geometry's arrays):
+    in the exact order the same bits as over the reference arrays, per order
+    and in one launch, from the library loop and from step graphs;
+    contracted within 1e-14 (another kernel instantiation may fuse other
+    products); FP32 shallow water within 1e-5 of FP64 (its MJI is 1/MJ in
+    FP32 here, the rounded FP64 MJI over the reference arrays)."""
+    mix = mixf()
+    print("prepared batch eqn=%d dim=%d N=%s" % (eqn, dim, [s.N for s, _ in mix]), flush=True)
+    for prec in ((FP64, FP64_FMA, FP32) if eqn == SWE else (FP64, FP64_FMA)):
+        ref = Batch(eqn, dim, prec)
+        rids = [ref.add(s, dt) for s, dt in mix]
+        ref.finalize()
+        ref.run(3, graph=False)
+        ref.check()
+        want = [ref.state(j) for j in rids]
+        del ref
+        ex = None
+        if prec == FP32:
+            ex = Batch(eqn, dim, FP64)
+            eids = [ex.add(s, dt) for s, dt in mix]
+            ex.finalize()
+            ex.run(3, graph=False)
+            ex = [ex.state(j) for j in eids]
+        for single in (False, True):
+            sub = [(k, s, dt) for k, (s, dt) in enumerate(mix) if not single or dim == 2 or
+                   s.N <= 5]
+            for graph in (False, True):
+                b = Batch(eqn, dim, prec, prepared=True)
+                ids = [b.add(s, dt) for _, s, dt in sub]
+                b.finalize(single_launch=single)
+                b.run(3, graph=graph)
+                b.check()
+                for (k, s, _), i in zip(sub, ids):
+                    got = b.state(i)
+                    tag = (prec, s.N, single, graph)
+                    if prec == FP64:
+                        assert max_ulp(got, want[k]) == 0, tag
+                    elif prec == FP64_FMA:
+                        assert rel_l2(got, want[k]) <= 1e-14, tag
+                    else:
+                        assert rel_l2(got[:, :3], ex[k][:, :3]) <= 1e-5, tag
+                del b
+        torch.cuda.empty_cache()
+
+
+def test_prepared_layout_must_match_the_entry(cuda):
+    """A prepared solve's entry launched through the reference-array entry
+    (and the other way round) is skipped and reported."""
+    s, dt = swe_mix()[0]
+    for prepared in (True, False):
+        b = Batch(SWE, 2, prepared=prepared)
+        i = b.add(s, dt)
+        b.finalize()
+        before = b.state(i)
+        items = np.zeros(1, dtype=ITEM_DTYPE)
+        items["solve"], items["N"], items["epw"], items["e0"] = 0, s.N, b.epw[s.N], 0
+        t = torch.from_numpy(items.view(np.uint8).copy()).cuda()
+        b.prepared = not prepared  # route through the other entry
+        b._launch(s.N, b.epw[s.N], t.data_ptr(), 1, b.parity, 0.0, 1.0, b._st())
+        b.prepared = prepared
+        with pytest.raises(CanaryError) as ei:
+            b.check()
+        assert ei.value.code == ERR_ARG
+        assert max_ulp(b.state(i), before) == 0

@@ -63,13 +63,44 @@ def test_perturb32_tracks_fp64(cuda, N, mixed):
         assert all(v <= 1e-5 for v in e.values()), e
 
 
-def test_perturb32_rejects_noncanonical_vmapM(cuda):
-    s = P.bubble3d(Ne=2, N=3, dtheta=2.0, jitter=0.0)
-    Q0 = P.bubble_state(s, dtheta=0.0, jitter=0.0)
-    s.vmapM = s.vmapM.copy(order="F")
-    s.vmapM[0, 0, 0] += 1
-    with pytest.raises(CanaryError):
-        Perturb32(s, Q0, s.Q)
+@pytest.mark.parametrize("mixed", [False, True])
+def test_perturb32_refuses_what_the_compact_arrays_drop(cuda, mixed):
+    """prepare checks every property the compact arrays rely on: vmapM the
+    canonical face mask, each vmapP column the canonical mask of one
+    neighbour face (one int32 code per face is kept), elemtobndy 0 or 1,
+    MJI == 1 ./ MJ (the stage divides)."""
+    from canary_amd._lib import ERR_ARG, ERR_BC, ERR_ORIENTATION, ERR_VMAP
+    base = P.bubble3d(Ne=2, N=3, dtheta=2.0, jitter=0.0)
+    Q0 = P.bubble_state(base, dtheta=0.0, jitter=0.0)
+
+    def expect(code, **change):
+        s = P.bubble3d(Ne=2, N=3, dtheta=2.0, jitter=0.0)
+        for name, fn in change.items():
+            a = getattr(s, name).copy(order="F")
+            fn(a, s)
+            setattr(s, name, a)
+        with pytest.raises(CanaryError) as ei:
+            Perturb32(s, Q0, s.Q, mixed=mixed)
+        assert ei.value.code == code, (change, ei.value.code)
+
+    def bump_M(a, s):
+        a[0, 0, 0] += 1
+
+    def flip_P(a, s):
+        e, f = next((e, f) for e in range(s.nelem) for f in range(6) if s.elemtobndy[f, e] == 0)
+        a[:, f, e] = a[::-1, f, e]
+
+    def bad_bc(a, s):
+        a[1, 0] = 2
+
+    def bad_MJI(a, s):
+        a[7, 10, 1] = np.nextafter(a[7, 10, 1], 0.0)  # field 10 = MJI
+
+    expect(ERR_VMAP, vmapM=bump_M)
+    expect(ERR_ORIENTATION, vmapP=flip_P)
+    expect(ERR_BC, elemtobndy=bad_bc)
+    expect(ERR_ARG, vgeo=bad_MJI)
+    Perturb32(base, Q0, base.Q, mixed=mixed).check()  # the untouched arrays prepare cleanly
 
 
 def state_errors(got, want, dim):
