@@ -141,9 +141,10 @@ def two_d_form(ne, N, steps, warmup):
 
 
 def sweep_form(steps=10, warmup=10):
-    """The measured loop (contracted fused stage, library LSRK54 loop) on the
-    3D bubble for N = 2..7, each at about 20 M DOF, with its roofline fraction
-    on the same byte rule."""
+    """The library LSRK54 loop over the contracted fused stage on the 3D
+    bubble for N = 2..7, each at about 20 M DOF, over the reference arrays
+    and over the prepared ones (the measured path), with the roofline
+    fraction of both on the reference-array byte rule B3(N)."""
     import numpy as np
     import torch
     import canary_amd.kernels as K
@@ -167,29 +168,40 @@ def sweep_form(steps=10, warmup=10):
             torch.zeros(s.Q.size, dtype=torch.float64, device=dev)
         torch.cuda.synchronize()
 
-        def loop(n):
+        prep = K.prepare_geometry(3, N, vgeo, sgeo, vM, vP, bnd, s.nelem)
+        K.check_errors()
+
+        def loop(n, prepared):
             with torch.cuda.stream(own.stream):
+                if prepared:
+                    return K.lsrk54_prepared(3, N, q, q2, r, prep, D, s.nelem, dt, n,
+                                             phys=s.phys, prec=FP64_FMA)
                 return K.lsrk54(3, N, q, r, vgeo, sgeo, D, s.nelem, vM, vP, bnd, dt, n,
                                 phys=s.phys, path=PATH_FUSED, prec=FP64_FMA, Qalt=q2)
-        loop(warmup)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(own.stream)
-        fin = loop(steps)
-        e1.record(own.stream)
-        torch.cuda.synchronize()
-        K.check_errors()
-        if not bool(torch.isfinite(fin).all()):
-            raise SystemExit("non-finite state in the sweep at N=%d" % N)
-        t = e0.elapsed_time(e1) / 1e3 / (5 * steps)
         dof = s.nelem * s.Np * 5
-        out.append({"N": N, "elements": s.nelem, "dof": dof, "us_per_stage": round(t * 1e6, 2),
-                    "gdof_stage_s": round(dof / t / 1e9, 3),
-                    "bytes_per_element_stage": algorithmic_bytes(3, N),
-                    "hbm_frac": round(algorithmic_bytes(3, N) * s.nelem / t / 1e9 / HBM_PEAK_GBS,
-                                      4)})
+        row = {"N": N, "elements": s.nelem, "dof": dof,
+               "bytes_per_element_stage": algorithmic_bytes(3, N),
+               "bytes_per_element_stage_prepared": prepared_bytes(3, N)}
+        for tag, prepared in (("", False), ("prepared_", True)):
+            loop(warmup, prepared)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(own.stream)
+            fin = loop(steps, prepared)
+            e1.record(own.stream)
+            torch.cuda.synchronize()
+            K.check_errors()
+            if not bool(torch.isfinite(fin).all()):
+                raise SystemExit("non-finite state in the sweep at N=%d" % N)
+            t = e0.elapsed_time(e1) / 1e3 / (5 * steps)
+            row[tag + "us_per_stage"] = round(t * 1e6, 2)
+            row[tag + "gdof_stage_s"] = round(dof / t / 1e9, 3)
+            row[tag + "hbm_frac"] = round(algorithmic_bytes(3, N) * s.nelem / t / 1e9
+                                          / HBM_PEAK_GBS, 4)
+            del fin
+        out.append(row)
         print("sweep N=%d: %s" % (N, out[-1]), file=sys.stderr, flush=True)
-        del vgeo, sgeo, D, vM, vP, bnd, q, q2, r, fin, s
+        del vgeo, sgeo, D, vM, vP, bnd, q, q2, r, s, prep
         torch.cuda.empty_cache()
     return {"workload": "3D Euler set2c rising bubble, about 20 M DOF per N, contracted fused "
                         "stage through canary_lsrk54", "steps": steps, "warmup": warmup,

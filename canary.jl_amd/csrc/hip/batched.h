@@ -52,6 +52,7 @@ __device__ __forceinline__ PhysParams params_of(const canary_phys& c) {
   p.swe_g = c.swe_gravity;
   p.dnl = c.delta_nl;
   p.ktab = nullptr;
+  p.ser = kPowSeries;
   return p;
 }
 

@@ -20,10 +20,27 @@
 
 namespace canary {
 
+// Coefficients of the contracted pressure's series (FastOps::powk): the
+// 2 atanh series of the logarithm, 1/19, 1/17, ..., 1/3, and the Taylor
+// series of the exponential, 1/14!, 1/13!, ..., 1/6!.
+constexpr int kLogTerms = 9, kExpTerms = 9;
+struct PowSeries {
+  double lg[kLogTerms];
+  double ex[kExpTerms];
+};
+constexpr PowSeries kPowSeries = {
+    {1.0 / 19, 1.0 / 17, 1.0 / 15, 1.0 / 13, 1.0 / 11, 1.0 / 9, 1.0 / 7, 1.0 / 5, 1.0 / 3},
+    {1.0 / 87178291200.0, 1.0 / 6227020800.0, 1.0 / 479001600.0, 1.0 / 39916800.0,
+     1.0 / 3628800.0, 1.0 / 362880.0, 1.0 / 40320.0, 1.0 / 5040.0, 1.0 / 720.0}};
+
 struct PhysParams {
   double gamma, p0, R, cp, cv, g, kappa;  // kappa = cp / cv
   double swe_g, dnl;
   const crpow::KappaTable* ktab;  // pow table of kappa (device memory), or null
+  // kPowSeries, carried as a kernel argument by the host-launched kernels:
+  // the coefficients then sit in scalar registers and each series step is
+  // one multiply-add (as literals, each double costs two vector moves)
+  PowSeries ser;
 };
 
 // Julia's max(a, b) for Float64: NaN if either is NaN, else the larger.
@@ -65,28 +82,17 @@ struct FastOps {
     if (!(x > 0.70710678118654752 && x < 1.4142135623730950)) return pow_by_exp_log(x, k);
     const double s = (x - 1) * recip(x + 1);
     const double z = s * s;
-    double a = 1.0 / 19;
-    a = fma(a, z, 1.0 / 17);
-    a = fma(a, z, 1.0 / 15);
-    a = fma(a, z, 1.0 / 13);
-    a = fma(a, z, 1.0 / 11);
-    a = fma(a, z, 1.0 / 9);
-    a = fma(a, z, 1.0 / 7);
-    a = fma(a, z, 1.0 / 5);
-    a = fma(a, z, 1.0 / 3);
+    const PowSeries& c = p.ser;
+    double a = c.lg[0];
+#pragma unroll
+    for (int i = 1; i < kLogTerms; ++i) a = fma(a, z, c.lg[i]);
     a = fma(a, z, 1.0);
     const double lx = (2 * s) * a;
     const double y = k * lx;
     if (!(fabs(y) <= 0.5)) return pow_by_exp_log(x, k);
-    double b = 1.0 / 87178291200.0;  // 1/14!
-    b = fma(b, y, 1.0 / 6227020800.0);
-    b = fma(b, y, 1.0 / 479001600.0);
-    b = fma(b, y, 1.0 / 39916800.0);
-    b = fma(b, y, 1.0 / 3628800.0);
-    b = fma(b, y, 1.0 / 362880.0);
-    b = fma(b, y, 1.0 / 40320.0);
-    b = fma(b, y, 1.0 / 5040.0);
-    b = fma(b, y, 1.0 / 720.0);
+    double b = c.ex[0];
+#pragma unroll
+    for (int i = 1; i < kExpTerms; ++i) b = fma(b, y, c.ex[i]);
     b = fma(b, y, 1.0 / 120.0);
     b = fma(b, y, 1.0 / 24.0);
     b = fma(b, y, 1.0 / 6.0);

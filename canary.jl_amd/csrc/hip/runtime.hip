@@ -129,6 +129,7 @@ canary::PhysParams to_params(const canary_phys* ph, void* stream = nullptr, int 
   p.swe_g = ph->swe_gravity;
   p.dnl = ph->delta_nl;
   p.ktab = eqn == CANARY_EULER ? canary::kappa_table(p.kappa, (hipStream_t)stream) : nullptr;
+  p.ser = canary::kPowSeries;
   return p;
 }
 
