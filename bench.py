@@ -66,15 +66,26 @@ def swe_bytes(N):
     return 8 * ((4 + 3 + 2 * 3) * Np + 6 * Np + 4 * Nfp * 3) + 2 * 4 * Nfp * 8 + 4 * 8
 
 
+def total_energy_bytes(N):
+    """Bytes per element-stage of the 2D total-energy equations by the B3
+    rule: Q and dQ r+w (4 states), the metric terms + MJ + MJI + y (7
+    vgeo fields), normals + sMJ, vmapM + vmapP and elemtobndy."""
+    Np, Nfp = (N + 1) ** 2, N + 1
+    return 8 * (16 * Np + 7 * Np + 4 * Nfp * 3) + 2 * 4 * Nfp * 8 + 4 * 8
+
+
 def two_d_form(ne, N, steps, warmup):
     """Single large 2D solves through the library's LSRK54 loop (fused
-    stage, launched from C): the Euler set2c isentropic vortex and shallow
-    water with a bump, ne^2 quads, contracted and exact-order arithmetic."""
+    stage, launched from C): the Euler set2c isentropic vortex, shallow
+    water with a bump and the total-energy example's rising bubble, ne^2
+    quads, contracted and exact-order arithmetic, over the reference arrays
+    and (not the total-energy form, whose pressure reads node heights the
+    prepared arrays do not hold) over the prepared ones."""
     import numpy as np
     import torch
     import canary_amd.kernels as K
     import canary_amd.problems as P
-    from canary_amd._lib import FP64, FP64_FMA, PATH_FUSED, SWE, EULER
+    from canary_amd._lib import FP64, FP64_FMA, PATH_FUSED, SWE, EULER, EULER_TE
     from canary_amd.batch import bytes_per_element_stage
     from canary_amd.streams import OwnedStream
 
@@ -86,23 +97,31 @@ def two_d_form(ne, N, steps, warmup):
     own = OwnedStream(dev)
     out = {}
     for name, make in (("euler2d_vortex", lambda: P.vortex2d(Ne=ne, N=N)),
-                       ("swe2d_bump", lambda: P.swe2d(Ne=ne, N=N))):
+                       ("swe2d_bump", lambda: P.swe2d(Ne=ne, N=N)),
+                       ("euler2d_total_energy_bubble", lambda: P.bubble2d_te(Ne=ne, N=N))):
         s = make()
-        eqn = EULER if name.startswith("euler") else SWE
-        dt = (0.5 * P.cfl(s, s.Q) / N ** np.sqrt(2)) if eqn == EULER else P.swe_courant(s, s.Q)
+        eqn = s.eqn
+        if eqn == EULER:
+            dt = 0.5 * P.cfl(s, s.Q) / N ** np.sqrt(2)
+        elif eqn == SWE:
+            dt = P.swe_courant(s, s.Q)
+        else:
+            dt = P.te_dt(s, s.Q)
         vgeo, sgeo, D = d(s.vgeo), d(s.sgeo), d(s.D)
         vM, vP, bnd = d(s.vmapM), d(s.vmapP), d(s.elemtobndy)
         Q0 = d(s.Q)
         dof = s.nelem * s.Np * (3 if eqn == SWE else 4)
-        nbytes = (swe_bytes(N) if eqn == SWE else algorithmic_bytes(2, N)) * s.nelem
-        res = {"workload": "%s, %d^2 quads, N=%d, periodic" % (s.label, ne, N), "dof": dof,
+        nbytes = {SWE: swe_bytes(N), EULER: algorithmic_bytes(2, N),
+                  EULER_TE: total_energy_bytes(N)}[eqn] * s.nelem
+        res = {"workload": "%s, %d^2 quads, N=%d" % (s.label, ne, N), "dof": dof,
                "bytes_per_element_stage": nbytes // s.nelem}
-        prep = K.prepare_geometry(2, N, vgeo, sgeo, vM, vP, bnd, s.nelem)
-        K.check_errors()
-        pbytes = bytes_per_element_stage(eqn, 2, N, prepared=True)
-        for tag, prec, prepared in (("fp64_fma", FP64_FMA, False), ("fp64", FP64, False),
-                                    ("fp64_fma_prepared", FP64_FMA, True),
-                                    ("fp64_prepared", FP64, True)):
+        forms = [("fp64_fma", FP64_FMA, False), ("fp64", FP64, False)]
+        if eqn != EULER_TE:
+            prep = K.prepare_geometry(2, N, vgeo, sgeo, vM, vP, bnd, s.nelem)
+            K.check_errors()
+            pbytes = bytes_per_element_stage(eqn, 2, N, prepared=True)
+            forms += [("fp64_fma_prepared", FP64_FMA, True), ("fp64_prepared", FP64, True)]
+        for tag, prec, prepared in forms:
             q, q2, r = Q0.clone(), torch.empty_like(Q0), torch.zeros_like(Q0)
             torch.cuda.synchronize()
 
@@ -131,8 +150,9 @@ def two_d_form(ne, N, steps, warmup):
                 res[tag]["hbm_frac_on_own_bytes"] = round(pbytes * s.nelem / t / 1e9
                                                           / HBM_PEAK_GBS, 4)
             del q, q2, r, fin
-        res["bytes_per_element_stage_prepared"] = pbytes
-        del prep
+        if eqn != EULER_TE:
+            res["bytes_per_element_stage_prepared"] = pbytes
+            del prep
         out[name] = res
         print("2D form %s: %s" % (name, res["fp64_fma"]), file=sys.stderr, flush=True)
         del vgeo, sgeo, D, vM, vP, bnd, Q0, s

@@ -14,7 +14,7 @@ LIBPATH = os.environ.get("CANARY_DG_LIB") or os.path.join(HERE, "libcanary_dg.so
 OK, ERR_ARG, ERR_UNSUPPORTED, ERR_LAUNCH, ERR_BC, ERR_VMAP = 0, 1, 2, 3, 4, 5
 ERR_NOCONVERGENCE, ERR_LOBATTO_N1, ERR_ORDERING, ERR_ORIENTATION = 6, 7, 8, 9
 ERR_NOMEM, ERR_CAPACITY = 10, 11
-EULER, SWE = 0, 1
+EULER, SWE, EULER_TE = 0, 1, 2  # EULER_TE: 2D total-energy equations (set3c)
 FP64, FP32, FP64_FMA = 0, 1, 2
 PATH_KERNELS, PATH_FUSED, PATH_FACE_UPDATE = 0, 1, 2
 REDUCE_WORK = 1024  # CANARY_REDUCE_WORK
@@ -90,6 +90,8 @@ _SIGS = {
     "canary_phys_default": (None, [C.POINTER(Phys)]),
     "canary_volumerhs": (INT, [INT, INT, INT, INT, P, P, P, P, I64, C.POINTER(Phys), P]),
     "canary_facerhs": (INT, [INT, INT, INT, INT, P, P, P, I64, I64, P, P, P, C.POINTER(Phys), P]),
+    "canary_facerhs_geo": (INT, [INT, INT, INT, INT, P, P, P, P, I64, I64, P, P, P,
+                                 C.POINTER(Phys), P]),
     "canary_updatesolution": (INT, [INT, INT, INT, INT, P, P, P, I64, D, D, D, INT, P]),
     "canary_stage_fused": (INT, [INT, INT, INT, INT, P, P, P, P, P, P, I64, I64, P, P, P,
                                  D, D, D, INT, C.POINTER(Phys), P]),

@@ -25,11 +25,12 @@ LIB = os.path.join(HERE, "libcanary_dg.so")
 
 HOST_SOURCES = ["host/quadrature.cpp", "host/mesh.cpp", "host/metric.cpp", "host/problems.cpp"]
 HIP_SOURCES = ["hip/runtime.hip", "hip/euler3d.hip", "hip/euler2d.hip", "hip/swe2d.hip",
+               "hip/euler2d_te.hip",
                "hip/batched.hip", "hip/geometry.hip", "hip/aux.hip", "hip/prep.hip",
                "hip/prepared.hip", "hip/driver.hip"]
 # kernels allowed to contract a*b+c into FMA
 HIP_FAST_SOURCES = ["hip/perturb32.hip", "hip/euler3d_fma.hip", "hip/euler2d_fma.hip",
-                    "hip/swe2d_fma.hip", "hip/batched_fma.hip"]
+                    "hip/swe2d_fma.hip", "hip/euler2d_te_fma.hip", "hip/batched_fma.hip"]
 
 ARCH = os.environ.get("CANARY_OFFLOAD_ARCH", "gfx950")
 

@@ -166,6 +166,7 @@ struct FaceItem {
   Real nrm[Phys::DIM];
   Real sMJ;
   Real qP[Phys::NS];
+  Real yP;  // height of the P node (equation sets whose pressure needs it)
   int64_t eP;
   int64_t idM, idP, bcode;  // map and bc entries as loaded (face_load)
   int vidM;  // -1: no contribution (an error was recorded)
@@ -322,14 +323,29 @@ __device__ __forceinline__ void face_decode(FaceItem<Phys>& it, int64_t e, int f
   it.bcode = wall ? 1 : 0;
 }
 
+// The height y of node v of element e (vgeo field iY), for the equation
+// sets whose pressure needs it.
+template <class Phys, int NQ>
+__device__ __forceinline__ typename Phys::Real node_y(const typename Phys::Real* __restrict__ vgeo,
+                                                      int64_t e, int v) {
+  if constexpr (NeedsY<Phys>::value) {
+    constexpr int Np = Shape<Phys, NQ>::Np;
+    return vgeo[(size_t)Phys::NVGEO * Np * (size_t)e + (size_t)Phys::iY * Np + v];
+  } else {
+    return typename Phys::Real(0);
+  }
+}
+
 template <class Phys, int NQ>
 __device__ __forceinline__ void face_gather(FaceItem<Phys>& it,
-                                            const typename Phys::Real* __restrict__ Q) {
+                                            const typename Phys::Real* __restrict__ Q,
+                                            const typename Phys::Real* __restrict__ vgeo) {
   constexpr int Np = Shape<Phys, NQ>::Np, NS = Phys::NS;
   if (it.vidM >= 0 && it.bc == 0) {
     const typename Phys::Real* qe = Q + (size_t)NS * Np * (size_t)it.eP;
 #pragma unroll
     for (int s = 0; s < NS; ++s) it.qP[s] = qe[it.vidP + Np * s];
+    it.yP = node_y<Phys, NQ>(vgeo, it.eP, it.vidP);
   }
 }
 
@@ -339,6 +355,7 @@ __device__ __forceinline__ void face_gather(FaceItem<Phys>& it,
 template <class Phys, int NQ>
 __device__ __forceinline__ void face_gather_any(FaceItem<Phys>& it,
                                                 const typename Phys::Real* __restrict__ Q,
+                                                const typename Phys::Real* __restrict__ vgeo,
                                                 bool valid, int64_t e_safe) {
   constexpr int Np = Shape<Phys, NQ>::Np, NS = Phys::NS;
   const bool ok = valid && it.vidM >= 0 && it.bc == 0;
@@ -347,9 +364,12 @@ __device__ __forceinline__ void face_gather_any(FaceItem<Phys>& it,
   typename Phys::Real g[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) g[s] = qe[v + Np * s];
-  if (ok)
+  const typename Phys::Real y = node_y<Phys, NQ>(vgeo, ok ? it.eP : e_safe, v);
+  if (ok) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) it.qP[s] = g[s];
+    it.yP = y;
+  }
 }
 
 // M-side state through qm_at(s, node) and pm_at(node, qM); out[NSR].
@@ -366,7 +386,7 @@ __device__ __forceinline__ int face_finish(FaceItem<Phys>& it, const PhysParams&
   Real PP = 0;
   if (it.bc == 0) {
     if (Phys::B_FROM_M) it.qP[NS - 1] = qm_at(NS - 1, it.vidP);
-    PP = Phys::HAS_P ? Phys::pressure(it.qP, p) : Real(0);
+    PP = Phys::HAS_P ? pressure_at<Phys>(it.qP, it.yP, p) : Real(0);
   } else {
     Phys::reflect(qM, PM, it.nrm, it.qP, PP);
   }
@@ -377,6 +397,7 @@ __device__ __forceinline__ int face_finish(FaceItem<Phys>& it, const PhysParams&
 // All three steps at once.
 template <class Phys, int NQ, class QM, class PMF>
 __device__ __forceinline__ int face_contribution(const typename Phys::Real* __restrict__ Q,
+                                                 const typename Phys::Real* __restrict__ vgeo,
                                                  const typename Phys::Real* __restrict__ sgeo,
                                                  const int64_t* __restrict__ vmapM,
                                                  const int64_t* __restrict__ vmapP,
@@ -385,7 +406,7 @@ __device__ __forceinline__ int face_contribution(const typename Phys::Real* __re
                                                  QM qm_at, PMF pm_at, typename Phys::Real* out) {
   FaceItem<Phys> it;
   face_meta<Phys, NQ>(it, sgeo, vmapM, vmapP, bndy, e, fi, ntotal);
-  face_gather<Phys, NQ>(it, Q);
+  face_gather<Phys, NQ>(it, Q, vgeo);
   return face_finish<Phys>(it, p, qm_at, pm_at, out);
 }
 
@@ -422,7 +443,7 @@ __global__ __launch_bounds__((VolumeCfg<Phys, NQ, EPW>::NT)) void volume_kernel(
 #pragma unroll
     for (int c = 0; c < DIM * DIM; ++c) met[c] = vg[n + Np * c];
     const double MJ = vg[n + Np * Phys::iMJ];
-    const double P = Phys::HAS_P ? Phys::pressure(q, p) : 0.0;
+    const double P = Phys::HAS_P ? pressure_at<Phys>(q, node_y<Phys, NQ>(vgeo, e, n), p) : 0.0;
     double F[DIM][NSR];
     Phys::volume_flux(q, P, met, MJ, p, F);
 #pragma unroll
@@ -550,6 +571,8 @@ __device__ __forceinline__ void stage_body(
   constexpr int NFN = S::NFN, Nfp = S::Nfp;
   constexpr int KF = (EPW * NFN + NT - 1) / NT;
   static_assert(!PREP || (VOLUME && UPDATE), "prepared arrays: the whole stage only");
+  static_assert(!NeedsY<Phys>::value || (PM_LDS && !PREP),
+                "a pressure that needs y: M side from LDS, reference arrays (which hold y)");
   // prepared arrays: psgeo in `sgeo`, the face codes in `vmapP`'s place
   const int32_t* __restrict__ pface = reinterpret_cast<const int32_t*>(vmapP);
   constexpr int KP = PF < KF ? PF : KF;
@@ -601,6 +624,8 @@ __device__ __forceinline__ void stage_body(
 #pragma unroll
   for (int s = 0; s < NS; ++s) q[s] = qe[nl + Np * s];
   const Real* vg = (VOLUME || UPDATE) ? vgeo + (size_t)NV * Np * (size_t)el : nullptr;
+  // the node's height, for a pressure that needs it (loaded with the rest)
+  const Real ynode = NeedsY<Phys>::value ? node_y<Phys, NQ>(vgeo, el, nl) : Real(0);
   Real MJ = 0;
   if constexpr (VOLUME) {
 #pragma unroll
@@ -624,7 +649,7 @@ __device__ __forceinline__ void stage_body(
       face_decode<Phys, NQ>(fit[k], fval[k] ? e0 + sl : e0, fval[k] ? item - sl * NFN : 0);
     face_resolve<Phys, NQ>(fit[k], fval[k] ? e0 + sl : e0, fval[k] ? item - sl * NFN : 0,
                            ntotal);
-    face_gather_any<Phys, NQ>(fit[k], Qin, fval[k], e0);
+    face_gather_any<Phys, NQ>(fit[k], Qin, vgeo, fval[k], e0);
   }
 
   if (VOLUME && t < NQ * NQ) sD[t] = (Real)dval;
@@ -632,7 +657,7 @@ __device__ __forceinline__ void stage_body(
   if (valid) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) sQ[slot * NS * Np + s * Np + n] = q[s];
-    const Real P = Phys::HAS_P ? Phys::pressure(q, p) : Real(0);
+    const Real P = Phys::HAS_P ? pressure_at<Phys>(q, ynode, p) : Real(0);
     if (SP) sP[slot * Np + n] = P;
     if constexpr (VOLUME) {
       Phys::volume_flux(q, P, met, MJ, p, Fv);
@@ -663,7 +688,10 @@ __device__ __forceinline__ void stage_body(
   // 4. face fluxes: M side from LDS, P side gathered
   auto pm_of = [&](int sl, int v, const Real* qm) -> Real {
     if (!Phys::HAS_P) return Real(0);
-    return SP ? sP[sl * Np + v] : Phys::pressure(qm, p);
+    if constexpr (SP)
+      return sP[sl * Np + v];
+    else
+      return Phys::pressure(qm, p);  // (a pressure that needs y always has SP)
   };
 #pragma unroll
   for (int k = 0; k < KP; ++k) {
@@ -690,10 +718,10 @@ __device__ __forceinline__ void stage_body(
       face_load_prep<Phys, NQ>(it, sgeo, pface, e0 + sl, fi);
       face_decode<Phys, NQ>(it, e0 + sl, fi);
       face_resolve<Phys, NQ>(it, e0 + sl, fi, ntotal);
-      face_gather<Phys, NQ>(it, Qin);
+      face_gather<Phys, NQ>(it, Qin, vgeo);
       vid = face_finish<Phys>(it, p, qm_at, pm_at, c);
     } else {
-      vid = face_contribution<Phys, NQ>(Qin, sgeo, vmapM, vmapP, bndy, e0 + sl, fi, ntotal, p,
+      vid = face_contribution<Phys, NQ>(Qin, vgeo, sgeo, vmapM, vmapP, bndy, e0 + sl, fi, ntotal, p,
                                         qm_at, pm_at, c);
     }
     sV[sl * NFN + fi] = vid < 0 ? NOVID : (VidT)vid;
@@ -806,19 +834,21 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, false>::NT)) void stage_pr
 // facerhs! alone (rhs -= sMJ * Rusanov flux of Q): the fused stage body with
 // its volume term and update switched off, so the face terms get the same
 // schedule (every load issued first, nodal Q and pressure staged in LDS for
-// the M side, non-temporal rhs stores) and the same bits.
+// the M side, non-temporal rhs stores) and the same bits.  vgeo is read only
+// by equation sets whose pressure needs the node height (else it may be
+// null).
 template <class Phys, int NQ, int EPW>
 __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, false>::NT)) void face_kernel(
-    double* __restrict__ rhs, const double* __restrict__ Q, const double* __restrict__ sgeo,
-    int64_t nreal, int64_t ntotal, const int64_t* __restrict__ vmapM,
-    const int64_t* __restrict__ vmapP, const int64_t* __restrict__ bndy, int64_t chunk,
-    int reverse, PhysParams p) {
+    double* __restrict__ rhs, const double* __restrict__ Q, const double* __restrict__ vgeo,
+    const double* __restrict__ sgeo, int64_t nreal, int64_t ntotal,
+    const int64_t* __restrict__ vmapM, const int64_t* __restrict__ vmapP,
+    const int64_t* __restrict__ bndy, int64_t chunk, int reverse, PhysParams p) {
   using C = FusedCfg<Phys, NQ, EPW, false>;
   __shared__ __attribute__((aligned(16))) char lds[C::LDS];
   const int64_t blk = stage_block(blockIdx.x, chunk, reverse);
   const int64_t e0 = blk * EPW;
   if (e0 >= nreal) return;  // whole workgroup leaves together
-  stage_body<Phys, NQ, EPW, C::NT, 1, true, false, false>(lds, rhs, Q, nullptr, nullptr, sgeo,
+  stage_body<Phys, NQ, EPW, C::NT, 1, true, false, false>(lds, rhs, Q, nullptr, vgeo, sgeo,
                                                           nullptr, e0, nreal, ntotal, vmapM,
                                                           vmapP, bndy, 0.0, 0.0, 0.0, p);
 }

@@ -63,8 +63,8 @@ int step(Run& r, int& cur) {
       rc = canary_volumerhs(r.eqn, r.dim, r.N, r.prec, r.rhs, q, r.vgeo, r.D, r.nreal, r.phys,
                             r.stream);
       if (!rc)
-        rc = canary_facerhs(r.eqn, r.dim, r.N, r.prec, r.rhs, q, r.sgeo, r.nreal, r.ntotal,
-                            r.vmapM, r.vmapP, r.bndy, r.phys, r.stream);
+        rc = canary_facerhs_geo(r.eqn, r.dim, r.N, r.prec, r.rhs, q, r.sgeo, r.vgeo, r.nreal,
+                                r.ntotal, r.vmapM, r.vmapP, r.bndy, r.phys, r.stream);
       if (!rc)
         rc = canary_updatesolution(r.eqn, r.dim, r.N, r.prec, r.rhs, q, r.vgeo, r.nreal, rka, rkb,
                                    r.dt, r.advection, r.stream);

@@ -84,6 +84,13 @@ void launch_stage_prep(int64_t nb, const KArgs& a) {
       walk_back(a.Q, a.Qout), a.phys, a.advection);
 }
 
+// The prepared arrays hold no node heights: equation sets whose pressure
+// needs them have no prepared stage.
+template <class Phys, int NQ, int EPW>
+void launch_stage_prep_if(int64_t nb, const KArgs& a) {
+  if constexpr (!NeedsY<Phys>::value) launch_stage_prep<Phys, NQ, EPW>(nb, a);
+}
+
 template <class Phys, int NQ, int EPW>
 void launch_face_update(int64_t nb, const KArgs& a) {
   constexpr int NT = FusedCfg<Phys, NQ, EPW, false>::NT;
@@ -106,8 +113,8 @@ int launch(int op, const KArgs& a) {
   if constexpr (STAGE_ONLY) {
     if (op == OP_STAGE)
       launch_stage<Phys, NQ, EPW>(nb, a);
-    else if (op == OP_STAGE_PREP)
-      launch_stage_prep<Phys, NQ, EPW>(nb, a);
+    else if (op == OP_STAGE_PREP && !NeedsY<Phys>::value)
+      launch_stage_prep_if<Phys, NQ, EPW>(nb, a);
     else if (op == OP_FACE_UPDATE)
       launch_face_update<Phys, NQ, EPW>(nb, a);
     else
@@ -127,8 +134,8 @@ int launch(int op, const KArgs& a) {
         constexpr int NT = FusedCfg<Phys, NQ, EPW, false>::NT;
         const int64_t chunk = (nb + 7) / 8;
         face_kernel<Phys, NQ, EPW><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
-            rhs, Q, (const double*)a.sgeo, a.nreal, a.ntotal, a.vmapM, a.vmapP, a.bndy, chunk,
-            inplace_direction(), a.phys);
+            rhs, Q, (const double*)a.vgeo, (const double*)a.sgeo, a.nreal, a.ntotal, a.vmapM,
+            a.vmapP, a.bndy, chunk, inplace_direction(), a.phys);
         break;
       }
       case OP_UPDATE: {
@@ -144,7 +151,8 @@ int launch(int op, const KArgs& a) {
         launch_stage<Phys, NQ, EPW>(nb, a);
         break;
       case OP_STAGE_PREP:
-        launch_stage_prep<Phys, NQ, EPW>(nb, a);
+        if (NeedsY<Phys>::value) return 2 /*CANARY_ERR_UNSUPPORTED*/;
+        launch_stage_prep_if<Phys, NQ, EPW>(nb, a);
         break;
       case OP_FACE_UPDATE:
         launch_face_update<Phys, NQ, EPW>(nb, a);

@@ -250,6 +250,98 @@ struct Euler2DT {
 struct Euler2D : Euler2DT<ExactOps> {};
 struct Euler2DFma : Euler2DT<FastOps> {};
 
+// ------------------------------------------- 2D Euler, total energy (set3c)
+// Q = (U, V, rho, E) with E the total energy (internal + kinetic +
+// potential); the pressure needs the node's height y (vgeo field 7):
+// P = (R/cv) (E - (U^2 + V^2) / (2 rho) - rho g y) (euler2d_set3c.jl:226,
+// :308, :320), the energy flux is U (E + P) / rho (:233-238).  The face
+// kernel reads y at the P node through vgeo (:318-320).
+template <class Ops>
+struct Euler2DTET {
+  using Real = double;
+  static constexpr int DIM = 2, NS = 4, NSR = 4, NVGEO = 8, NSGEO = 4;
+  static constexpr int iMJ = 4, iMJI = 5, iY = 7;
+  static constexpr bool HAS_P = true, B_FROM_M = false, HAS_ADVECTION = false, NEEDS_Y = true;
+  enum { U = 0, V = 1, RHO = 2, E = 3 };
+
+  __device__ static double pressure_y(const double* q, double y, const PhysParams& p) {
+    const double Uq = q[U], Vq = q[V], rho = q[RHO];
+    return (p.R / p.cv) * (q[E] - (Uq * Uq + Vq * Vq) / (2 * rho) - rho * p.g * y);
+  }
+  __device__ static void volume_flux(const double* q, double P, const double* met, double MJ,
+                                     const PhysParams&, double F[DIM][NSR]) {
+    const double Uq = q[U], Vq = q[V], rho = q[RHO], Eq = q[E];
+    const double ri = Ops::recip(rho);
+    const double EP = Eq + P;
+    const double frx = Uq, fUx = ri * Uq * Uq + P, fVx = ri * Vq * Uq, fEx = ri * Uq * EP;
+    const double fry = Vq, fUy = ri * Uq * Vq, fVy = ri * Vq * Vq + P, fEy = ri * Vq * EP;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const double mx = met[d], my = met[d + 2];
+      F[d][RHO] = MJ * (mx * frx + my * fry);
+      F[d][U] = MJ * (mx * fUx + my * fUy);
+      F[d][V] = MJ * (mx * fVx + my * fVy);
+      F[d][E] = MJ * (mx * fEx + my * fEy);
+    }
+  }
+  __device__ static void source(const double* q, double MJ, const PhysParams& p, double* r) {
+    r[V] -= MJ * q[RHO] * p.g;
+  }
+  __device__ static void reflect(const double* qM, double PM, const double* n, double* qP, double& PP) {
+    const double UnM = n[0] * qM[U] + n[1] * qM[V];
+    qP[U] = qM[U] - 2 * UnM * n[0];
+    qP[V] = qM[V] - 2 * UnM * n[1];
+    qP[RHO] = qM[RHO];
+    qP[E] = qM[E];
+    PP = PM;
+  }
+  __device__ static void face_flux(const double* qM, double PM, const double* qP, double PP,
+                                   const double* n, double sMJ, const PhysParams& p, double* out) {
+    const double nx = n[0], ny = n[1];
+    const double UM = qM[U], VM = qM[V], rM = qM[RHO], EM = qM[E];
+    const double UP = qP[U], VP = qP[V], rP = qP[RHO], EP = qP[E];
+    const double rMi = Ops::recip(rM);
+    const double EPM = EM + PM, EPP = EP + PP;
+    const double frMx = UM, fUMx = rMi * UM * UM + PM, fVMx = rMi * VM * UM, fEMx = rMi * UM * EPM;
+    const double frMy = VM, fUMy = rMi * UM * VM, fVMy = rMi * VM * VM + PM, fEMy = rMi * VM * EPM;
+    const double rPi = Ops::recip(rP);
+    const double frPx = UP, fUPx = rPi * UP * UP + PP, fVPx = rPi * VP * UP, fEPx = rPi * UP * EPP;
+    const double frPy = VP, fUPy = rPi * UP * VP, fVPy = rPi * VP * VP + PP, fEPy = rPi * VP * EPP;
+    const double lM = rMi * fabs(nx * UM + ny * VM) + sqrt(rMi * p.gamma * PM);
+    const double lP = rPi * fabs(nx * UP + ny * VP) + sqrt(rPi * p.gamma * PP);
+    const double l = jmax(lM, lP);
+    out[RHO] = sMJ * ((nx * (frMx + frPx) + ny * (frMy + frPy) + -(l * (rP - rM))) / 2);
+    out[U] = sMJ * ((nx * (fUMx + fUPx) + ny * (fUMy + fUPy) + -(l * (UP - UM))) / 2);
+    out[V] = sMJ * ((nx * (fVMx + fVPx) + ny * (fVMy + fVPy) + -(l * (VP - VM))) / 2);
+    out[E] = sMJ * ((nx * (fEMx + fEPx) + ny * (fEMy + fEPy) + -(l * (EP - EM))) / 2);
+  }
+};
+
+struct Euler2DTE : Euler2DTET<ExactOps> {};
+struct Euler2DTEFma : Euler2DTET<FastOps> {};
+
+// Does the pressure of this equation set need the node's height y?
+template <class Phys, class = void>
+struct NeedsY {
+  static constexpr bool value = false;
+};
+template <class Phys>
+struct NeedsY<Phys, decltype((void)Phys::NEEDS_Y)> {
+  static constexpr bool value = Phys::NEEDS_Y;
+};
+
+// The pressure at a node of height y (ignored where the pressure does not
+// depend on it).
+template <class Phys>
+__device__ __forceinline__ typename Phys::Real pressure_at(const typename Phys::Real* q,
+                                                           typename Phys::Real y,
+                                                           const PhysParams& p) {
+  if constexpr (NeedsY<Phys>::value)
+    return Phys::pressure_y(q, y, p);
+  else
+    return Phys::pressure(q, p);
+}
+
 // ------------------------------------------------------------ shallow water
 // Q = (H u, H v, h, b); b (bathymetry / still depth) has no tendency.
 struct SWE2D {

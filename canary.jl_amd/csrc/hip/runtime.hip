@@ -13,6 +13,10 @@ extern "C" {
 int canary_k_euler3d(int op, int N, const canary::KArgs* a);
 int canary_k_euler2d(int op, int N, const canary::KArgs* a);
 int canary_k_swe2d(int op, int N, const canary::KArgs* a);
+int canary_k_euler2d_te(int op, int N, const canary::KArgs* a);
+int canary_k_euler2d_te_fma(int op, int N, const canary::KArgs* a);
+int canary_k_euler2d_te_status(int64_t* info, int clear);
+int canary_k_euler2d_te_fma_status(int64_t* info, int clear);
 int canary_k_euler3d_fma(int op, int N, const canary::KArgs* a);
 int canary_k_euler2d_fma(int op, int N, const canary::KArgs* a);
 int canary_k_swe2d_fma(int op, int N, const canary::KArgs* a);
@@ -109,6 +113,8 @@ KFn select(int eqn, int dim, int prec) {
   if (eqn == CANARY_EULER && dim == 3) return fma ? canary_k_euler3d_fma : canary_k_euler3d;
   if (eqn == CANARY_EULER && dim == 2) return fma ? canary_k_euler2d_fma : canary_k_euler2d;
   if (eqn == CANARY_SWE && dim == 2) return fma ? canary_k_swe2d_fma : canary_k_swe2d;
+  if (eqn == CANARY_EULER_TE && dim == 2)
+    return fma ? canary_k_euler2d_te_fma : canary_k_euler2d_te;
   return nullptr;
 }
 
@@ -166,19 +172,22 @@ extern "C" int canary_volumerhs(int eqn, int dim, int N, int prec, void* rhs, co
   return fn(canary::OP_VOLUME, N, &a);
 }
 
-extern "C" int canary_facerhs(int eqn, int dim, int N, int prec, void* rhs, const void* Q,
-                              const void* sgeo, int64_t nreal, int64_t nelem_total,
-                              const int64_t* vmapM, const int64_t* vmapP,
-                              const int64_t* elemtobndy, const canary_phys* phys, void* stream) {
+extern "C" int canary_facerhs_geo(int eqn, int dim, int N, int prec, void* rhs, const void* Q,
+                                  const void* sgeo, const void* vgeo, int64_t nreal,
+                                  int64_t nelem_total, const int64_t* vmapM,
+                                  const int64_t* vmapP, const int64_t* elemtobndy,
+                                  const canary_phys* phys, void* stream) {
   KFn fn;
   int rc = check_common(eqn, dim, N, prec, nreal, &fn);
   if (rc) return rc;
   if (nreal == 0) return 0;
   if (nelem_total < nreal) return CANARY_ERR_ARG;
   if (!rhs || !Q || !sgeo || !vmapP || !elemtobndy) return CANARY_ERR_ARG;
+  if (eqn == CANARY_EULER_TE && !vgeo) return CANARY_ERR_ARG;  // the pressure reads y
   canary::KArgs a = {};
   a.rhs = rhs;
   a.Q = Q;
+  a.vgeo = vgeo;
   a.sgeo = sgeo;
   a.nreal = nreal;
   a.ntotal = nelem_total;
@@ -188,6 +197,14 @@ extern "C" int canary_facerhs(int eqn, int dim, int N, int prec, void* rhs, cons
   a.phys = to_params(phys, stream, eqn);
   a.stream = (hipStream_t)stream;
   return fn(canary::OP_FACE, N, &a);
+}
+
+extern "C" int canary_facerhs(int eqn, int dim, int N, int prec, void* rhs, const void* Q,
+                              const void* sgeo, int64_t nreal, int64_t nelem_total,
+                              const int64_t* vmapM, const int64_t* vmapP,
+                              const int64_t* elemtobndy, const canary_phys* phys, void* stream) {
+  return canary_facerhs_geo(eqn, dim, N, prec, rhs, Q, sgeo, nullptr, nreal, nelem_total, vmapM,
+                            vmapP, elemtobndy, phys, stream);
 }
 
 extern "C" int canary_updatesolution(int eqn, int dim, int N, int prec, void* rhs, void* Q,
@@ -322,11 +339,14 @@ extern "C" int canary_check_errors(void* stream, int64_t* info) {
   if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return CANARY_ERR_LAUNCH;
   int64_t tmp[4];
   int first = 0;
-  int (*readers[12])(int64_t*, int) = {
-      canary_k_euler3d_status,     canary_k_euler2d_status,     canary_k_swe2d_status,
-      canary_k_euler3d_fma_status, canary_k_euler2d_fma_status, canary_k_swe2d_fma_status,
-      canary_k_perturb32_status,   canary_k_batch_status,       canary_k_batch_fma_status,
-      canary_k_geometry_status,    canary_k_aux_status,         canary_k_prepared_status};
+  int (*readers[14])(int64_t*, int) = {
+      canary_k_euler3d_status,        canary_k_euler2d_status,
+      canary_k_swe2d_status,          canary_k_euler2d_te_status,
+      canary_k_euler3d_fma_status,    canary_k_euler2d_fma_status,
+      canary_k_swe2d_fma_status,      canary_k_euler2d_te_fma_status,
+      canary_k_perturb32_status,      canary_k_batch_status,
+      canary_k_batch_fma_status,      canary_k_geometry_status,
+      canary_k_aux_status,            canary_k_prepared_status};
   for (auto rd : readers) {
     int rc = rd(tmp, 1);
     if (rc && !first) {

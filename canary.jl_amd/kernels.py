@@ -15,9 +15,9 @@ import ctypes as C
 import torch
 
 from . import _lib
-from ._lib import EULER, SWE, FP64, FP64_FMA, check, lib
+from ._lib import EULER, EULER_TE, SWE, FP64, FP64_FMA, check, lib
 
-_NS = {(EULER, 3): 5, (EULER, 2): 4, (SWE, 2): 4}
+_NS = {(EULER, 3): 5, (EULER, 2): 4, (SWE, 2): 4, (EULER_TE, 2): 4}
 _NV = {3: 14, 2: 8}
 _NSG = {3: 5, 2: 4}
 
@@ -77,8 +77,10 @@ def volumerhs(dim, N, rhs, Q, vgeo, D, elems, eqn=EULER, phys=None, stream=None)
 
 
 def facerhs(dim, N, rhs, Q, sgeo, elems, vmapM, vmapP, elemtobndy, eqn=EULER, phys=None,
-            nelem_total=None, stream=None):
-    """facerhs!(Val(dim), Val(N), rhs, Q, sgeo, elems, vmapM, vmapP, elemtobndy)."""
+            nelem_total=None, stream=None, vgeo=None):
+    """facerhs!(Val(dim), Val(N), rhs, Q, sgeo, elems, vmapM, vmapP, elemtobndy);
+    the total-energy equations' fluxrhs! also takes vgeo (its pressure reads
+    the node height), of all nelem_total elements."""
     n = _nreal(elems)
     Np, Nfp, nf = (N + 1) ** dim, (N + 1) ** (dim - 1), 2 * dim
     ns = _NS[(eqn, dim)]
@@ -89,9 +91,13 @@ def facerhs(dim, N, rhs, Q, sgeo, elems, vmapM, vmapP, elemtobndy, eqn=EULER, ph
     pM = _mapM(vmapM, Nfp * nf * n)
     _need(vmapP, "vmapP", Nfp * nf * n, torch.int64)
     _need(elemtobndy, "elemtobndy", nf * n, torch.int64)
-    rc = lib().canary_facerhs(eqn, dim, N, FP64, _ptr(rhs), _ptr(Q), _ptr(sgeo), n, ntot,
-                              pM, _ptr(vmapP), _ptr(elemtobndy), _phys(phys),
-                              _stream(stream))
+    if eqn == EULER_TE and vgeo is None:
+        raise ValueError("the total-energy face term needs vgeo (node heights)")
+    if vgeo is not None:
+        _need(vgeo, "vgeo", Np * _NV[dim] * ntot)
+    rc = lib().canary_facerhs_geo(eqn, dim, N, FP64, _ptr(rhs), _ptr(Q), _ptr(sgeo),
+                                  None if vgeo is None else _ptr(vgeo), n, ntot, pM,
+                                  _ptr(vmapP), _ptr(elemtobndy), _phys(phys), _stream(stream))
     check(rc, "facerhs")
 
 

@@ -13,7 +13,7 @@ from . import _lib
 from . import mesh as M
 from ._lib import check, lib
 
-NSTATE = {(_lib.EULER, 3): 5, (_lib.EULER, 2): 4, (_lib.SWE, 2): 4}
+NSTATE = {(_lib.EULER, 3): 5, (_lib.EULER, 2): 4, (_lib.SWE, 2): 4, (_lib.EULER_TE, 2): 4}
 NVGEO = {3: 14, 2: 8}
 NSGEO = {3: 5, 2: 4}
 IMJ = {3: 9, 2: 4}
@@ -360,6 +360,68 @@ def bubble2d(Ne=10, N=4, L=1000.0, gravity=10.0, dtheta=0.5, jitter=0.0, seed=1)
     s.Q = bubble_state(s, dtheta=dtheta, center=(500.0, 350.0), jitter=jitter, seed=seed)
     s.Q0 = bubble_state(s, dtheta=0.0, center=(500.0, 350.0))
     return s
+
+
+def bubble2d_te(Ne=10, N=4, L=1000.0, gravity=10.0, theta_c=0.5, rc=250.0, center=(500.0, 350.0),
+                theta_ref=300.0):
+    """The 2D rising bubble of the total-energy example (euler2d_set3c.jl
+    main :1474-1556: [0, L]^2, periodic in x, walls in y): ic :1489-1515 in
+    (rho, u, v, theta), then convert_set2nc_to_set3c :1306-1329 to
+    (rho u, rho v, rho, E) with E = rho (c_v T + (u^2 + v^2)/2 + g y).  Q0 is
+    the same state without the bubble."""
+    phys = _lib.default_phys(gravity=gravity)
+    x = linrange(0.0, L, Ne + 1)
+    s = build(_lib.EULER_TE, 2, N, (x, x), (True, False), phys=phys,
+              label="euler2d total-energy bubble %d^2 N=%d" % (Ne, N))
+    s.Q = te_bubble_state(s, theta_c=theta_c, rc=rc, center=center, theta_ref=theta_ref)
+    s.Q0 = te_bubble_state(s, theta_c=0.0, rc=rc, center=center, theta_ref=theta_ref)
+    return s
+
+
+def te_bubble_state(s, theta_c=0.5, rc=250.0, center=(500.0, 350.0), theta_ref=300.0):
+    """ic + convert_set2nc_to_set3c of euler2d_set3c.jl on a setup's nodes."""
+    ph = s.phys
+    x, y = s.vgeo[:, 6, :], s.vgeo[:, 7, :]
+    r = np.sqrt((x - center[0]) ** 2 + (y - center[1]) ** 2)
+    dth = np.where(r <= rc, 0.5 * theta_c * (1.0 + np.cos(np.pi * r / rc)), 0.0)
+    th = theta_ref + dth
+    pi_k = 1.0 - ph.gravity / (ph.c_p * th) * y
+    rho = ph.p0 / (ph.R_gas * th) * pi_k ** (ph.c_v / ph.R_gas)
+    u = np.zeros_like(rho)
+    v = np.zeros_like(rho)
+    P = ph.p0 * (rho * ph.R_gas * th / ph.p0) ** (ph.c_p / ph.c_v)
+    T = P / (rho * ph.R_gas)
+    E = ph.c_v * T + 0.5 * (u ** 2 + v ** 2) + ph.gravity * y
+    Q = s.zeros_state()
+    Q[:, 0, :], Q[:, 1, :], Q[:, 2, :], Q[:, 3, :] = rho * u, rho * v, rho, rho * E
+    return Q
+
+
+def te_dt(s, Q):
+    """courantnumber's time step of the total-energy example
+    (euler2d_set3c.jl:121-148): min over nodes of dx / (|u| + c) / N."""
+    ph = s.phys
+    U, V, rho, E = (Q[:, k, :] for k in range(4))
+    xix, etay, y = s.vgeo[:, 0, :], s.vgeo[:, 3, :], s.vgeo[:, 7, :]
+    P = (ph.R_gas / ph.c_v) * (E - (U * U + V * V) / (2 * rho) - rho * ph.gravity * y)
+    u, v = U / rho, V / rho
+    dx = np.sqrt((1.0 / (2 * xix)) ** 2 + (1.0 / (2 * etay)) ** 2)
+    ws = np.sqrt(u * u + v * v) + np.sqrt(ph.gamma * P / rho)
+    return float(np.min(1.0 * dx / ws / s.N))
+
+
+def te_to_set2nc(s, Q):
+    """convert_set3c_to_set2nc (euler2d_set3c.jl:1331-1356): (u, v, rho,
+    theta) from the total-energy state, for the energy norm and plots."""
+    ph = s.phys
+    U, V, rho, E = (np.array(Q[:, k, :]) for k in range(4))
+    y = s.vgeo[:, 7, :]
+    u, v, e = U / rho, V / rho, E / rho
+    P = (ph.R_gas / ph.c_v) * rho * (e - 0.5 * (u * u + v * v) - ph.gravity * y)
+    th = ph.p0 / (rho * ph.R_gas) * (P / ph.p0) ** (ph.c_v / ph.c_p)
+    out = np.empty_like(np.asarray(Q), order="F")
+    out[:, 0, :], out[:, 1, :], out[:, 2, :], out[:, 3, :] = u, v, rho, th
+    return out
 
 
 def vortex_phys(gamma=1.4):

@@ -51,6 +51,7 @@ extern "C" {
 /* ---- equation sets, precisions ----------------------------------------- */
 #define CANARY_EULER 0   /* set2c: (rho u, rho v[, rho w], rho, rho theta)  */
 #define CANARY_SWE 1     /* (H u, H v, h, b), b read only                    */
+#define CANARY_EULER_TE 2 /* 2D total energy (set3c): (rho u, rho v, rho, E)  */
 #define CANARY_FP64 0
 #define CANARY_FP32 1
 /* FP64 with a*b+c contracted into fused multiply-adds and 1/rho as a
@@ -91,7 +92,8 @@ void canary_phys_default(canary_phys* p);
 /* volumerhs!(Val(dim), Val(N), rhs, Q, vgeo, D, elems[, gravity, δnl])
  *   replaces euler3d_set2c.jl:713-718 (CPU :144-231),
  *            euler2d_set2c.jl:598-603 (CPU :140-205),
- *            shallow_water.jl:560-563 (CPU :121-173).
+ *            shallow_water.jl:560-563 (CPU :121-173),
+ *            euler2d_set3c.jl:975-980 (CPU :214-279; eqn CANARY_EULER_TE).
  *   rhs += weak-form volume term (+ buoyancy for Euler). */
 int canary_volumerhs(int eqn, int dim, int N, int prec, void* rhs, const void* Q,
                      const void* vgeo, const double* D, int64_t nreal,
@@ -106,6 +108,16 @@ int canary_facerhs(int eqn, int dim, int N, int prec, void* rhs, const void* Q,
                    const void* sgeo, int64_t nreal, int64_t nelem_total,
                    const int64_t* vmapM, const int64_t* vmapP,
                    const int64_t* elemtobndy, const canary_phys* phys, void* stream);
+/* fluxrhs!(Val(dim), Val(N), rhs, Q, sgeo, vgeo, elems, vmapM, vmapP,
+ * elemtobndy) of the total-energy equations (eqn = CANARY_EULER_TE)
+ *   replaces euler2d_set3c.jl:982-988 (CPU :282-378): the pressure reads
+ *   vgeo's y at the M and P nodes, so vgeo (of all nelem_total elements) is
+ *   required.  For the other equation sets vgeo may be NULL and the call is
+ *   canary_facerhs. */
+int canary_facerhs_geo(int eqn, int dim, int N, int prec, void* rhs, const void* Q,
+                       const void* sgeo, const void* vgeo, int64_t nreal, int64_t nelem_total,
+                       const int64_t* vmapM, const int64_t* vmapP, const int64_t* elemtobndy,
+                       const canary_phys* phys, void* stream);
 
 /* updatesolution!(Val(dim), Val(N), rhs, Q, vgeo, elems, rka, rkb, dt[, advection])
  *   replaces euler3d_set2c.jl:728-734 (CPU :352-358),

@@ -17,6 +17,13 @@
  *              volumerhs! :121-173, fluxrhs! :178-263 (including its bP
  *              read from element eM, :202), updatesolution! :267-298
  *              (all four states, optional velocity freeze), cfl :43-67
+ *   euler2dte_* examples/2d_kernels/euler2d_set3c.jl (total energy E)
+ *              volumerhs! :214-279, fluxrhs! :282-378 (the pressure reads
+ *              vgeo's y at the M and P nodes), courantnumber :121-168; the
+ *              update is the device one, knl_updatesolution! :886-901 (no
+ *              viscous term: the CPU update adds visc * rhs_gradQ, which at
+ *              the example's visc = 0 can only turn a -0.0 rhs entry into
+ *              +0.0, and the viscous LDG operators are not on this path)
  * Layouts are the reference's column-major ones (see include/canary_dg.h).
  * The per-face-node heap slice sgeo[:, n, f, e] of the Julia face kernel
  * (:251) is not reproduced; it does not change any value.
@@ -383,6 +390,168 @@ int oracle_euler2d_facerhs(int N, double* rhs, const double* Q, const double* sg
   return bad ? RET_BADBC : 0;
 }
 
+/* ------------------------------------------- 2D Euler, total energy (set3c) */
+enum { YV2 = 7 };
+
+/* P = (R/cv) (E - (U^2 + V^2) / (2 rho) - rho g y)  (euler2d_set3c.jl:226) */
+static double te_pressure(double R_gas, double c_v, double gravity, double U, double V, double rho,
+                          double E, double y) {
+  return (R_gas / c_v) * (E - (U * U + V * V) / (2 * rho) - rho * gravity * y);
+}
+
+int oracle_euler2dte_volumerhs(int N, double* rhs, const double* Q, const double* vgeo,
+                               const double* D, int64_t nelem, const ophys* ph, int nthreads) {
+  const int Nq = N + 1, Np = Nq * Nq, ns = 4, nv = 8;
+  const double R_gas = ph->R_gas, c_v = ph->c_v, gravity = ph->gravity;
+  set_threads(nthreads);
+#pragma omp parallel
+  {
+    double* sF = (double*)malloc(sizeof(double) * Np * ns);
+    double* sG = (double*)malloc(sizeof(double) * Np * ns);
+#pragma omp for schedule(static)
+    for (int64_t e = 0; e < nelem; ++e) {
+      const double* q = Q + (size_t)Np * ns * e;
+      const double* vg = vgeo + (size_t)Np * nv * e;
+      double* r = rhs + (size_t)Np * ns * e;
+      for (int j = 0; j < Nq; ++j)
+        for (int i = 0; i < Nq; ++i) {
+          int n = i + Nq * j;
+          double MJ = vg[n + Np * MJ2];
+          double xix = vg[n + Np * XIX2], xiy = vg[n + Np * XIY2];
+          double etax = vg[n + Np * ETAX2], etay = vg[n + Np * ETAY2];
+          double y = vg[n + Np * YV2];
+          double U = q[n + Np * U2], V = q[n + Np * V2], rho = q[n + Np * R2], E = q[n + Np * E2];
+          double P = te_pressure(R_gas, c_v, gravity, U, V, rho, E, y);
+          double rinv = 1 / rho;
+          double frx = U, fUx = rinv * U * U + P, fVx = rinv * V * U, fEx = rinv * U * (E + P);
+          double fry = V, fUy = rinv * U * V, fVy = rinv * V * V + P, fEy = rinv * V * (E + P);
+          sF[n + Np * R2] = MJ * (xix * frx + xiy * fry);
+          sF[n + Np * U2] = MJ * (xix * fUx + xiy * fUy);
+          sF[n + Np * V2] = MJ * (xix * fVx + xiy * fVy);
+          sF[n + Np * E2] = MJ * (xix * fEx + xiy * fEy);
+          sG[n + Np * R2] = MJ * (etax * frx + etay * fry);
+          sG[n + Np * U2] = MJ * (etax * fUx + etay * fUy);
+          sG[n + Np * V2] = MJ * (etax * fVx + etay * fVy);
+          sG[n + Np * E2] = MJ * (etax * fEx + etay * fEy);
+          r[n + Np * V2] -= MJ * rho * gravity;
+        }
+      for (int s = 0; s < ns; ++s)
+        for (int j = 0; j < Nq; ++j)
+          for (int i = 0; i < Nq; ++i)
+            for (int k = 0; k < Nq; ++k)
+              r[i + Nq * j + Np * s] += D[k + Nq * i] * sF[k + Nq * j + Np * s];
+      for (int s = 0; s < ns; ++s)
+        for (int j = 0; j < Nq; ++j)
+          for (int i = 0; i < Nq; ++i)
+            for (int k = 0; k < Nq; ++k)
+              r[i + Nq * j + Np * s] += D[k + Nq * j] * sG[i + Nq * k + Np * s];
+    }
+    free(sF);
+    free(sG);
+  }
+  return 0;
+}
+
+int oracle_euler2dte_fluxrhs(int N, double* rhs, const double* Q, const double* sgeo,
+                             const double* vgeo, int64_t nelem, const int64_t* vmapM,
+                             const int64_t* vmapP, const int64_t* elemtobndy, const ophys* ph,
+                             int nthreads) {
+  const int Nq = N + 1, Np = Nq * Nq, Nfp = Nq, nface = 4, ns = 4, nv = 8;
+  const double gamma = ph->gamma, R_gas = ph->R_gas, c_v = ph->c_v, gravity = ph->gravity;
+  int bad = 0;
+  set_threads(nthreads);
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (int64_t e = 0; e < nelem; ++e) {
+    for (int f = 0; f < nface; ++f) {
+      int64_t bc = elemtobndy[f + nface * e];
+      if (bc != 0 && bc != 1) {
+        bad = 1;
+        continue;
+      }
+      for (int n = 0; n < Nfp; ++n) {
+        size_t fo = n + (size_t)Nfp * (f + (size_t)nface * e);
+        const double* sg = sgeo + 4 * fo;
+        double nxM = sg[0], nyM = sg[1], sMJ = sg[2];
+        int64_t idM = vmapM[fo], idP = vmapP[fo];
+        int64_t eM = e, eP = (idP - 1) / Np;
+        int64_t vidM = (idM - 1) % Np, vidP = (idP - 1) % Np;
+        const double* qM = Q + (size_t)Np * ns * eM;
+        const double* qP = Q + (size_t)Np * ns * eP;
+        double rhoM = qM[vidM + Np * R2], UM = qM[vidM + Np * U2], VM = qM[vidM + Np * V2],
+               EM = qM[vidM + Np * E2];
+        double yM = vgeo[vidM + (size_t)Np * (YV2 + (size_t)nv * eM)];
+        double PM = te_pressure(R_gas, c_v, gravity, UM, VM, rhoM, EM, yM);
+        double rhoP, UP, VP, EP, PP;
+        if (bc == 0) {
+          rhoP = qP[vidP + Np * R2];
+          UP = qP[vidP + Np * U2];
+          VP = qP[vidP + Np * V2];
+          EP = qP[vidP + Np * E2];
+          double yP = vgeo[vidP + (size_t)Np * (YV2 + (size_t)nv * eP)];
+          PP = te_pressure(R_gas, c_v, gravity, UP, VP, rhoP, EP, yP);
+        } else {
+          double UnM = nxM * UM + nyM * VM;
+          UP = UM - 2 * UnM * nxM;
+          VP = VM - 2 * UnM * nyM;
+          rhoP = rhoM;
+          EP = EM;
+          PP = PM;
+        }
+        double rMi = 1 / rhoM;
+        double frMx = UM, fUMx = rMi * UM * UM + PM, fVMx = rMi * VM * UM,
+               fEMx = rMi * UM * (EM + PM);
+        double frMy = VM, fUMy = rMi * UM * VM, fVMy = rMi * VM * VM + PM,
+               fEMy = rMi * VM * (EM + PM);
+        double rPi = 1 / rhoP;
+        double frPx = UP, fUPx = rPi * UP * UP + PP, fVPx = rPi * VP * UP,
+               fEPx = rPi * UP * (EP + PP);
+        double frPy = VP, fUPy = rPi * UP * VP, fVPy = rPi * VP * VP + PP,
+               fEPy = rPi * VP * (EP + PP);
+        double lM = rMi * fabs(nxM * UM + nyM * VM) + sqrt(rMi * gamma * PM);
+        double lP = rPi * fabs(nxM * UP + nyM * VP) + sqrt(rPi * gamma * PP);
+        double l = lM > lP ? lM : lP;
+        if (isnan(lM) || isnan(lP)) l = NAN;
+        double frS = (nxM * (frMx + frPx) + nyM * (frMy + frPy) + -(l * (rhoP - rhoM))) / 2;
+        double fUS = (nxM * (fUMx + fUPx) + nyM * (fUMy + fUPy) + -(l * (UP - UM))) / 2;
+        double fVS = (nxM * (fVMx + fVPx) + nyM * (fVMy + fVPy) + -(l * (VP - VM))) / 2;
+        double fES = (nxM * (fEMx + fEPx) + nyM * (fEMy + fEPy) + -(l * (EP - EM))) / 2;
+        double* r = rhs + (size_t)Np * ns * eM;
+        r[vidM + Np * R2] -= sMJ * frS;
+        r[vidM + Np * U2] -= sMJ * fUS;
+        r[vidM + Np * V2] -= sMJ * fVS;
+        r[vidM + Np * E2] -= sMJ * fES;
+      }
+    }
+  }
+  return bad ? RET_BADBC : 0;
+}
+
+/* courantnumber's time step (euler2d_set3c.jl:121-148): the minimum over
+ * nodes of dx / (|u| + sqrt(gamma P / rho)) / N, dx = sqrt((1/(2 xi_x))^2 +
+ * (1/(2 eta_y))^2). */
+double oracle_euler2dte_dt(int N, const double* vgeo, const double* Q, int64_t nelem,
+                           const ophys* ph) {
+  const int Np = (N + 1) * (N + 1), ns = 4, nv = 8;
+  const double gamma = ph->gamma, R_gas = ph->R_gas, c_v = ph->c_v, gravity = ph->gravity;
+  double dt = 1.7976931348623157e308;
+  for (int64_t e = 0; e < nelem; ++e)
+    for (int n = 0; n < Np; ++n) {
+      const double* q = Q + (size_t)Np * ns * e;
+      const double* vg = vgeo + (size_t)Np * nv * e;
+      double rho = q[n + Np * R2], U = q[n + Np * U2], V = q[n + Np * V2], E = q[n + Np * E2];
+      double xix = vg[n + Np * XIX2], etay = vg[n + Np * ETAY2], y = vg[n + Np * YV2];
+      double P = te_pressure(R_gas, c_v, gravity, U, V, rho, E, y);
+      double u = U / rho, v = V / rho;
+      double a = 1.0 / (2 * xix), b = 1.0 / (2 * etay);
+      double dx = sqrt(a * a + b * b);
+      double vel = sqrt(u * u + v * v);
+      double ws = vel + sqrt(gamma * P / rho);
+      double loc = 1.0 * dx / ws / N;
+      if (loc < dt) dt = loc;
+    }
+  return dt;
+}
+
 /* ------------------------------------------------------------- shallow water */
 enum { SU = 0, SV, SH, SB };
 
@@ -555,7 +724,8 @@ void oracle_lsrk54_coefs(double* rka, double* rkb, double* rkc) {
   rkc[4] = (double)2802321613138 / (double)2924317926251;
 }
 
-/* eqn: 0 = Euler (dim 2 or 3), 1 = SWE (dim 2).  nsteps full LSRK54 steps of
+/* eqn: 0 = Euler (dim 2 or 3), 1 = SWE (dim 2), 2 = Euler total energy (dim
+ * 2).  nsteps full LSRK54 steps of
  * the reference stage loop (volume; face; update with RKA[s%5+1], RKB[s]). */
 int oracle_lsrk(int eqn, int dim, int N, double* Q, double* rhs, const double* vgeo,
                 const double* sgeo, const double* D, int64_t nelem, const int64_t* vmapM,
@@ -574,6 +744,11 @@ int oracle_lsrk(int eqn, int dim, int N, double* Q, double* rhs, const double* v
       } else if (eqn == 0 && dim == 2) {
         oracle_euler2d_volumerhs(N, rhs, Q, vgeo, D, nelem, ph, nthreads);
         rc = oracle_euler2d_facerhs(N, rhs, Q, sgeo, nelem, vmapM, vmapP, elemtobndy, ph, nthreads);
+        oracle_updatesolution(Np, 4, 8, MJI2, rhs, Q, vgeo, nelem, A[(s + 1) % 5], B[s], dt, nthreads);
+      } else if (eqn == 2 && dim == 2) {
+        oracle_euler2dte_volumerhs(N, rhs, Q, vgeo, D, nelem, ph, nthreads);
+        rc = oracle_euler2dte_fluxrhs(N, rhs, Q, sgeo, vgeo, nelem, vmapM, vmapP, elemtobndy, ph,
+                                      nthreads);
         oracle_updatesolution(Np, 4, 8, MJI2, rhs, Q, vgeo, nelem, A[(s + 1) % 5], B[s], dt, nthreads);
       } else if (eqn == 1 && dim == 2) {
         oracle_swe_volumerhs(N, rhs, Q, vgeo, D, nelem, ph, nthreads);

@@ -34,8 +34,12 @@ def lib():
         L = C.CDLL(LIB)
         V, I64, D, INT = C.c_void_p, C.c_int64, C.c_double, C.c_int
         PP = C.POINTER(OPhys)
-        for name in ("oracle_euler3d_volumerhs", "oracle_euler2d_volumerhs", "oracle_swe_volumerhs"):
+        for name in ("oracle_euler3d_volumerhs", "oracle_euler2d_volumerhs", "oracle_swe_volumerhs",
+                     "oracle_euler2dte_volumerhs"):
             getattr(L, name).argtypes = [INT, V, V, V, V, I64, PP, INT]
+        L.oracle_euler2dte_fluxrhs.argtypes = [INT, V, V, V, V, I64, V, V, V, PP, INT]
+        L.oracle_euler2dte_dt.argtypes = [INT, V, V, I64, PP]
+        L.oracle_euler2dte_dt.restype = D
         for name in ("oracle_euler3d_facerhs", "oracle_euler2d_facerhs", "oracle_swe_fluxrhs"):
             getattr(L, name).argtypes = [INT, V, V, V, I64, V, V, V, PP, INT]
         L.oracle_updatesolution.argtypes = [INT, INT, INT, INT, V, V, V, I64, D, D, D, INT]
@@ -79,7 +83,7 @@ def ophys(phys):
 
 
 def _tag(s):
-    return {(0, 3): "euler3d", (0, 2): "euler2d", (1, 2): "swe"}[(s.eqn, s.dim)]
+    return {(0, 3): "euler3d", (0, 2): "euler2d", (1, 2): "swe", (2, 2): "euler2dte"}[(s.eqn, s.dim)]
 
 
 def volumerhs(s, rhs, Q, nthreads=1, nelem=None):
@@ -90,9 +94,13 @@ def volumerhs(s, rhs, Q, nthreads=1, nelem=None):
 
 def facerhs(s, rhs, Q, nthreads=1, nelem=None, elemtobndy=None, vmapP=None):
     n = s.nelem if nelem is None else nelem
-    name = "oracle_swe_fluxrhs" if s.eqn == 1 else "oracle_%s_facerhs" % _tag(s)
     bnd = s.elemtobndy if elemtobndy is None else elemtobndy
     vP = s.vmapP if vmapP is None else vmapP
+    if s.eqn == 2:  # total energy: fluxrhs! reads vgeo's y (euler2d_set3c.jl:282)
+        return lib().oracle_euler2dte_fluxrhs(s.N, _p(rhs), _p(Q), _p(s.sgeo), _p(s.vgeo), n,
+                                              _p(s.vmapM), _p(vP), _p(bnd),
+                                              C.byref(ophys(s.phys)), nthreads)
+    name = "oracle_swe_fluxrhs" if s.eqn == 1 else "oracle_%s_facerhs" % _tag(s)
     return getattr(lib(), name)(s.N, _p(rhs), _p(Q), _p(s.sgeo), n, _p(s.vmapM), _p(vP),
                                 _p(bnd), C.byref(ophys(s.phys)), nthreads)
 
@@ -133,6 +141,11 @@ def lsrk54_coefs():
 def l2energysquared(s, Q):
     imj = 9 if s.dim == 3 else 4
     return lib().oracle_l2energysquared(s.Np, s.nstate, s.nvgeo, imj, _p(Q), _p(s.vgeo), s.nelem)
+
+
+def euler2dte_dt(s, Q):
+    """courantnumber's time step of the total-energy example."""
+    return lib().oracle_euler2dte_dt(s.N, _p(s.vgeo), _p(Q), s.nelem, C.byref(ophys(s.phys)))
 
 
 def euler_cfl(s, Q):

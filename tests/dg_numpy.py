@@ -19,8 +19,18 @@ signs, index maps and face normals are right.
 import numpy as np
 
 
-def _fluxes(eqn, dim, q, phys):
-    """Physical flux f[c][s] (c = x, y[, z]) at every node; q (Np, ns, nelem)."""
+def _fluxes(eqn, dim, q, phys, y=None):
+    """Physical flux f[c][s] (c = x, y[, z]) at every node; q (Np, ns, nelem);
+    y the node heights (the total-energy pressure needs them)."""
+    if eqn == 2:  # total energy (set3c): (U, V, rho, E)
+        U, V, rho, E = (q[:, k, :] for k in range(4))
+        P = (phys.R_gas / phys.c_v) * (E - 0.5 * (U * U + V * V) / rho - rho * phys.gravity * y)
+        out = []
+        for c, m in enumerate((U, V)):
+            u = m / rho
+            mom = [U * u + (P if c == 0 else 0.0), V * u + (P if c == 1 else 0.0)]
+            out.append(mom + [m, (E + P) * u])
+        return out, P
     if eqn == 0:
         if dim == 3:
             U, V, W, rho, E = (q[:, k, :] for k in range(5))
@@ -77,7 +87,8 @@ def rhs_strong(s, Q, include_source=True):
     else:
         Wt = np.einsum("j,i->ij", w, w).reshape(-1, order="F")
     J = MJ / Wt[:, None]
-    f, P = _fluxes(eqn, dim, Q, s.phys)
+    ynode = vg[:, 7, :] if eqn == 2 else None
+    f, P = _fluxes(eqn, dim, Q, s.phys, ynode)
     rhs = np.zeros((Np, Q.shape[1], Q.shape[2]), order="F")
     for st in range(nsr):
         div = np.zeros_like(J)
@@ -85,7 +96,7 @@ def rhs_strong(s, Q, include_source=True):
             G = J * sum(vg[:, d + dim * c, :] * f[c][st] for c in range(dim))
             div += _deriv(s.D, G, d, dim, Nq)
         rhs[:, st, :] = -Wt[:, None] * div
-    if include_source and eqn == 0:
+    if include_source and eqn in (0, 2):
         iW = dim - 1
         irho = dim
         rhs[:, iW, :] -= MJ * Q[:, irho, :] * s.phys.gravity
@@ -112,14 +123,18 @@ def rhs_strong(s, Q, include_source=True):
         qP[wall] = qM[wall]
         qP[wall, mom] = qM[wall][:, mom] - 2 * un[:, None] * nrm.T[wall]
 
-    def fl(qq):
-        ff, PP = _fluxes(eqn, dim, qq[:, :, None], s.phys)
+    yflat = vg[:, 7, :].reshape(-1, order="F") if eqn == 2 else None
+
+    def fl(qq, yy=None):
+        ff, PP = _fluxes(eqn, dim, qq[:, :, None], s.phys, None if yy is None else yy[:, None])
         flat = [[np.asarray(x).reshape(-1) for x in fc] for fc in ff]
         return flat, (None if PP is None else np.asarray(PP).reshape(-1))
 
-    fM, PM = fl(qM)
-    fPP, PPP = fl(qP)
-    if eqn == 0:
+    fM, PM = fl(qM, None if yflat is None else yflat[vM])
+    # a wall's P state is the mirrored M state at the M node's height
+    yPn = None if yflat is None else np.where(wall, yflat[vM], yflat[vP])
+    fPP, PPP = fl(qP, yPn)
+    if eqn in (0, 2):
         irho = dim
         unM = np.sum(nrm.T * qM[:, :dim], axis=1) / qM[:, irho]
         unP = np.sum(nrm.T * qP[:, :dim], axis=1) / qP[:, irho]

@@ -39,6 +39,28 @@ def pow_free_euler_state(s, seed=0, vel=10.0):
     return Q
 
 
+def smooth_te_state(s, seed=0, noise=1e-3, vel=10.0):
+    """A generic total-energy state (U, V, rho, E): smooth rho, velocity and
+    temperature plus per-node noise, E = rho (c_v T + |u|^2 / 2 + g y), so
+    the pressure rho R T stays positive."""
+    rng = np.random.default_rng(seed)
+    X = s.coords()
+    ph = rng.uniform(0, 2 * np.pi, size=8)
+    L = [np.ptp(x) if np.ptp(x) > 0 else 1.0 for x in X]
+    arg = [2 * np.pi * X[c] / L[c] for c in range(s.dim)]
+    sm = lambda k: np.sin(arg[0] + ph[k]) * np.cos(arg[-1] - ph[k + 1])
+    nz = lambda: noise * rng.standard_normal(X[0].shape)
+    rho = 1.0 + 0.1 * sm(0) + nz()
+    T = 300.0 + 5.0 * sm(2) + nz()
+    u = vel * (sm(4) + nz())
+    v = vel * (sm(5) + nz())
+    p = s.phys
+    Q = s.zeros_state()
+    Q[:, 0, :], Q[:, 1, :], Q[:, 2, :] = rho * u, rho * v, rho
+    Q[:, 3, :] = rho * (p.c_v * T + 0.5 * (u * u + v * v) + p.gravity * X[1])
+    return Q
+
+
 def smooth_swe_state(s, seed=0, depth=10.0, noise=1e-4):
     rng = np.random.default_rng(seed)
     X = s.coords()

@@ -176,3 +176,45 @@ def test_lsrk_loop_on_linear_swe():
         term = Lop(term)
         acc[:, :3, :] += c[k] * term[:, :3, :]
     assert rel_l2(Q[:, :3, :] - Q0[:, :3, :], acc[:, :3, :] - Q0[:, :3, :]) < 1e-10
+
+
+# ---- 2D Euler in total-energy form (euler2d_set3c.jl) -----------------------
+@pytest.mark.parametrize("N,periodic", [(3, (True, False)), (4, (False, False)), (6, (True, True))])
+def test_total_energy_weak_equals_strong(N, periodic):
+    from states import smooth_te_state
+    s = small_setup(2, 2, N, Ne=2, periodic=periodic)
+    Q = smooth_te_state(s, seed=N)
+    a = oracle_rhs(s, Q)
+    b = rhs_strong(s, Q)
+    assert rel_l2(a, b) < 1e-11
+
+
+def test_total_energy_warped_and_conservative():
+    """Free stream without gravity on warped elements (metric and the
+    M/P heights both enter), and conservation of rho and E with walls and a
+    periodic direction (no source acts on either)."""
+    from states import smooth_te_state
+    s = warped_setup(2, 2, 4, Ne=3, periodic=(True, True), gravity=0.0)
+    Q = s.zeros_state()
+    Q[:, 2, :] = 1.2
+    Q[:, 0, :], Q[:, 1, :] = 1.2 * 5.0, 1.2 * -3.0
+    Q[:, 3, :] = 1.2 * (717.5 * 300.0 + 0.5 * 34.0)
+    r = oracle_rhs(s, Q)
+    scale = np.max(np.abs(s.vgeo[:, 4, :])) * 1e5 * 2 / 500.0
+    assert np.max(np.abs(r)) < 1e-10 * scale
+    s = small_setup(2, 2, 3, Ne=2, periodic=(True, False), gravity=9.81)
+    Q = smooth_te_state(s, seed=7)
+    r = oracle_rhs(s, Q)
+    for st in (2, 3):
+        assert abs(r[:, st, :].sum()) < 1e-12 * np.abs(r[:, st, :]).sum()
+
+
+def test_total_energy_bubble_at_rest():
+    """The example's background at rest has no mass or energy tendency (both
+    fluxes vanish with the velocity) but a momentum one; the oracle's and
+    the host's courantnumber time steps agree."""
+    s = P.bubble2d_te(Ne=3, N=3)
+    r = oracle_rhs(s, s.Q0)
+    assert np.all(r[:, 2, :] == 0) and np.all(r[:, 3, :] == 0)
+    assert np.max(np.abs(r[:, 1, :])) > 0
+    assert O.euler2dte_dt(s, s.Q) == P.te_dt(s, s.Q)
