@@ -74,6 +74,52 @@ def total_energy_bytes(N):
     return 8 * (16 * Np + 7 * Np + 4 * Nfp * 3) + 2 * 4 * Nfp * 8 + 4 * 8
 
 
+def matrix_core_form(s, K, pnative, timed_own, Q0, q_measured, vgeo, D, steps, warmup,
+                     bytes_launch):
+    """The derivative contractions on the matrix cores (FP64_MFMA) against
+    the same loop and kernel with them on the vector ALUs (FP64_FMA): the
+    prepared-array loop of the measured value, and the volume term alone
+    (canary_volumerhs, µs per launch by HIP events over 20 launches)."""
+    import torch
+    from canary_amd._lib import FP64_FMA, FP64_MFMA
+    stages = 5 * steps
+    dof = s.nelem * s.Np * 5
+    out = {}
+    qm, qm2, rm = Q0.clone(), torch.empty_like(Q0), torch.zeros_like(Q0)
+    torch.cuda.synchronize()
+    qm, qm2 = pnative(qm, qm2, rm, warmup, prec=FP64_MFMA)
+    hold = {}
+
+    def go():
+        hold["q"] = pnative(qm, qm2, rm, steps, prec=FP64_MFMA)
+
+    t, _ = timed_own(go)
+    K.check_errors()
+    qf = hold["q"][0]
+    out["prepared_loop_gdof_stage_s"] = round(dof * stages / t / 1e9, 3)
+    out["prepared_loop_us_per_stage"] = round(t / stages * 1e6, 2)
+    out["prepared_loop_frac"] = round(bytes_launch / (t / stages) / 1e9 / HBM_PEAK_GBS, 4)
+    out["vs_vector_alu_state_rel_l2"] = float(
+        "%.3e" % (torch.linalg.vector_norm(qf - q_measured)
+                  / torch.linalg.vector_norm(q_measured)).item())
+    del qm, qm2, rm, hold, qf
+    rv = torch.zeros_like(Q0)
+    for name, prec in (("vector_alu", FP64_FMA), ("matrix_core", FP64_MFMA)):
+        for _ in range(3):
+            K.volumerhs(3, s.N, rv, Q0, vgeo, D, s.nelem, phys=s.phys, prec=prec)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            K.volumerhs(3, s.N, rv, Q0, vgeo, D, s.nelem, phys=s.phys, prec=prec)
+        e1.record()
+        torch.cuda.synchronize()
+        K.check_errors()
+        out["volumerhs_%s_us" % name] = round(e0.elapsed_time(e1) / 20 * 1e3, 2)
+    del rv
+    return out
+
+
 def two_d_form(ne, N, steps, warmup):
     """Single large 2D solves through the library's LSRK54 loop (fused
     stage, launched from C): the Euler set2c isentropic vortex, shallow
@@ -164,12 +210,13 @@ def sweep_form(steps=10, warmup=10):
     """The library LSRK54 loop over the contracted fused stage on the 3D
     bubble for N = 2..7, each at about 20 M DOF, over the reference arrays
     and over the prepared ones (the measured path), with the roofline
-    fraction of both on the reference-array byte rule B3(N)."""
+    fraction of both on the reference-array byte rule B3(N); and the prepared
+    loop with the derivative contractions on the matrix cores (FP64_MFMA)."""
     import numpy as np
     import torch
     import canary_amd.kernels as K
     import canary_amd.problems as P
-    from canary_amd._lib import FP64_FMA, PATH_FUSED
+    from canary_amd._lib import FP64_FMA, FP64_MFMA, PATH_FUSED
     from canary_amd.streams import OwnedStream
 
     dev = torch.device("cuda:0")
@@ -191,23 +238,24 @@ def sweep_form(steps=10, warmup=10):
         prep = K.prepare_geometry(3, N, vgeo, sgeo, vM, vP, bnd, s.nelem)
         K.check_errors()
 
-        def loop(n, prepared):
+        def loop(n, prepared, prec=FP64_FMA):
             with torch.cuda.stream(own.stream):
                 if prepared:
                     return K.lsrk54_prepared(3, N, q, q2, r, prep, D, s.nelem, dt, n,
-                                             phys=s.phys, prec=FP64_FMA)
+                                             phys=s.phys, prec=prec)
                 return K.lsrk54(3, N, q, r, vgeo, sgeo, D, s.nelem, vM, vP, bnd, dt, n,
                                 phys=s.phys, path=PATH_FUSED, prec=FP64_FMA, Qalt=q2)
         dof = s.nelem * s.Np * 5
         row = {"N": N, "elements": s.nelem, "dof": dof,
                "bytes_per_element_stage": algorithmic_bytes(3, N),
                "bytes_per_element_stage_prepared": prepared_bytes(3, N)}
-        for tag, prepared in (("", False), ("prepared_", True)):
-            loop(warmup, prepared)
+        for tag, prepared, prec in (("", False, FP64_FMA), ("prepared_", True, FP64_FMA),
+                                    ("prepared_mfma_", True, FP64_MFMA)):
+            loop(warmup, prepared, prec)
             torch.cuda.synchronize()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(own.stream)
-            fin = loop(steps, prepared)
+            fin = loop(steps, prepared, prec)
             e1.record(own.stream)
             torch.cuda.synchronize()
             K.check_errors()
@@ -480,6 +528,8 @@ def main():
     extra["prepared_vs_reference_arrays_rel_l2"] = float(
         "%.3e" % (torch.linalg.vector_norm(qpf - qf) / torch.linalg.vector_norm(qf)).item())
     del qp_, qp2_, rp_
+    extra["matrix_core_form"] = matrix_core_form(s, K, pnative, timed_own, Q0, qpf, vgeo, D,
+                                                 args.steps, args.warmup, bytes_launch)
 
     # the same loop replaying two captured steps per hipGraph launch (capture
     # and instantiation inside the timed call)
@@ -620,7 +670,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "stage_prep_kernel<Euler3DFma,5,1,1> (fused volume+face+update "
+                     "kernel": "stage_prep_kernel<Euler3DFma,5,1,1,false> (fused volume+face+update "
                                "over the prepared arrays)",
                      "bytes_per_element_stage": algorithmic_bytes(3, s.N),
                      "bytes_per_element_stage_own": own_bytes,
@@ -1103,7 +1153,7 @@ def served_form(nreq, rate, seed=2024):
                                     "median": float("%.3e" % float(np.median(errs)))}}
 
 
-HEADLINE_KERNEL = "stage_prep_kernel<canary::Euler3DFma, 5, 1, 1>"
+HEADLINE_KERNEL = "stage_prep_kernel<canary::Euler3DFma, 5, 1, 1, false>"
 
 
 def _csv_rows(d, pattern):

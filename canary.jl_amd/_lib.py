@@ -15,7 +15,7 @@ OK, ERR_ARG, ERR_UNSUPPORTED, ERR_LAUNCH, ERR_BC, ERR_VMAP = 0, 1, 2, 3, 4, 5
 ERR_NOCONVERGENCE, ERR_LOBATTO_N1, ERR_ORDERING, ERR_ORIENTATION = 6, 7, 8, 9
 ERR_NOMEM, ERR_CAPACITY = 10, 11
 EULER, SWE, EULER_TE = 0, 1, 2  # EULER_TE: 2D total-energy equations (set3c)
-FP64, FP32, FP64_FMA = 0, 1, 2
+FP64, FP32, FP64_FMA, FP64_MFMA = 0, 1, 2, 3  # FP64_MFMA: contractions on the matrix cores
 PATH_KERNELS, PATH_FUSED, PATH_FACE_UPDATE = 0, 1, 2
 REDUCE_WORK = 1024  # CANARY_REDUCE_WORK
 ENDPT_NEITHER, ENDPT_LEFT, ENDPT_RIGHT, ENDPT_BOTH = 0, 1, 2, 3

@@ -30,7 +30,8 @@ HIP_SOURCES = ["hip/runtime.hip", "hip/euler3d.hip", "hip/euler2d.hip", "hip/swe
                "hip/prepared.hip", "hip/driver.hip"]
 # kernels allowed to contract a*b+c into FMA
 HIP_FAST_SOURCES = ["hip/perturb32.hip", "hip/euler3d_fma.hip", "hip/euler2d_fma.hip",
-                    "hip/swe2d_fma.hip", "hip/euler2d_te_fma.hip", "hip/batched_fma.hip"]
+                    "hip/swe2d_fma.hip", "hip/euler2d_te_fma.hip", "hip/batched_fma.hip",
+                    "hip/euler3d_mfma.hip"]
 
 ARCH = os.environ.get("CANARY_OFFLOAD_ARCH", "gfx950")
 

@@ -31,6 +31,7 @@
 #include <type_traits>
 
 #include "phys.h"
+#include "mfma.h"
 
 namespace canary {
 
@@ -410,6 +411,34 @@ __device__ __forceinline__ int face_contribution(const typename Phys::Real* __re
   return face_finish<Phys>(it, p, qm_at, pm_at, out);
 }
 
+// The face terms of volume node n, subtracted from r in ascending face order
+// (the order facerhs! applies them: it walks the faces in turn, and a node
+// lies on at most one face of each pair 2d, 2d+1): with vmapM the canonical
+// face mask, the term of face item f*Nfp + fn lands on node canon_vid(f, fn),
+// so the node owner finds its one to DIM items from its own indices.  sC:
+// the items' terms [fi][s]; sV: their node ids, NOVID where an item has no
+// term (an error was recorded).
+template <int DIM, int NQ, int NSR, class Real, class VidT>
+__device__ __forceinline__ void face_terms_at_node(Real* r, const Real* sC, const VidT* sV, int n,
+                                                   VidT novid) {
+  constexpr int Nfp = ipow(NQ, DIM - 1);
+  const int i = n % NQ, j = (n / NQ) % NQ, k = DIM == 3 ? n / (NQ * NQ) : 0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const int idx = d == 0 ? i : (d == 1 ? j : k);
+    if (idx == 0 || idx == NQ - 1) {
+      const int f = 2 * d + (idx == 0 ? 0 : 1);
+      const int fn = DIM == 2 ? (d == 0 ? j : i)
+                              : (d == 0 ? j + NQ * k : (d == 1 ? i + NQ * k : i + NQ * j));
+      const int fi = f * Nfp + fn;
+      if (sV[fi] != novid) {
+#pragma unroll
+        for (int s = 0; s < NSR; ++s) r[s] = r[s] - sC[fi * NSR + s];
+      }
+    }
+  }
+}
+
 // ================================================================ kernels
 template <class Phys, int NQ, int EPW>
 struct VolumeCfg {
@@ -417,7 +446,9 @@ struct VolumeCfg {
   static constexpr int NT = round64(EPW * S::Np);
 };
 
-template <class Phys, int NQ, int EPW>
+// MFMA: the derivative contractions on the matrix cores (MfmaSweep), in
+// place in sF, then r + xi + eta + zeta at each node; else volume_sweeps.
+template <class Phys, int NQ, int EPW, bool MFMA = false>
 __global__ __launch_bounds__((VolumeCfg<Phys, NQ, EPW>::NT)) void volume_kernel(
     double* __restrict__ rhs, const double* __restrict__ Q, const double* __restrict__ vgeo,
     const double* __restrict__ D, int64_t nreal, int64_t chunk, int reverse, PhysParams p) {
@@ -447,18 +478,40 @@ __global__ __launch_bounds__((VolumeCfg<Phys, NQ, EPW>::NT)) void volume_kernel(
     double F[DIM][NSR];
     Phys::volume_flux(q, P, met, MJ, p, F);
 #pragma unroll
-    for (int d = 0; d < DIM; ++d)
+    for (int d = 0; d < DIM; ++d) {
+      // MFMA: each direction line by line (MfmaSweep)
+      const int w = MFMA ? MfmaSweep<DIM, NQ, NSR>::slot_of(d, n) : n;
 #pragma unroll
-      for (int s = 0; s < NSR; ++s) sF[d][slot][s * Np + n] = F[d][s];
+      for (int s = 0; s < NSR; ++s) sF[d][slot][s * Np + w] = F[d][s];
+    }
     const double* re = rhs + (size_t)NS * Np * (size_t)e;
 #pragma unroll
     for (int s = 0; s < NSR; ++s) r[s] = re[n + Np * s];
     Phys::source(q, MJ, p, r);
   }
   lds_barrier();
-  if (valid) {
+  if constexpr (MFMA) {
+    const int nv = nreal - blk * EPW < EPW ? (int)(nreal - blk * EPW) : EPW;
+    mfma_sweeps<DIM, NQ, NSR, VolumeCfg<Phys, NQ, EPW>::NT, EPW, DIM>(
+        &sF[0][0][0], EPW * NSR * Np, NSR * Np, nv, sD);
+    lds_barrier();
+    if (valid) {
+      int w[DIM];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) w[d] = MfmaSweep<DIM, NQ, NSR>::slot_of(d, n);
+#pragma unroll
+      for (int s = 0; s < NSR; ++s) {
+        double a = r[s];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) a = a + sF[d][slot][s * Np + w[d]];
+        r[s] = a;
+      }
+    }
+  } else if (valid) {
     volume_sweeps<DIM, NQ, NSR>(r, sD, sF[0][slot], sF[DIM > 1 ? 1 : 0][slot],
                                 sF[DIM > 2 ? 2 : 0][slot], n);
+  }
+  if (valid) {
     double* re = rhs + (size_t)NS * Np * (size_t)e;
 #pragma unroll
     for (int s = 0; s < NSR; ++s) re[n + Np * s] = r[s];
@@ -541,7 +594,8 @@ struct FusedCfg {
   static constexpr int OFF_P = OFF_Q + a16(sizeof(Real) * EPW * Phys::NS * S::Np);
   static constexpr int OFF_B = OFF_P + a16(sizeof(Real) * EPW * (Phys::HAS_P ? S::Np : 1));
   static constexpr int OFF_V = OFF_B + a16(sizeof(Real) * EPW * BUF);
-  static constexpr int LDS = OFF_V + a16(sizeof(VidT) * EPW * S::NFN);
+  static constexpr int OFF_F = OFF_V + a16(sizeof(VidT) * EPW * S::NFN);  // one flag word
+  static constexpr int LDS = OFF_F + 16;
 };
 
 // One fused stage of the EPW-element block starting at e0, computed by NT
@@ -554,7 +608,7 @@ struct FusedCfg {
 // MJI alone and D is not read.  UPDATE = false (with VOLUME = false) is
 // facerhs! alone: rhs -= face terms, Q untouched, neither vgeo nor Qout read.
 template <class Phys, int NQ, int EPW, int NT, int PF, bool PM_LDS, bool VOLUME = true,
-          bool UPDATE = true, bool PREP = false>
+          bool UPDATE = true, bool PREP = false, bool MFMA = false>
 __device__ __forceinline__ void stage_body(
     char* __restrict__ lds, typename Phys::Real* __restrict__ rhs,
     const typename Phys::Real* __restrict__ Qin, typename Phys::Real* __restrict__ Qout,
@@ -581,11 +635,14 @@ __device__ __forceinline__ void stage_body(
   using VidT = typename C::VidT;
   constexpr VidT NOVID = (VidT)~(VidT)0;
   static_assert(NT >= EPW * Np, "one volume node per thread");
+  static_assert(!MFMA || (VOLUME && std::is_same<Real, double>::value),
+                "matrix-core contractions: FP64 volume term");
   Real* sD = reinterpret_cast<Real*>(lds + C::OFF_D);
   Real* sQ = reinterpret_cast<Real*>(lds + C::OFF_Q);  // [EPW][NS*Np]
   Real* sP = reinterpret_cast<Real*>(lds + C::OFF_P);  // [EPW][Np]
   Real* sB = reinterpret_cast<Real*>(lds + C::OFF_B);  // [EPW][BUF]
   VidT* sV = reinterpret_cast<VidT*>(lds + C::OFF_V);  // [EPW][NFN]
+  int* sFlag = reinterpret_cast<int*>(lds + C::OFF_F);  // a non-canonical M-side node seen
 
   const int t = threadIdx.x;
   const int slot = t / Np, n = t - slot * Np;
@@ -653,6 +710,7 @@ __device__ __forceinline__ void stage_body(
   }
 
   if (VOLUME && t < NQ * NQ) sD[t] = (Real)dval;
+  if (!PREP && t == 0) *sFlag = 0;  // read after the face terms, barriers later
   Real Fv[DIM][NSR];
   if (valid) {
 #pragma unroll
@@ -668,12 +726,24 @@ __device__ __forceinline__ void stage_body(
 #pragma unroll
   for (int d = 0; d < (VOLUME ? DIM : 0); ++d) {
     if (d > 0) lds_barrier();  // the previous direction's fluxes are consumed
+    // MFMA: the direction line by line (MfmaSweep)
+    const int w = MFMA ? MfmaSweep<DIM, NQ, NSR>::slot_of(d, n) : n;
     if (valid) {
 #pragma unroll
-      for (int s = 0; s < NSR; ++s) sB[slot * BUF + s * Np + n] = Fv[d][s];
+      for (int s = 0; s < NSR; ++s) sB[slot * BUF + s * Np + w] = Fv[d][s];
     }
     lds_barrier();
-    if (valid) {
+    if constexpr (MFMA) {
+      // the direction's contractions in place in sB, then added by the node
+      const int nvb = nreal - e0 < EPW ? (int)(nreal - e0) : EPW;
+      mfma_sweeps<DIM, NQ, NSR, NT, EPW, 1>(reinterpret_cast<double*>(sB), 0, BUF, nvb,
+                                            reinterpret_cast<const double*>(sD));
+      lds_barrier();
+      if (valid) {
+#pragma unroll
+        for (int s = 0; s < NSR; ++s) r[s] = r[s] + sB[slot * BUF + s * Np + w];
+      }
+    } else if (valid) {
       const Real* b = sB + slot * BUF;
       if (d == 0) volume_sweep_dir<DIM, NQ, NSR, Real, 0>(r, sD, b, n);
       if (d == 1) volume_sweep_dir<DIM, NQ, NSR, Real, (DIM > 1 ? 1 : 0)>(r, sD, b, n);
@@ -686,6 +756,7 @@ __device__ __forceinline__ void stage_body(
     for (int s = 0; s < NSR; ++s) sB[slot * BUF + NFN * NSR + s * Np + n] = r[s];
   }
   // 4. face fluxes: M side from LDS, P side gathered
+  bool noncanon = false;  // an item whose M-side node is not the canonical one
   auto pm_of = [&](int sl, int v, const Real* qm) -> Real {
     if (!Phys::HAS_P) return Real(0);
     if constexpr (SP)
@@ -702,6 +773,7 @@ __device__ __forceinline__ void stage_body(
     auto pm_at = [&](int v, const Real* qm) { return pm_of(sl, v, qm); };
     Real c[NSR];
     const int vid = face_finish<Phys>(fit[k], p, qm_at, pm_at, c);
+    if constexpr (!PREP) noncanon |= vid >= 0 && vid != canon_vid<DIM, NQ>(fi / Nfp, fi % Nfp);
     sV[sl * NFN + fi] = vid < 0 ? NOVID : (VidT)vid;
 #pragma unroll
     for (int s = 0; s < NSR; ++s) sB[sl * BUF + fi * NSR + s] = c[s];
@@ -723,29 +795,50 @@ __device__ __forceinline__ void stage_body(
     } else {
       vid = face_contribution<Phys, NQ>(Qin, vgeo, sgeo, vmapM, vmapP, bndy, e0 + sl, fi, ntotal, p,
                                         qm_at, pm_at, c);
+      noncanon |= vid >= 0 && vid != canon_vid<DIM, NQ>(fi / Nfp, fi % Nfp);
     }
     sV[sl * NFN + fi] = vid < 0 ? NOVID : (VidT)vid;
 #pragma unroll
     for (int s = 0; s < NSR; ++s) sB[sl * BUF + fi * NSR + s] = c[s];
   }
+  // The face terms reach rhs (staged in LDS) in ascending face order: with
+  // the canonical face mask (always over the prepared arrays; over the
+  // reference arrays when every item of the workgroup maps to its canonical
+  // node) the node owner gathers its own (face_terms_at_node, no barrier);
+  // else the face pairs are scattered round by round.  Same operations in
+  // the same order either way.
+  if (!PREP && noncanon) *sFlag = 1;
   lds_barrier();
+  const bool scatter = !PREP && *sFlag != 0;
+  if (scatter) {
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) {
-    for (int it = t; it < EPW * 2 * Nfp; it += NT) {
-      const int sl = it / (2 * Nfp), fi = 2 * d * Nfp + (it - sl * 2 * Nfp);
-      const VidT v = sV[sl * NFN + fi];
-      if (e0 + sl < nreal && v != NOVID) {
-        Real* rr = sB + sl * BUF + NFN * NSR;
+    for (int d = 0; d < DIM; ++d) {
+      for (int it = t; it < EPW * 2 * Nfp; it += NT) {
+        const int sl = it / (2 * Nfp), fi = 2 * d * Nfp + (it - sl * 2 * Nfp);
+        const VidT v = sV[sl * NFN + fi];
+        if (e0 + sl < nreal && v != NOVID) {
+          Real* rr = sB + sl * BUF + NFN * NSR;
 #pragma unroll
-        for (int s = 0; s < NSR; ++s) rr[s * Np + v] = rr[s * Np + v] - sB[sl * BUF + fi * NSR + s];
+          for (int s = 0; s < NSR; ++s)
+            rr[s * Np + v] = rr[s * Np + v] - sB[sl * BUF + fi * NSR + s];
+        }
       }
+      lds_barrier();
     }
-    lds_barrier();
+  } else if (!UPDATE && valid) {
+    Real rv[NSR];
+    Real* rr = sB + slot * BUF + NFN * NSR + n;
+#pragma unroll
+    for (int s = 0; s < NSR; ++s) rv[s] = rr[s * Np];
+    face_terms_at_node<DIM, NQ, NSR>(rv, sB + slot * BUF, sV + slot * NFN, n, NOVID);
+#pragma unroll
+    for (int s = 0; s < NSR; ++s) rr[s * Np] = rv[s];
   }
   // 5. update: Qout = Q + ((rkb*dt)*rhs)*MJI, rhs *= rka, in LDS, by the
-  //    thread that holds the node's MJI.  Shallow water with `advection`
-  //    keeps the velocity: u = Hu / (h + b) before the update, Hu = (h + b) u
-  //    after it (shallow_water.jl:274-297).
+  //    thread that holds the node's MJI (which, unless the face terms were
+  //    scattered, first takes its own face terms).  Shallow water with
+  //    `advection` keeps the velocity: u = Hu / (h + b) before the update,
+  //    Hu = (h + b) u after it (shallow_water.jl:274-297).
   if (UPDATE && valid) {
     // prepared arrays hold no MJI: the reference's is 1 ./ MJ (IEEE division)
     if constexpr (PREP) MJI = Real(1) / MJ;
@@ -758,13 +851,16 @@ __device__ __forceinline__ void stage_body(
       uu = qn[0] / Ht;
       vv = qn[Np] / Ht;
     }
+    Real rv[NSR];
+#pragma unroll
+    for (int s = 0; s < NSR; ++s) rv[s] = sB[slot * BUF + NFN * NSR + s * Np + n];
+    if (!scatter) face_terms_at_node<DIM, NQ, NSR>(rv, sB + slot * BUF, sV + slot * NFN, n, NOVID);
 #pragma unroll
     for (int s = 0; s < NSR; ++s) {
-      Real& rr = sB[slot * BUF + NFN * NSR + s * Np + n];
       Real& qq = qn[s * Np];
-      const Real rs = rr;
+      const Real rs = rv[s];
       qq = qq + bdt * rs * MJI;
-      rr = rs * ka;
+      sB[slot * BUF + NFN * NSR + s * Np + n] = rs * ka;
     }
     if (Phys::HAS_ADVECTION && advection) {
       const Real Ht = qn[Np * 2] + qn[Np * 3];
@@ -794,7 +890,7 @@ __device__ __forceinline__ void stage_body(
   }
 }
 
-template <class Phys, int NQ, int EPW, bool WIDE, int PF, bool PM_LDS>
+template <class Phys, int NQ, int EPW, bool WIDE, int PF, bool PM_LDS, bool MFMA = false>
 __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, WIDE>::NT)) void stage_kernel(
     double* __restrict__ rhs, const double* __restrict__ Qin, double* __restrict__ Qout,
     const double* __restrict__ vgeo, const double* __restrict__ sgeo,
@@ -807,14 +903,14 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, WIDE>::NT)) void stage_ker
   const int64_t blk = stage_block(blockIdx.x, chunk, reverse);
   const int64_t e0 = blk * EPW;
   if (e0 >= nreal) return;  // whole workgroup leaves together
-  stage_body<Phys, NQ, EPW, C::NT, PF, PM_LDS>(lds, rhs, Qin, Qout, vgeo, sgeo, D, e0, nreal,
-                                               ntotal, vmapM, vmapP, bndy, rka, rkb, dt, p,
-                                               advection);
+  stage_body<Phys, NQ, EPW, C::NT, PF, PM_LDS, true, true, false, MFMA>(
+      lds, rhs, Qin, Qout, vgeo, sgeo, D, e0, nreal, ntotal, vmapM, vmapP, bndy, rka, rkb, dt, p,
+      advection);
 }
 
 // The fused stage over the prepared arrays (see Prepared): the same
 // arithmetic and order, fewer bytes.
-template <class Phys, int NQ, int EPW, int PF>
+template <class Phys, int NQ, int EPW, int PF, bool MFMA = false>
 __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, false>::NT)) void stage_prep_kernel(
     double* __restrict__ rhs, const double* __restrict__ Qin, double* __restrict__ Qout,
     const double* __restrict__ pvgeo, const double* __restrict__ psgeo,
@@ -826,7 +922,7 @@ __global__ __launch_bounds__((FusedCfg<Phys, NQ, EPW, false>::NT)) void stage_pr
   const int64_t blk = stage_block(blockIdx.x, chunk, reverse);
   const int64_t e0 = blk * EPW;
   if (e0 >= nreal) return;  // whole workgroup leaves together
-  stage_body<Phys, NQ, EPW, C::NT, PF, true, true, true, true>(
+  stage_body<Phys, NQ, EPW, C::NT, PF, true, true, true, true, MFMA>(
       lds, rhs, Qin, Qout, pvgeo, psgeo, D, e0, nreal, ntotal, nullptr,
       reinterpret_cast<const int64_t*>(pface), nullptr, rka, rkb, dt, p, advection);
 }

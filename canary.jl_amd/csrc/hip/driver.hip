@@ -79,10 +79,14 @@ int step(Run& r, int& cur) {
                               r.dt, r.advection, r.phys, r.stream);
       cur = 1 - cur;
     } else {
+      // contracted: the volume term in the exact order; matrix-core: its
+      // matrix-core form, and face + update (no contraction) contracted
       rc = canary_volumerhs(r.eqn, r.dim, r.N, r.prec == CANARY_FP64_FMA ? CANARY_FP64 : r.prec,
                             r.rhs, r.Q[cur], r.vgeo, r.D, r.nreal, r.phys, r.stream);
       if (!rc)
-        rc = canary_facerhs_update(r.eqn, r.dim, r.N, r.prec, r.rhs, r.Q[cur], r.Q[1 - cur],
+        rc = canary_facerhs_update(r.eqn, r.dim, r.N,
+                                   r.prec == CANARY_FP64_MFMA ? CANARY_FP64_FMA : r.prec, r.rhs,
+                                   r.Q[cur], r.Q[1 - cur],
                                    r.vgeo, r.sgeo, r.nreal, r.ntotal, r.vmapM, r.vmapP, r.bndy,
                                    rka, rkb, r.dt, r.advection, r.phys, r.stream);
       cur = 1 - cur;

@@ -33,6 +33,13 @@ struct KArgs {
 
 constexpr int kMaxN = 7;  // N = 1..7 compiled in
 
+// The kernel families of a translation unit: the exact-order unit holds every
+// operation; the contracted unit (built with -ffp-contract=fast) the fused
+// stages, face + update and the volume term; the matrix-core unit the
+// contracted unit's volume term and fused stages with the derivative
+// contractions on the MFMA units (MfmaSweep).
+enum Unit { UNIT_EXACT = 0, UNIT_FMA = 1, UNIT_MFMA = 2 };
+
 inline int launch_status() {
   hipError_t err = hipGetLastError();
   return err == hipSuccess ? 0 : 3 /*CANARY_ERR_LAUNCH*/;
@@ -55,7 +62,7 @@ inline int walk_back(const void* qin, const void* qout) {
   return (uintptr_t)qin > (uintptr_t)qout ? 1 : 0;
 }
 
-template <class Phys, int NQ, int EPW>
+template <class Phys, int NQ, int EPW, bool MFMA = false>
 void launch_stage(int64_t nb, const KArgs& a) {
   // measured on 3D N=4 (32^3): one prefetched face item per thread, the
   // nodal pressure cached in LDS, workgroup sized by the volume nodes.
@@ -67,18 +74,18 @@ void launch_stage(int64_t nb, const KArgs& a) {
   constexpr bool PM_LDS = true;
   constexpr int NT = FusedCfg<Phys, NQ, EPW, WIDE>::NT;
   const int64_t chunk = (nb + 7) / 8;
-  stage_kernel<Phys, NQ, EPW, WIDE, PF, PM_LDS><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
+  stage_kernel<Phys, NQ, EPW, WIDE, PF, PM_LDS, MFMA><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
       (double*)a.rhs, (const double*)a.Q, (double*)a.Qout, (const double*)a.vgeo,
       (const double*)a.sgeo, a.D, a.nreal, a.ntotal, a.vmapM, a.vmapP, a.bndy, a.rka, a.rkb,
       a.dt, chunk, walk_back(a.Q, a.Qout), a.phys, a.advection);
 }
 
-template <class Phys, int NQ, int EPW>
+template <class Phys, int NQ, int EPW, bool MFMA = false>
 void launch_stage_prep(int64_t nb, const KArgs& a) {
   constexpr int PF = (Phys::DIM == 3 && NQ == 7) ? 0 : 1;  // as launch_stage
   constexpr int NT = FusedCfg<Phys, NQ, EPW, false>::NT;
   const int64_t chunk = (nb + 7) / 8;
-  stage_prep_kernel<Phys, NQ, EPW, PF><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
+  stage_prep_kernel<Phys, NQ, EPW, PF, MFMA><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
       (double*)a.rhs, (const double*)a.Q, (double*)a.Qout, (const double*)a.vgeo,
       (const double*)a.sgeo, a.pface, a.D, a.nreal, a.ntotal, a.rka, a.rkb, a.dt, chunk,
       walk_back(a.Q, a.Qout), a.phys, a.advection);
@@ -86,9 +93,18 @@ void launch_stage_prep(int64_t nb, const KArgs& a) {
 
 // The prepared arrays hold no node heights: equation sets whose pressure
 // needs them have no prepared stage.
-template <class Phys, int NQ, int EPW>
+template <class Phys, int NQ, int EPW, bool MFMA = false>
 void launch_stage_prep_if(int64_t nb, const KArgs& a) {
-  if constexpr (!NeedsY<Phys>::value) launch_stage_prep<Phys, NQ, EPW>(nb, a);
+  if constexpr (!NeedsY<Phys>::value) launch_stage_prep<Phys, NQ, EPW, MFMA>(nb, a);
+}
+
+template <class Phys, int NQ, int EPW, bool MFMA>
+void launch_volume(int64_t nb, const KArgs& a) {
+  constexpr int NT = VolumeCfg<Phys, NQ, EPW>::NT;
+  const int64_t chunk = (nb + 7) / 8;
+  volume_kernel<Phys, NQ, EPW, MFMA><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
+      (double*)a.rhs, (const double*)a.Q, (const double*)a.vgeo, a.D, a.nreal, chunk,
+      inplace_direction(), a.phys);
 }
 
 template <class Phys, int NQ, int EPW>
@@ -101,35 +117,35 @@ void launch_face_update(int64_t nb, const KArgs& a) {
       chunk, inplace_direction(), a.phys, a.advection);
 }
 
-// STAGE_ONLY: the fused kernels alone (stage, face + update) are
-// instantiated (the contracted-FP64 units); the other operations report
-// CANARY_ERR_UNSUPPORTED.
-template <class Phys, int NQ, bool STAGE_ONLY>
+// The operations a unit does not hold report CANARY_ERR_UNSUPPORTED.
+template <class Phys, int NQ, int UNIT>
 int launch(int op, const KArgs& a) {
   using S = Shape<Phys, NQ>;
   constexpr int EPW = default_epw<S::Np>();
+  constexpr bool MFMA = UNIT == UNIT_MFMA;
   if (a.nreal <= 0) return 0;
   const int64_t nb = (a.nreal + EPW - 1) / EPW;
-  if constexpr (STAGE_ONLY) {
+  if constexpr (UNIT != UNIT_EXACT) {
     if (op == OP_STAGE)
-      launch_stage<Phys, NQ, EPW>(nb, a);
+      launch_stage<Phys, NQ, EPW, MFMA>(nb, a);
     else if (op == OP_STAGE_PREP && !NeedsY<Phys>::value)
-      launch_stage_prep_if<Phys, NQ, EPW>(nb, a);
-    else if (op == OP_FACE_UPDATE)
-      launch_face_update<Phys, NQ, EPW>(nb, a);
-    else
+      launch_stage_prep_if<Phys, NQ, EPW, MFMA>(nb, a);
+    else if (op == OP_VOLUME)
+      launch_volume<Phys, NQ, EPW, MFMA>(nb, a);
+    else if (op == OP_FACE_UPDATE) {
+      // (kept out of the matrix-core unit: one unit per kernel)
+      if constexpr (MFMA) return 2 /*CANARY_ERR_UNSUPPORTED*/;
+      else launch_face_update<Phys, NQ, EPW>(nb, a);
+    } else {
       return 2 /*CANARY_ERR_UNSUPPORTED*/;
+    }
   } else {
     double* rhs = (double*)a.rhs;
     const double* Q = (const double*)a.Q;
     switch (op) {
-      case OP_VOLUME: {
-        constexpr int NT = VolumeCfg<Phys, NQ, EPW>::NT;
-        const int64_t chunk = (nb + 7) / 8;
-        volume_kernel<Phys, NQ, EPW><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
-            rhs, Q, (const double*)a.vgeo, a.D, a.nreal, chunk, inplace_direction(), a.phys);
+      case OP_VOLUME:
+        launch_volume<Phys, NQ, EPW, false>(nb, a);
         break;
-      }
       case OP_FACE: {
         constexpr int NT = FusedCfg<Phys, NQ, EPW, false>::NT;
         const int64_t chunk = (nb + 7) / 8;
@@ -164,16 +180,16 @@ int launch(int op, const KArgs& a) {
   return launch_status();
 }
 
-template <class Phys, bool STAGE_ONLY = false>
+template <class Phys, int UNIT = UNIT_EXACT>
 int dispatch(int op, int N, const KArgs& a) {
   switch (N) {
-    case 1: return launch<Phys, 2, STAGE_ONLY>(op, a);
-    case 2: return launch<Phys, 3, STAGE_ONLY>(op, a);
-    case 3: return launch<Phys, 4, STAGE_ONLY>(op, a);
-    case 4: return launch<Phys, 5, STAGE_ONLY>(op, a);
-    case 5: return launch<Phys, 6, STAGE_ONLY>(op, a);
-    case 6: return launch<Phys, 7, STAGE_ONLY>(op, a);
-    case 7: return launch<Phys, 8, STAGE_ONLY>(op, a);
+    case 1: return launch<Phys, 2, UNIT>(op, a);
+    case 2: return launch<Phys, 3, UNIT>(op, a);
+    case 3: return launch<Phys, 4, UNIT>(op, a);
+    case 4: return launch<Phys, 5, UNIT>(op, a);
+    case 5: return launch<Phys, 6, UNIT>(op, a);
+    case 6: return launch<Phys, 7, UNIT>(op, a);
+    case 7: return launch<Phys, 8, UNIT>(op, a);
   }
   return 2 /*CANARY_ERR_UNSUPPORTED*/;
 }
@@ -195,13 +211,24 @@ inline int read_status(int64_t* info, int clear) {
 
 }  // namespace canary
 
-// The contracted-FP64 unit of an equation set: the fused stage only, built
-// with -ffp-contract=fast from a physics policy whose pressure avoids pow.
+// The contracted-FP64 unit of an equation set (fused stages, face + update,
+// volume term), built with -ffp-contract=fast from a physics policy whose
+// pressure avoids pow.
 #define CANARY_DEFINE_FMA_STAGE(NAME, PHYS)                                       \
   extern "C" int canary_k_##NAME##_fma(int op, int N, const canary::KArgs* a) {  \
-    return canary::dispatch<PHYS, true>(op, N, *a);                               \
+    return canary::dispatch<PHYS, canary::UNIT_FMA>(op, N, *a);                   \
   }                                                                               \
   extern "C" int canary_k_##NAME##_fma_status(int64_t* info, int clear) {        \
+    return canary::read_status(info, clear);                                      \
+  }
+
+// The matrix-core unit: the contracted unit's volume term and fused stages
+// with the derivative contractions on the MFMA units.
+#define CANARY_DEFINE_MFMA_UNIT(NAME, PHYS)                                       \
+  extern "C" int canary_k_##NAME##_mfma(int op, int N, const canary::KArgs* a) { \
+    return canary::dispatch<PHYS, canary::UNIT_MFMA>(op, N, *a);                  \
+  }                                                                               \
+  extern "C" int canary_k_##NAME##_mfma_status(int64_t* info, int clear) {       \
     return canary::read_status(info, clear);                                      \
   }
 

@@ -18,6 +18,8 @@ int canary_k_euler2d_te_fma(int op, int N, const canary::KArgs* a);
 int canary_k_euler2d_te_status(int64_t* info, int clear);
 int canary_k_euler2d_te_fma_status(int64_t* info, int clear);
 int canary_k_euler3d_fma(int op, int N, const canary::KArgs* a);
+int canary_k_euler3d_mfma(int op, int N, const canary::KArgs* a);
+int canary_k_euler3d_mfma_status(int64_t* info, int clear);
 int canary_k_euler2d_fma(int op, int N, const canary::KArgs* a);
 int canary_k_swe2d_fma(int op, int N, const canary::KArgs* a);
 int canary_k_euler3d_status(int64_t* info, int clear);
@@ -106,8 +108,11 @@ namespace {
 using KFn = int (*)(int, int, const canary::KArgs*);
 
 // CANARY_FP64 selects the exact-order units, CANARY_FP64_FMA the contracted
-// ones (which hold the fused stage only).
+// ones (fused stages, face + update, volume term), CANARY_FP64_MFMA the
+// matrix-core unit (3D Euler: volume term and fused stages).
 KFn select(int eqn, int dim, int prec) {
+  if (prec == CANARY_FP64_MFMA)
+    return eqn == CANARY_EULER && dim == 3 ? canary_k_euler3d_mfma : nullptr;
   const bool fma = prec == CANARY_FP64_FMA;
   if (prec != CANARY_FP64 && !fma) return nullptr;
   if (eqn == CANARY_EULER && dim == 3) return fma ? canary_k_euler3d_fma : canary_k_euler3d;
@@ -139,10 +144,13 @@ canary::PhysParams to_params(const canary_phys* ph, void* stream = nullptr, int 
   return p;
 }
 
-// stage_only: CANARY_FP64_FMA is accepted (the fused entries only).
+// contracted: the entry point accepts the contracted forms (CANARY_FP64_FMA,
+// CANARY_FP64_MFMA); an operation the selected unit does not hold reports
+// CANARY_ERR_UNSUPPORTED from the unit.
 int check_common(int eqn, int dim, int N, int prec, int64_t nreal, KFn* fn,
-                 bool stage_only = false) {
-  if (prec == CANARY_FP64_FMA && !stage_only) return CANARY_ERR_UNSUPPORTED;
+                 bool contracted = false) {
+  if ((prec == CANARY_FP64_FMA || prec == CANARY_FP64_MFMA) && !contracted)
+    return CANARY_ERR_UNSUPPORTED;
   *fn = select(eqn, dim, prec);
   if (!*fn) return CANARY_ERR_UNSUPPORTED;
   if (N < 1 || N > canary::kMaxN) return CANARY_ERR_UNSUPPORTED;
@@ -156,7 +164,7 @@ extern "C" int canary_volumerhs(int eqn, int dim, int N, int prec, void* rhs, co
                                 const void* vgeo, const double* D, int64_t nreal,
                                 const canary_phys* phys, void* stream) {
   KFn fn;
-  int rc = check_common(eqn, dim, N, prec, nreal, &fn);
+  int rc = check_common(eqn, dim, N, prec, nreal, &fn, true);
   if (rc) return rc;
   if (nreal == 0) return 0;
   if (!rhs || !Q || !vgeo || !D) return CANARY_ERR_ARG;
@@ -339,14 +347,15 @@ extern "C" int canary_check_errors(void* stream, int64_t* info) {
   if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return CANARY_ERR_LAUNCH;
   int64_t tmp[4];
   int first = 0;
-  int (*readers[14])(int64_t*, int) = {
+  int (*readers[15])(int64_t*, int) = {
       canary_k_euler3d_status,        canary_k_euler2d_status,
       canary_k_swe2d_status,          canary_k_euler2d_te_status,
       canary_k_euler3d_fma_status,    canary_k_euler2d_fma_status,
       canary_k_swe2d_fma_status,      canary_k_euler2d_te_fma_status,
       canary_k_perturb32_status,      canary_k_batch_status,
       canary_k_batch_fma_status,      canary_k_geometry_status,
-      canary_k_aux_status,            canary_k_prepared_status};
+      canary_k_aux_status,            canary_k_prepared_status,
+      canary_k_euler3d_mfma_status};
   for (auto rd : readers) {
     int rc = rd(tmp, 1);
     if (rc && !first) {

@@ -15,7 +15,7 @@ import ctypes as C
 import torch
 
 from . import _lib
-from ._lib import EULER, EULER_TE, SWE, FP64, FP64_FMA, check, lib
+from ._lib import EULER, EULER_TE, SWE, FP64, FP64_FMA, FP64_MFMA, check, lib
 
 _NS = {(EULER, 3): 5, (EULER, 2): 4, (SWE, 2): 4, (EULER_TE, 2): 4}
 _NV = {3: 14, 2: 8}
@@ -62,8 +62,14 @@ def _phys(phys):
     return C.byref(phys) if phys is not None else C.byref(_lib.default_phys())
 
 
-def volumerhs(dim, N, rhs, Q, vgeo, D, elems, eqn=EULER, phys=None, stream=None):
-    """volumerhs!(Val(dim), Val(N), rhs, Q, vgeo, D, elems): rhs += volume term."""
+def volumerhs(dim, N, rhs, Q, vgeo, D, elems, eqn=EULER, phys=None, stream=None, prec=FP64):
+    """volumerhs!(Val(dim), Val(N), rhs, Q, vgeo, D, elems): rhs += volume term.
+
+    prec: FP64 (the reference evaluation order), FP64_FMA (contracted) or
+    FP64_MFMA (contracted, the derivative contractions on the matrix cores;
+    3D Euler only)."""
+    if prec not in (FP64, FP64_FMA, FP64_MFMA):
+        raise ValueError("volumerhs: prec must be FP64, FP64_FMA or FP64_MFMA")
     n = _nreal(elems)
     Np = (N + 1) ** dim
     ns = _NS[(eqn, dim)]
@@ -71,7 +77,7 @@ def volumerhs(dim, N, rhs, Q, vgeo, D, elems, eqn=EULER, phys=None, stream=None)
     _need(Q, "Q", Np * ns * n)
     _need(vgeo, "vgeo", Np * _NV[dim] * n)
     _need(D, "D", (N + 1) ** 2)
-    rc = lib().canary_volumerhs(eqn, dim, N, FP64, _ptr(rhs), _ptr(Q), _ptr(vgeo), _ptr(D), n,
+    rc = lib().canary_volumerhs(eqn, dim, N, prec, _ptr(rhs), _ptr(Q), _ptr(vgeo), _ptr(D), n,
                                 _phys(phys), _stream(stream))
     check(rc, "volumerhs")
 
@@ -125,12 +131,13 @@ def stage_fused(dim, N, rhs, Qin, Qout, vgeo, sgeo, D, elems, vmapM, vmapP, elem
     """One LSRK stage (volume + faces + update) in one pass; Qin -> Qout.
 
     prec: FP64 (the reference evaluation order, bit-identical to volumerhs +
-    facerhs + updatesolution) or FP64_FMA (multiply-adds fused, pressure via
-    exp/log; within 1e-12 relative L2 per stage, faster).  advection
-    (shallow water): the velocity is kept through the update, as in
-    updatesolution."""
-    if prec not in (FP64, FP64_FMA):
-        raise ValueError("stage_fused: prec must be FP64 or FP64_FMA")
+    facerhs + updatesolution), FP64_FMA (multiply-adds fused, pressure via
+    exp/log; within 1e-12 relative L2 per stage, faster) or FP64_MFMA
+    (FP64_FMA with the derivative contractions on the matrix cores; 3D Euler
+    only).  advection (shallow water): the velocity is kept through the
+    update, as in updatesolution."""
+    if prec not in (FP64, FP64_FMA, FP64_MFMA):
+        raise ValueError("stage_fused: prec must be FP64, FP64_FMA or FP64_MFMA")
     n = _nreal(elems)
     Np, Nfp, nf = (N + 1) ** dim, (N + 1) ** (dim - 1), 2 * dim
     ns = _NS[(eqn, dim)]
@@ -240,9 +247,10 @@ def prepare_geometry(dim, N, vgeo, sgeo, vmapM, vmapP, elemtobndy, nelem, nelem_
 
 def stage_prepared(dim, N, rhs, Qin, Qout, prep, D, elems, rka, rkb, dt, eqn=EULER, phys=None,
                    nelem_total=None, stream=None, prec=FP64, advection=False):
-    """stage_fused over the prepared arrays `prep` (prepare_geometry)."""
-    if prec not in (FP64, FP64_FMA):
-        raise ValueError("stage_prepared: prec must be FP64 or FP64_FMA")
+    """stage_fused over the prepared arrays `prep` (prepare_geometry); prec as
+    for stage_fused."""
+    if prec not in (FP64, FP64_FMA, FP64_MFMA):
+        raise ValueError("stage_prepared: prec must be FP64, FP64_FMA or FP64_MFMA")
     n = _nreal(elems)
     Np, Nfp, nf = (N + 1) ** dim, (N + 1) ** (dim - 1), 2 * dim
     ns = _NS[(eqn, dim)]

@@ -57,10 +57,19 @@ extern "C" {
 /* FP64 with a*b+c contracted into fused multiply-adds and 1/rho as a
  * Newton-refined reciprocal: fewer FP64 instructions, not the reference's
  * bit pattern (within 1e-12 relative L2 per stage of it).  Accepted by the
- * fused entry points and canary_lsrk54; CANARY_FP64 is the exact reference
- * order (both evaluate the Euler pressure's pow as exp(kappa log x) by
- * series, exact at x = 1). */
+ * fused entry points, canary_volumerhs and canary_lsrk54; CANARY_FP64 is the
+ * exact reference order (both evaluate the Euler pressure's pow as
+ * exp(kappa log x) by series, exact at x = 1). */
 #define CANARY_FP64_FMA 2
+/* CANARY_FP64_FMA with the volume term's derivative contractions (the sums
+ * over D along xi, eta, zeta) on the matrix cores (FP64 MFMA, 16x16x4
+ * tiles): 3D Euler only, accepted by canary_volumerhs, canary_stage_fused,
+ * canary_stage_prepared and canary_lsrk54 / canary_lsrk54_prepared (whose
+ * face + update path runs that fusion contracted).  Within the contracted
+ * form's tolerance of the reference.  A tile does useful work in at most half
+ * of its multiply-adds (NB = 16 / (N+1) lines side by side), and on MI355X
+ * the FP64 matrix rate equals the FP64 vector rate. */
+#define CANARY_FP64_MFMA 3
 
 /* Physical constants passed at run time (reference defaults in
  * canary_phys_default: examples/3d_kernels/euler3d_set2c.jl:58-63 and

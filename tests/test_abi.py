@@ -49,9 +49,16 @@ def test_device_entry_points_reject_bad_arguments_before_launch():
     assert lib.canary_facerhs(0, 3, 4, 0, None, None, None, 4, 2, None, None, None, ctypes.byref(p), None) == L.ERR_ARG
     # zero elements is a no-op
     assert lib.canary_updatesolution(0, 2, 3, 0, None, None, None, 0, 0.0, 0.0, 0.0, 0, None) == 0
-    # the contracted FP64 form exists for the fused stage only
-    assert lib.canary_volumerhs(0, 3, 4, L.FP64_FMA, None, None, None, None, 1, ctypes.byref(p), None) == L.ERR_UNSUPPORTED
-    assert lib.canary_updatesolution(0, 2, 3, L.FP64_FMA, None, None, None, 1, 0.0, 0.0, 0.0, 0, None) == L.ERR_UNSUPPORTED
+    # the contracted FP64 forms: volume term and fused stages (the matrix-core
+    # one for 3D Euler only), no facerhs or updatesolution of their own
+    for prec in (L.FP64_FMA, L.FP64_MFMA):
+        assert lib.canary_volumerhs(0, 3, 4, prec, None, None, None, None, 1, ctypes.byref(p), None) == L.ERR_ARG
+        assert lib.canary_updatesolution(0, 2, 3, prec, None, None, None, 1, 0.0, 0.0, 0.0, 0, None) == L.ERR_UNSUPPORTED
+        assert lib.canary_facerhs(0, 3, 4, prec, None, None, None, 4, 4, None, None, None, ctypes.byref(p), None) == L.ERR_UNSUPPORTED
+    for eqn, dim in ((0, 2), (1, 2), (2, 2)):
+        assert lib.canary_volumerhs(eqn, dim, 4, L.FP64_MFMA, None, None, None, None, 1, ctypes.byref(p), None) == L.ERR_UNSUPPORTED
+        assert lib.canary_stage_fused(eqn, dim, 4, L.FP64_MFMA, None, None, None, None, None, None, 1, 1,
+                                      None, None, None, 0.0, 0.0, 0.0, 0, ctypes.byref(p), None) == L.ERR_UNSUPPORTED
     for eqn, dim in ((0, 3), (0, 2), (1, 2)):
         assert lib.canary_stage_fused(eqn, dim, 4, L.FP64_FMA, None, None, None, None, None, None, 0, 0,
                                       None, None, None, 0.0, 0.0, 0.0, 0, ctypes.byref(p), None) == 0

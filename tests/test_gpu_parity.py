@@ -270,6 +270,54 @@ def test_null_vmapM_is_the_canonical_face_mask(cuda, eqn, dim, N):
     assert not bad, "; ".join(bad)
 
 
+@pytest.mark.parametrize("eqn,dim,N", [(0, 3, 4), (0, 3, 1), (0, 3, 2), (0, 2, 3), (1, 2, 4)])
+def test_non_canonical_vmapM(cuda, eqn, dim, N):
+    """A valid vmapM that is not the canonical face mask (the M-side ids of
+    two nodes of one face swapped in every third element): the face terms
+    follow the map (the kernels fall back from gathering them by node to
+    scattering them by face item, per workgroup), bit for bit the oracle's
+    in facerhs, the fused stage and the face + update fusion."""
+    import copy
+    print("non-canonical vmapM eqn=%d dim=%d N=%d" % (eqn, dim, N), flush=True)
+    s = case_setup(eqn, dim, N, periodic=(True,) + (False,) * (dim - 1))
+    vM = np.array(s.vmapM, order="F", copy=True)
+    for e in range(0, s.nelem, 3):
+        f = e % (2 * dim)
+        vM[0, f, e], vM[1, f, e] = vM[1, f, e], vM[0, f, e]
+    so = copy.copy(s)
+    so.vmapM = vM
+    ds = DeviceSetup(so)
+    Q = np.asfortranarray(state(s, seed=13, powfree=eqn == 0))
+    rhs0 = np.asfortranarray(np.random.default_rng(14).standard_normal(Q.shape) * 10)
+    if eqn == 1:
+        rhs0[:, 3, :] = 0.0
+    A, B, _ = P.rk_coefs()
+    want_r = rhs0.copy(order="F")
+    assert O.facerhs(so, want_r, Q) == 0
+    r = dev(rhs0)
+    K.facerhs(dim, N, r, dev(Q), ds.sgeo, s.nelem, ds.vmapM, ds.vmapP, ds.bndy, eqn=eqn,
+              phys=s.phys)
+    K.check_errors()
+    assert max_ulp(host(r, Q.shape), want_r) == 0
+    want_r = rhs0.copy(order="F")
+    want_q = Q.copy(order="F")
+    O.volumerhs(so, want_r, want_q)
+    assert O.facerhs(so, want_r, want_q) == 0
+    O.updatesolution(so, want_r, want_q, A[4], B[3], 0.01)
+    qo, r = dev(np.zeros_like(Q)), dev(rhs0)
+    K.stage_fused(dim, N, r, dev(Q), qo, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM, ds.vmapP,
+                  ds.bndy, A[4], B[3], 0.01, eqn=eqn, phys=s.phys)
+    K.check_errors()
+    assert max_ulp(host(qo, Q.shape), want_q) == 0 and max_ulp(host(r, Q.shape), want_r) == 0
+    r = dev(rhs0)
+    K.volumerhs(dim, N, r, dev(Q), ds.vgeo, ds.D, s.nelem, eqn=eqn, phys=s.phys)
+    qo = dev(np.zeros_like(Q))
+    K.facerhs_update(dim, N, r, dev(Q), qo, ds.vgeo, ds.sgeo, s.nelem, ds.vmapM, ds.vmapP,
+                     ds.bndy, A[4], B[3], 0.01, eqn=eqn, phys=s.phys)
+    K.check_errors()
+    assert max_ulp(host(qo, Q.shape), want_q) == 0 and max_ulp(host(r, Q.shape), want_r) == 0
+
+
 @pytest.mark.parametrize("eqn,dim,N", [(0, 3, 4), (0, 3, 2), (0, 2, 5), (1, 2, 3), (1, 2, 7)])
 def test_both_element_orders_give_the_same_bits(cuda, eqn, dim, N):
     """A ping-pong stage walks its element blocks forwards or backwards by the
