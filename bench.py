@@ -299,6 +299,8 @@ def main():
     ap.add_argument("--served-requests", type=int, default=64,
                     help="requests of the served-form run (0: skip)")
     ap.add_argument("--served-rate", type=float, default=200.0)
+    ap.add_argument("--served-graph", action="store_true",
+                    help="served form: also serve the stream with step-graph replay")
     ap.add_argument("--form", choices=["batched", "served", "sweep"], default=None,
                     help="run only that added form and print its JSON (used by the main run, "
                          "which measures each added form in a child process)")
@@ -334,7 +336,7 @@ def main():
         elif args.form == "sweep":
             res = sweep_form()
         else:
-            res = served_form(args.served_requests, args.served_rate)
+            res = served_form(args.served_requests, args.served_rate, graph=args.served_graph)
         print(json.dumps(res), flush=True)
         return
 
@@ -1054,7 +1056,7 @@ def batched_form(nsolves, steps, seed=42):
     return out
 
 
-def served_form(nreq, rate, seed=2024):
+def served_form(nreq, rate, seed=2024, graph=False):
     """Poisson request stream (half 2D vortices, half 3D bubbles) through the
     continuous-batching server (FP32 perturbation form; each request's
     preparation -- host mesh, device geometry and states -- inside its
@@ -1101,6 +1103,21 @@ def served_form(nreq, rate, seed=2024):
     print("served in lock step: p50 %.3f s p99 %.3f s" % (lock["p50_latency_s"],
                                                           lock["p99_latency_s"]),
           file=sys.stderr, flush=True)
+    graphed = None
+    if graph:
+        # the tiered server again, each batch step replayed from its captured
+        # hipGraph (re-captured as requests join and leave); same results
+        srv_g = Server("fp32", tier_work=5e8, tier_ratio=8, hbm_budget=budget, graph=True)
+        out_g = srv_g.serve(poisson_stream(n=nreq, rate=rate, seed=seed))
+        st_g = summary(out_g, srv_g.makespan)
+        by_rid = {r.rid: r.result for r in out_g}
+        graphed = {k: round(st_g[k], 4) for k in ("p50_latency_s", "p99_latency_s",
+                                                  "max_latency_s", "makespan_s",
+                                                  "aggregate_gdof_stage_s")}
+        graphed["same_results"] = all(np.array_equal(r.result, by_rid[r.rid]) for r in out_reqs)
+        del out_g, by_rid, srv_g
+        print("served with graph replay: p50 %.3f s p99 %.3f s" % (
+            graphed["p50_latency_s"], graphed["p99_latency_s"]), file=sys.stderr, flush=True)
     # sequential plain FP64 solves of the same requests
     timed = _gpu_timer()
     A, B, _ = P.rk_coefs()
@@ -1147,6 +1164,7 @@ def served_form(nreq, rate, seed=2024):
             "requests_that_waited_for_room": sum(1 for r in out_reqs
                                                  if r.admitted - r.prepared > 5e-3),
             "lock_step": lock,
+            "graph_replay": graphed,
             "sequential_fp64_gpu_s": round(t_seq, 4),
             "sequential_fp64_gdof_stage_s": round(st["dof_stages"] / t_seq / 1e9, 3),
             "fp32_rel_l2_vs_fp64": {"max": float("%.3e" % max(errs)),

@@ -56,36 +56,49 @@ struct MfmaSweep {
     return (n % (NQ * NQ)) * NQ + n / (NQ * NQ);
   }
 
-  // This lane's A operand of each K step (the block-diagonal D^T).
-  __device__ static __forceinline__ void a_operand(const double* sD, int lane, double* a) {
+  // The tile-invariant part of a lane's work: its A operand of each K step
+  // (the block-diagonal D^T) and, for its B operand words and its four
+  // result words, the offset from column*NQ and the bound on its column in
+  // line product 0 (0: the word is padding).
+  struct Lane {
+    double a[KS];
+    int boff[KS], blim[KS];
+    int ooff[4], olim[4];
+  };
+
+  __device__ static __forceinline__ void lane_setup(const double* sD, int lane, Lane& L) {
     const int row = lane % 16, rb = row / NQ, i = row - rb * NQ;
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
       const int k = 4 * ks + lane / 16, kb = k / NQ, m = k - kb * NQ;
       const bool on = rb == kb && rb < NB;
       const double v = sD[on ? m + NQ * i : 0];
-      a[ks] = on ? v : 0.0;
+      L.a[ks] = on ? v : 0.0;
+      L.boff[ks] = kb * 16 * NQ + m;
+      L.blim[ks] = kb < NB ? NCOL - kb * 16 : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int orow = lane / 16 + 4 * r, ob = orow / NQ, oi = orow - ob * NQ;
+      L.ooff[r] = ob * 16 * NQ + oi;
+      L.olim[r] = ob < NB ? NCOL - ob * 16 : 0;
     }
   }
 
   // One tile, in place in `buf` (one direction of one element, line by line).
-  __device__ static __forceinline__ void tile(double* buf, int t, const double* a, int lane) {
+  __device__ static __forceinline__ void tile(double* buf, int t, const Lane& L, int lane) {
     const int c = t * COLS + lane % 16;  // this lane's column in line product 0
+    const int cw = c * NQ;
     mfma_v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
-      const int k = 4 * ks + lane / 16, kb = k / NQ, m = k - kb * NQ;
-      const int gc = c + kb * 16;
-      const bool on = kb < NB && gc < NCOL;
-      const double v = buf[on ? gc * NQ + m : 0];
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ks], on ? v : 0.0, acc, 0, 0, 0);
+      const bool on = c < L.blim[ks];
+      const double v = buf[on ? cw + L.boff[ks] : 0];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(L.a[ks], on ? v : 0.0, acc, 0, 0, 0);
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = lane / 16 + 4 * r, rb = row / NQ, i = row - rb * NQ;
-      const int gc = c + rb * 16;
-      if (rb < NB && gc < NCOL) buf[gc * NQ + i] = acc[r];
-    }
+    for (int r = 0; r < 4; ++r)
+      if (c < L.olim[r]) buf[cw + L.ooff[r]] = acc[r];
   }
 };
 
@@ -100,14 +113,14 @@ __device__ __forceinline__ void mfma_sweeps(double* base, int dstride, int sstri
   constexpr int NW = NT / 64;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  double a[M::KS];
-  M::a_operand(sD, lane, a);
+  typename M::Lane L;
+  M::lane_setup(sD, lane, L);
   const int nitem = ND * nvalid * M::NTILE;
   for (int it = w; it < nitem; it += NW) {
     const int dd = it / (nvalid * M::NTILE);
     const int rest = it - dd * nvalid * M::NTILE;
     const int sl = rest / M::NTILE, t = rest - sl * M::NTILE;
-    M::tile(base + dd * dstride + sl * sstride, t, a, lane);
+    M::tile(base + dd * dstride + sl * sstride, t, L, lane);
   }
 }
 

@@ -14,8 +14,8 @@ A request names an equation set / problem, a mesh size, N and an end time
     dimension (Batch32: FP32 perturbation form; or Batch: plain FP64), so
     the requests on the GPU advance together, the 2D and 3D batches on their
     own streams (k steps per call of the library's batch loop,
-    canary_batch_run: replaying captured step graphs inside this loop is
-    switched off, see Server),
+    canary_batch_run, or, with graph=True, k replays of the batch's captured
+    step graph, re-captured when requests join or leave),
   * retires a request when it reaches its end time (its state is copied to
     the host) and admits waiting ones in the freed room.
 
@@ -125,15 +125,11 @@ class Server:
         self.max_solves = int(max_solves)
         self.step_chunk = int(step_chunk)
         self.prep_workers = int(prep_workers)
-        if graph:
-            # Replaying step graphs inside the serving loop (batches whose
-            # graphs are re-captured as requests join and leave, worker
-            # threads preparing meshes meanwhile) ended twice in a host
-            # segmentation fault inside hipGraphLaunch; its cause is not
-            # established, so this path stays off.  Batches outside the loop
-            # (Batch.run) replay graphs normally.
-            raise NotImplementedError("graph replay in the serving loop is switched off")
-        self.graph = False
+        # graph: replay each batch's step graphs (Batch.run(graph=True)); the
+        # worker threads preparing meshes meanwhile make no HIP call that
+        # could race with the launches (a collected stream is queued, not
+        # destroyed, and reaped from this thread: streams.reap)
+        self.graph = bool(graph)
         self.keep_setups = keep_setups
         self.device = torch.device(device)
         self.epw = dict(epw or {})   # (dim, N) -> elements per workgroup

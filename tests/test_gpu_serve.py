@@ -81,6 +81,26 @@ def test_served_in_mix_equals_served_alone(cuda):
         assert max_ulp(got.result, r.result) == 0
 
 
+def test_graph_replay_in_the_serving_loop(cuda):
+    """Each batch step replayed from its captured hipGraph (graphs
+    re-captured as requests join and leave, meshes prepared on worker
+    threads meanwhile, work tiers): every result equals the launch-by-launch
+    server's, bit for bit."""
+    reqs = small_stream(seed=13, n=8)
+    hosts = {r.rid: make_setup_host(r) for r in reqs}
+    works = sorted(hosts[r.rid].ndof() * r.nsteps for r in reqs)
+    plain = Server("fp32", step_chunk=2).serve(reqs, setups=hosts)
+    by = {r.rid: r for r in plain}
+    for tier in (None, works[len(works) // 2]):
+        again = [Request(rid=r.rid, kind=r.kind, Ne=r.Ne, N=r.N, nsteps=r.nsteps,
+                         arrival=r.arrival, seed=r.seed) for r in reqs]
+        print("graph replay, tier_work %s" % tier, flush=True)
+        got = Server("fp32", step_chunk=2, graph=True, tier_work=tier, tier_ratio=3).serve(
+            again, setups=None if tier is None else hosts)
+        for r in got:
+            assert max_ulp(r.result, by[r.rid].result) == 0, r.rid
+
+
 def test_work_tiers_give_the_same_results(cuda):
     """With work tiers (small requests in a batch of their own, the large
     ones advanced every third iteration) every result equals the untiered

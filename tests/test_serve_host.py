@@ -42,9 +42,9 @@ def test_summary_percentiles():
     assert abs(s["aggregate_gdof_stage_s"] - 100 * 5 * 1000 * 10 / 200.0 / 1e9) < 1e-18
 
 
-def test_graph_replay_in_serving_loop_is_off():
-    with pytest.raises(NotImplementedError):
-        Server("fp32", graph=True)
+def test_graph_replay_is_an_option_of_the_serving_loop():
+    assert Server("fp32", graph=True).graph is True
+    assert Server("fp32").graph is False
 
 
 def test_collected_streams_are_queued_not_destroyed():
