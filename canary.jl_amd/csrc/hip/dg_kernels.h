@@ -571,7 +571,9 @@ __global__ __launch_bounds__(256) void update_kernel(double* __restrict__ rhs,
 //      arrived; volume fluxes -> LDS;
 //   3. the xi/eta/zeta sweeps run from LDS while the gathers are in flight
 //      (barriers fence LDS only, so loads stay outstanding across them);
-//   4. face fluxes -> LDS, scatter onto the staged rhs (DIM rounds);
+//   4. face fluxes -> LDS, taken into the staged rhs by each node's thread
+//      in ascending face order (or scattered in DIM rounds when vmapM is not
+//      the canonical face mask);
 //   5. update: Qout = Q + ((rkb*dt)*rhs)*MJI, rhs *= rka, in LDS;
 //   6. the block's Qout and rhs written out in storage order.
 template <class Phys, int NQ, int EPW, bool WIDE>
