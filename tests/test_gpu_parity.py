@@ -193,6 +193,29 @@ def test_fma_stage_parity(cuda, eqn, dim, N):
             assert np.array_equal(got_q[:, 3, :], Q[:, 3, :])  # b is never evolved
 
 
+@pytest.mark.parametrize("eqn,dim,N", CASES)
+def test_contracted_volume_term(cuda, eqn, dim, N):
+    """canary_volumerhs in the contracted form (CANARY_FP64_FMA): within
+    1e-12 of the oracle's volumerhs!, rhs_in zero and random; shallow
+    water leaves b's rhs untouched."""
+    print("contracted volume eqn=%d dim=%d N=%d" % (eqn, dim, N), flush=True)
+    s = case_setup(eqn, dim, N, (True,) * dim if N == 3 else None)
+    ds = DeviceSetup(s)
+    Q = np.asfortranarray(state(s, seed=23 + N))
+    rng = np.random.default_rng(N)
+    for rhs0 in (s.zeros_state(), np.asfortranarray(rng.standard_normal(Q.shape))):
+        want = rhs0.copy(order="F")
+        O.volumerhs(s, want, Q)
+        r = dev(rhs0)
+        K.volumerhs(dim, N, r, dev(Q), ds.vgeo, ds.D, s.nelem, eqn=eqn, phys=s.phys,
+                    prec=FP64_FMA)
+        K.check_errors()
+        got = host(r, Q.shape)
+        assert rel_l2(got, want) <= 1e-12
+        if eqn == 1:
+            assert np.array_equal(got[:, 3, :], rhs0[:, 3, :])
+
+
 @pytest.mark.parametrize("eqn,dim,N", [(0, 3, 4), (0, 3, 2), (0, 2, 5), (1, 2, 4), (1, 2, 7)])
 def test_facerhs_update_equals_face_then_update(cuda, eqn, dim, N):
     """The face + update fusion after volumerhs: bit-identical to facerhs then

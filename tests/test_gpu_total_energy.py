@@ -125,6 +125,24 @@ def test_contracted_loop_after_100_steps(cuda):
     assert np.max(np.abs(Q[:, 1, :])) > 0
 
 
+@pytest.mark.parametrize("N", [2, 4, 7])
+def test_contracted_volume_term(cuda, N):
+    """canary_volumerhs in the contracted form (CANARY_FP64_FMA) reads the
+    node heights too: within 1e-12 of the oracle's volumerhs!."""
+    s = setup(N, (True, False), warped=N == 4)
+    ds = DeviceSetup(s)
+    Q = np.asfortranarray(smooth_te_state(s, seed=N))
+    want = s.zeros_state()
+    O.volumerhs(s, want, Q)
+    r = dev(s.zeros_state())
+    K.volumerhs(2, N, r, dev(Q), ds.vgeo, ds.D, s.nelem, eqn=EULER_TE, phys=s.phys,
+                prec=FP64_FMA)
+    K.check_errors()
+    e = rel_l2(host(r, Q.shape), want)
+    print("contracted total-energy volume term N=%d: rel L2 %.3e" % (N, e), flush=True)
+    assert e <= 1e-12
+
+
 def test_face_term_needs_the_heights(cuda):
     """canary_facerhs has no vgeo: the total-energy face term is refused
     there, and the prepared arrays (which hold no heights) have no
