@@ -375,12 +375,13 @@ def main():
                 q, q2 = q2, q
         return q, q2
 
-    def run_plain(q, rhs, nsteps):
+    def run_plain(q, rhs, nsteps, prec=FP64):
         for _ in range(nsteps):
             for st in range(5):
-                K.volumerhs(3, s.N, rhs, q, vgeo, D, s.nelem, phys=s.phys)
-                K.facerhs(3, s.N, rhs, q, sgeo, s.nelem, vM, vP, bnd, phys=s.phys)
-                K.updatesolution(3, s.N, rhs, q, vgeo, s.nelem, A[(st + 1) % 5], B[st], dt)
+                K.volumerhs(3, s.N, rhs, q, vgeo, D, s.nelem, phys=s.phys, prec=prec)
+                K.facerhs(3, s.N, rhs, q, sgeo, s.nelem, vM, vP, bnd, phys=s.phys, prec=prec)
+                K.updatesolution(3, s.N, rhs, q, vgeo, s.nelem, A[(st + 1) % 5], B[st], dt,
+                                 prec=prec)
 
     def timed(fn):
         torch.cuda.synchronize()
@@ -615,6 +616,13 @@ def main():
         st_plain = 5 * max(1, args.steps // 2)
         extra["plain_three_kernel_gdof_stage_s"] = round(dof * st_plain / t_plain / 1e9, 3)
         extra["plain_ms_per_stage"] = round(t_plain / st_plain * 1e3, 4)
+        # the same three kernels in the contracted form (CANARY_FP64_FMA)
+        qc3, rc3 = Q0.clone(), torch.zeros_like(Q0)
+        run_plain(qc3, rc3, 1, FP64_FMA)
+        t_c3, _ = timed(lambda: run_plain(qc3, rc3, max(1, args.steps // 2), FP64_FMA))
+        K.check_errors()
+        extra["plain_three_kernel_contracted_gdof_stage_s"] = round(dof * st_plain / t_c3 / 1e9, 3)
+        del qc3, rc3
         # volumerhs, then facerhs + updatesolution fused (Q ping-pong)
         q2p = torch.empty_like(qp)
 

@@ -59,14 +59,17 @@ int step(Run& r, int& cur) {
     const double rka = r.A[(s + 1) % 5], rkb = r.B[s];
     int rc = 0;
     if (r.path == CANARY_PATH_KERNELS) {
+      // matrix-core form: its volume term, then facerhs and the update
+      // contracted (they have no derivative contraction)
       void* q = r.Q[0];
+      const int pf = r.prec == CANARY_FP64_MFMA ? CANARY_FP64_FMA : r.prec;
       rc = canary_volumerhs(r.eqn, r.dim, r.N, r.prec, r.rhs, q, r.vgeo, r.D, r.nreal, r.phys,
                             r.stream);
       if (!rc)
-        rc = canary_facerhs_geo(r.eqn, r.dim, r.N, r.prec, r.rhs, q, r.sgeo, r.vgeo, r.nreal,
+        rc = canary_facerhs_geo(r.eqn, r.dim, r.N, pf, r.rhs, q, r.sgeo, r.vgeo, r.nreal,
                                 r.ntotal, r.vmapM, r.vmapP, r.bndy, r.phys, r.stream);
       if (!rc)
-        rc = canary_updatesolution(r.eqn, r.dim, r.N, r.prec, r.rhs, q, r.vgeo, r.nreal, rka, rkb,
+        rc = canary_updatesolution(r.eqn, r.dim, r.N, pf, r.rhs, q, r.vgeo, r.nreal, rka, rkb,
                                    r.dt, r.advection, r.stream);
     } else if (r.path == PATH_PREPARED) {
       rc = canary_stage_prepared(r.eqn, r.dim, r.N, r.prec, r.rhs, r.Q[cur], r.Q[1 - cur], r.vgeo,
@@ -195,7 +198,8 @@ extern "C" int canary_lsrk54(int eqn, int dim, int N, int prec, int path, int64_
   if (!result_in_alt || nsteps < 0) return CANARY_ERR_ARG;
   if (path != CANARY_PATH_KERNELS && path != CANARY_PATH_FUSED && path != CANARY_PATH_FACE_UPDATE)
     return CANARY_ERR_ARG;
-  if (path == CANARY_PATH_KERNELS && prec != CANARY_FP64) return CANARY_ERR_UNSUPPORTED;
+  if (prec != CANARY_FP64 && prec != CANARY_FP64_FMA && prec != CANARY_FP64_MFMA)
+    return CANARY_ERR_UNSUPPORTED;
   if (advection && eqn != CANARY_SWE) return CANARY_ERR_ARG;
   if (path != CANARY_PATH_KERNELS && (!Qalt || Qalt == Q)) return CANARY_ERR_ARG;
   // the loop exchanges no ghost elements: they would be frozen in Q and, on

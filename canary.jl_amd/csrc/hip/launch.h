@@ -33,11 +33,10 @@ struct KArgs {
 
 constexpr int kMaxN = 7;  // N = 1..7 compiled in
 
-// The kernel families of a translation unit: the exact-order unit holds every
-// operation; the contracted unit (built with -ffp-contract=fast) the fused
-// stages, face + update and the volume term; the matrix-core unit the
-// contracted unit's volume term and fused stages with the derivative
-// contractions on the MFMA units (MfmaSweep).
+// The kernel families of a translation unit: the exact-order unit and the
+// contracted unit (built with -ffp-contract=fast) hold every operation; the
+// matrix-core unit the contracted unit's volume term and fused stages with
+// the derivative contractions on the MFMA units (MfmaSweep).
 enum Unit { UNIT_EXACT = 0, UNIT_FMA = 1, UNIT_MFMA = 2 };
 
 inline int launch_status() {
@@ -108,6 +107,27 @@ void launch_volume(int64_t nb, const KArgs& a) {
 }
 
 template <class Phys, int NQ, int EPW>
+void launch_face(int64_t nb, const KArgs& a) {
+  constexpr int NT = FusedCfg<Phys, NQ, EPW, false>::NT;
+  const int64_t chunk = (nb + 7) / 8;
+  face_kernel<Phys, NQ, EPW><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
+      (double*)a.rhs, (const double*)a.Q, (const double*)a.vgeo, (const double*)a.sgeo, a.nreal,
+      a.ntotal, a.vmapM, a.vmapP, a.bndy, chunk, inplace_direction(), a.phys);
+}
+
+// Nonzero: the grid would not fit a launch.
+template <class Phys, int NQ>
+int launch_update(const KArgs& a) {
+  const int64_t total = a.nreal * Shape<Phys, NQ>::Np;
+  const int64_t chunk = ((total + 255) / 256 + 7) / 8;
+  if (8 * chunk > 0x7fffffffLL) return 1;
+  update_kernel<Phys, NQ><<<dim3((unsigned)(8 * chunk)), dim3(256), 0, a.stream>>>(
+      (double*)a.rhs, (double*)a.Qout, (const double*)a.vgeo, a.nreal, a.rka, a.rkb, a.dt,
+      a.advection, chunk, inplace_direction());
+  return 0;
+}
+
+template <class Phys, int NQ, int EPW>
 void launch_face_update(int64_t nb, const KArgs& a) {
   constexpr int NT = FusedCfg<Phys, NQ, EPW, false>::NT;
   const int64_t chunk = (nb + 7) / 8;
@@ -132,37 +152,30 @@ int launch(int op, const KArgs& a) {
       launch_stage_prep_if<Phys, NQ, EPW, MFMA>(nb, a);
     else if (op == OP_VOLUME)
       launch_volume<Phys, NQ, EPW, MFMA>(nb, a);
-    else if (op == OP_FACE_UPDATE) {
-      // (kept out of the matrix-core unit: one unit per kernel)
-      if constexpr (MFMA) return 2 /*CANARY_ERR_UNSUPPORTED*/;
-      else launch_face_update<Phys, NQ, EPW>(nb, a);
+    else if (op == OP_FACE_UPDATE || op == OP_FACE || op == OP_UPDATE) {
+      // no derivative contraction in these: the contracted unit holds them,
+      // the matrix-core unit does not (each kernel lives in one unit)
+      if constexpr (MFMA) {
+        return 2 /*CANARY_ERR_UNSUPPORTED*/;
+      } else {
+        if (op == OP_FACE_UPDATE) launch_face_update<Phys, NQ, EPW>(nb, a);
+        if (op == OP_FACE) launch_face<Phys, NQ, EPW>(nb, a);
+        if (op == OP_UPDATE && launch_update<Phys, NQ>(a)) return 1 /*CANARY_ERR_ARG*/;
+      }
     } else {
       return 2 /*CANARY_ERR_UNSUPPORTED*/;
     }
   } else {
-    double* rhs = (double*)a.rhs;
-    const double* Q = (const double*)a.Q;
     switch (op) {
       case OP_VOLUME:
         launch_volume<Phys, NQ, EPW, false>(nb, a);
         break;
-      case OP_FACE: {
-        constexpr int NT = FusedCfg<Phys, NQ, EPW, false>::NT;
-        const int64_t chunk = (nb + 7) / 8;
-        face_kernel<Phys, NQ, EPW><<<dim3((unsigned)(8 * chunk)), dim3(NT), 0, a.stream>>>(
-            rhs, Q, (const double*)a.vgeo, (const double*)a.sgeo, a.nreal, a.ntotal, a.vmapM,
-            a.vmapP, a.bndy, chunk, inplace_direction(), a.phys);
+      case OP_FACE:
+        launch_face<Phys, NQ, EPW>(nb, a);
         break;
-      }
-      case OP_UPDATE: {
-        const int64_t total = a.nreal * S::Np;
-        const int64_t chunk = ((total + 255) / 256 + 7) / 8;
-        if (8 * chunk > 0x7fffffffLL) return 1 /*CANARY_ERR_ARG*/;
-        update_kernel<Phys, NQ><<<dim3((unsigned)(8 * chunk)), dim3(256), 0, a.stream>>>(
-            rhs, (double*)a.Qout, (const double*)a.vgeo, a.nreal, a.rka, a.rkb, a.dt, a.advection,
-            chunk, inplace_direction());
+      case OP_UPDATE:
+        if (launch_update<Phys, NQ>(a)) return 1 /*CANARY_ERR_ARG*/;
         break;
-      }
       case OP_STAGE:
         launch_stage<Phys, NQ, EPW>(nb, a);
         break;
@@ -211,9 +224,8 @@ inline int read_status(int64_t* info, int clear) {
 
 }  // namespace canary
 
-// The contracted-FP64 unit of an equation set (fused stages, face + update,
-// volume term), built with -ffp-contract=fast from a physics policy whose
-// pressure avoids pow.
+// The contracted-FP64 unit of an equation set (every operation), built with
+// -ffp-contract=fast from a physics policy whose pressure avoids pow.
 #define CANARY_DEFINE_FMA_STAGE(NAME, PHYS)                                       \
   extern "C" int canary_k_##NAME##_fma(int op, int N, const canary::KArgs* a) {  \
     return canary::dispatch<PHYS, canary::UNIT_FMA>(op, N, *a);                   \

@@ -108,8 +108,8 @@ namespace {
 using KFn = int (*)(int, int, const canary::KArgs*);
 
 // CANARY_FP64 selects the exact-order units, CANARY_FP64_FMA the contracted
-// ones (fused stages, face + update, volume term), CANARY_FP64_MFMA the
-// matrix-core unit (3D Euler: volume term and fused stages).
+// ones (every operation), CANARY_FP64_MFMA the matrix-core unit (3D Euler:
+// volume term and fused stages).
 KFn select(int eqn, int dim, int prec) {
   if (prec == CANARY_FP64_MFMA)
     return eqn == CANARY_EULER && dim == 3 ? canary_k_euler3d_mfma : nullptr;
@@ -186,7 +186,7 @@ extern "C" int canary_facerhs_geo(int eqn, int dim, int N, int prec, void* rhs, 
                                   const int64_t* vmapP, const int64_t* elemtobndy,
                                   const canary_phys* phys, void* stream) {
   KFn fn;
-  int rc = check_common(eqn, dim, N, prec, nreal, &fn);
+  int rc = check_common(eqn, dim, N, prec, nreal, &fn, true);
   if (rc) return rc;
   if (nreal == 0) return 0;
   if (nelem_total < nreal) return CANARY_ERR_ARG;
@@ -219,7 +219,7 @@ extern "C" int canary_updatesolution(int eqn, int dim, int N, int prec, void* rh
                                      const void* vgeo, int64_t nreal, double rka, double rkb,
                                      double dt, int advection, void* stream) {
   KFn fn;
-  int rc = check_common(eqn, dim, N, prec, nreal, &fn);
+  int rc = check_common(eqn, dim, N, prec, nreal, &fn, true);
   if (rc) return rc;
   if (nreal == 0) return 0;
   if (!rhs || !Q || !vgeo) return CANARY_ERR_ARG;

@@ -83,10 +83,13 @@ def volumerhs(dim, N, rhs, Q, vgeo, D, elems, eqn=EULER, phys=None, stream=None,
 
 
 def facerhs(dim, N, rhs, Q, sgeo, elems, vmapM, vmapP, elemtobndy, eqn=EULER, phys=None,
-            nelem_total=None, stream=None, vgeo=None):
+            nelem_total=None, stream=None, vgeo=None, prec=FP64):
     """facerhs!(Val(dim), Val(N), rhs, Q, sgeo, elems, vmapM, vmapP, elemtobndy);
     the total-energy equations' fluxrhs! also takes vgeo (its pressure reads
-    the node height), of all nelem_total elements."""
+    the node height), of all nelem_total elements.  prec: FP64 (the
+    reference evaluation order) or FP64_FMA (contracted)."""
+    if prec not in (FP64, FP64_FMA):
+        raise ValueError("facerhs: prec must be FP64 or FP64_FMA")
     n = _nreal(elems)
     Np, Nfp, nf = (N + 1) ** dim, (N + 1) ** (dim - 1), 2 * dim
     ns = _NS[(eqn, dim)]
@@ -101,7 +104,7 @@ def facerhs(dim, N, rhs, Q, sgeo, elems, vmapM, vmapP, elemtobndy, eqn=EULER, ph
         raise ValueError("the total-energy face term needs vgeo (node heights)")
     if vgeo is not None:
         _need(vgeo, "vgeo", Np * _NV[dim] * ntot)
-    rc = lib().canary_facerhs_geo(eqn, dim, N, FP64, _ptr(rhs), _ptr(Q), _ptr(sgeo),
+    rc = lib().canary_facerhs_geo(eqn, dim, N, prec, _ptr(rhs), _ptr(Q), _ptr(sgeo),
                                   None if vgeo is None else _ptr(vgeo), n, ntot, pM,
                                   _ptr(vmapP), _ptr(elemtobndy), _phys(phys), _stream(stream))
     check(rc, "facerhs")
@@ -111,15 +114,18 @@ fluxrhs = facerhs  # the shallow-water example's name for the face kernel
 
 
 def updatesolution(dim, N, rhs, Q, vgeo, elems, rka, rkb, dt, eqn=EULER, advection=False,
-                   stream=None):
-    """updatesolution!(Val(dim), Val(N), rhs, Q, vgeo, elems, rka, rkb, dt)."""
+                   stream=None, prec=FP64):
+    """updatesolution!(Val(dim), Val(N), rhs, Q, vgeo, elems, rka, rkb, dt); prec
+    FP64 (the reference evaluation order) or FP64_FMA (contracted)."""
+    if prec not in (FP64, FP64_FMA):
+        raise ValueError("updatesolution: prec must be FP64 or FP64_FMA")
     n = _nreal(elems)
     Np = (N + 1) ** dim
     ns = _NS[(eqn, dim)]
     _need(rhs, "rhs", Np * ns * n)
     _need(Q, "Q", Np * ns * n)
     _need(vgeo, "vgeo", Np * _NV[dim] * n)
-    rc = lib().canary_updatesolution(eqn, dim, N, FP64, _ptr(rhs), _ptr(Q), _ptr(vgeo), n,
+    rc = lib().canary_updatesolution(eqn, dim, N, prec, _ptr(rhs), _ptr(Q), _ptr(vgeo), n,
                                      float(rka), float(rkb), float(dt), 1 if advection else 0,
                                      _stream(stream))
     check(rc, "updatesolution")

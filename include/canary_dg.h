@@ -56,16 +56,19 @@ extern "C" {
 #define CANARY_FP32 1
 /* FP64 with a*b+c contracted into fused multiply-adds and 1/rho as a
  * Newton-refined reciprocal: fewer FP64 instructions, not the reference's
- * bit pattern (within 1e-12 relative L2 per stage of it).  Accepted by the
- * fused entry points, canary_volumerhs and canary_lsrk54; CANARY_FP64 is the
- * exact reference order (both evaluate the Euler pressure's pow as
- * exp(kappa log x) by series, exact at x = 1). */
+ * bit pattern (within 1e-12 relative L2 per stage of it; the Euler
+ * pressure's pow as exp(kappa log x) by series, exact at x = 1).  Accepted
+ * by every FP64 entry point (the three drop-in kernels, the fused ones and
+ * the library loops); CANARY_FP64 is the exact reference order, the
+ * pressure's pow correctly rounded. */
 #define CANARY_FP64_FMA 2
 /* CANARY_FP64_FMA with the volume term's derivative contractions (the sums
  * over D along xi, eta, zeta) on the matrix cores (FP64 MFMA, 16x16x4
  * tiles): 3D Euler only, accepted by canary_volumerhs, canary_stage_fused,
  * canary_stage_prepared and canary_lsrk54 / canary_lsrk54_prepared (whose
- * face + update path runs that fusion contracted).  Within the contracted
+ * three-kernel and face + update paths run facerhs, updatesolution and that
+ * fusion contracted: they have no derivative contraction; called directly
+ * with this prec they report CANARY_ERR_UNSUPPORTED).  Within the contracted
  * form's tolerance of the reference.  A tile does useful work in at most half
  * of its multiply-adds (NB = 16 / (N+1) lines side by side), and on MI355X
  * the FP64 matrix rate equals the FP64 vector rate. */

@@ -49,12 +49,15 @@ def test_device_entry_points_reject_bad_arguments_before_launch():
     assert lib.canary_facerhs(0, 3, 4, 0, None, None, None, 4, 2, None, None, None, ctypes.byref(p), None) == L.ERR_ARG
     # zero elements is a no-op
     assert lib.canary_updatesolution(0, 2, 3, 0, None, None, None, 0, 0.0, 0.0, 0.0, 0, None) == 0
-    # the contracted FP64 forms: volume term and fused stages (the matrix-core
-    # one for 3D Euler only), no facerhs or updatesolution of their own
+    # the contracted FP64 form has every operation; the matrix-core one (3D
+    # Euler only) the volume term and fused stages, so its facerhs and
+    # updatesolution are refused by the unit, after the argument checks
     for prec in (L.FP64_FMA, L.FP64_MFMA):
         assert lib.canary_volumerhs(0, 3, 4, prec, None, None, None, None, 1, ctypes.byref(p), None) == L.ERR_ARG
-        assert lib.canary_updatesolution(0, 2, 3, prec, None, None, None, 1, 0.0, 0.0, 0.0, 0, None) == L.ERR_UNSUPPORTED
-        assert lib.canary_facerhs(0, 3, 4, prec, None, None, None, 4, 4, None, None, None, ctypes.byref(p), None) == L.ERR_UNSUPPORTED
+        assert lib.canary_updatesolution(0, 3, 3, prec, None, None, None, 1, 0.0, 0.0, 0.0, 0, None) == L.ERR_ARG
+        assert lib.canary_facerhs(0, 3, 4, prec, None, None, None, 4, 4, None, None, None, ctypes.byref(p), None) == L.ERR_ARG
+    assert lib.canary_updatesolution(0, 2, 3, L.FP64_MFMA, None, None, None, 1, 0.0, 0.0, 0.0, 0, None) == L.ERR_UNSUPPORTED
+    assert lib.canary_facerhs(1, 2, 4, L.FP64_MFMA, None, None, None, 4, 4, None, None, None, ctypes.byref(p), None) == L.ERR_UNSUPPORTED
     for eqn, dim in ((0, 2), (1, 2), (2, 2)):
         assert lib.canary_volumerhs(eqn, dim, 4, L.FP64_MFMA, None, None, None, None, 1, ctypes.byref(p), None) == L.ERR_UNSUPPORTED
         assert lib.canary_stage_fused(eqn, dim, 4, L.FP64_MFMA, None, None, None, None, None, None, 1, 1,
@@ -86,9 +89,9 @@ def test_lsrk54_tables_match_the_host_ones():
     assert lib.canary_lsrk54(0, 3, 4, 0, 1, 5, 0.1, None, None, None, None, None, None, 1, 1,
                              None, None, None, ctypes.byref(p), 0, 0, ctypes.byref(alt),
                              None) == L.ERR_ARG  # fused path needs Qalt
-    assert lib.canary_lsrk54(0, 3, 4, L.FP64_FMA, 0, 5, 0.1, None, None, None, None, None, None,
+    assert lib.canary_lsrk54(0, 3, 4, L.FP32, 0, 5, 0.1, None, None, None, None, None, None,
                              1, 1, None, None, None, ctypes.byref(p), 0, 0, ctypes.byref(alt),
-                             None) == L.ERR_UNSUPPORTED  # three kernels are exact order only
+                             None) == L.ERR_UNSUPPORTED  # FP64 forms only
     assert lib.canary_lsrk54(0, 3, 4, 0, 0, 0, 0.1, None, None, None, None, None, None, 1, 1,
                              None, None, None, ctypes.byref(p), 0, 0, ctypes.byref(alt),
                              None) == 0 and alt.value == 0  # no steps

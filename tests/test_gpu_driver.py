@@ -25,10 +25,12 @@ def python_loop(s, ds, Q, dt, nsteps, path, prec):
         for st in range(5):
             a, b = A[(st + 1) % 5], B[st]
             if path == PATH_KERNELS:
-                K.volumerhs(s.dim, s.N, rhs, q, ds.vgeo, ds.D, s.nelem, eqn=s.eqn, phys=s.phys)
+                K.volumerhs(s.dim, s.N, rhs, q, ds.vgeo, ds.D, s.nelem, eqn=s.eqn, phys=s.phys,
+                            prec=prec)
                 K.facerhs(s.dim, s.N, rhs, q, ds.sgeo, s.nelem, ds.vmapM, ds.vmapP, ds.bndy,
-                          eqn=s.eqn, phys=s.phys)
-                K.updatesolution(s.dim, s.N, rhs, q, ds.vgeo, s.nelem, a, b, dt, eqn=s.eqn)
+                          eqn=s.eqn, phys=s.phys, prec=prec)
+                K.updatesolution(s.dim, s.N, rhs, q, ds.vgeo, s.nelem, a, b, dt, eqn=s.eqn,
+                                 prec=prec)
                 continue
             if path == PATH_FUSED:
                 K.stage_fused(s.dim, s.N, rhs, q, q2, ds.vgeo, ds.sgeo, ds.D, s.nelem, ds.vmapM,
@@ -61,8 +63,9 @@ CASES = [(0, 3, 3), (0, 2, 4), (1, 2, 5)]
 
 
 @pytest.mark.parametrize("eqn,dim,N", CASES)
-@pytest.mark.parametrize("path,prec", [(PATH_KERNELS, FP64), (PATH_FUSED, FP64),
-                                       (PATH_FUSED, FP64_FMA), (PATH_FACE_UPDATE, FP64)])
+@pytest.mark.parametrize("path,prec", [(PATH_KERNELS, FP64), (PATH_KERNELS, FP64_FMA),
+                                       (PATH_FUSED, FP64), (PATH_FUSED, FP64_FMA),
+                                       (PATH_FACE_UPDATE, FP64)])
 @pytest.mark.parametrize("graph", [False, True])
 def test_native_loop_equals_python_loop(cuda, eqn, dim, N, path, prec, graph):
     print("driver eqn=%d dim=%d N=%d path=%d prec=%d graph=%s" % (eqn, dim, N, path, prec, graph),
@@ -83,10 +86,11 @@ def test_native_loop_matches_oracle_lowstoragerk(cuda):
     dt, nsteps = 0.02, 40
     Qo, ro = np.asfortranarray(s.Q.copy()), s.zeros_state()
     assert O.lsrk(s, Qo, ro, dt, nsteps, nthreads=8) == 0
-    for prec in (FP64, FP64_FMA):
-        got, _ = native_loop(s, ds, s.Q, dt, nsteps, PATH_FUSED, prec, True)
+    for path, prec in ((PATH_FUSED, FP64), (PATH_FUSED, FP64_FMA), (PATH_KERNELS, FP64_FMA)):
+        got, _ = native_loop(s, ds, s.Q, dt, nsteps, path, prec, True)
         e = rel_l2(got, Qo)
-        print("native loop (prec %d) vs oracle after %d steps: %.3e" % (prec, nsteps, e), flush=True)
+        print("native loop (path %d prec %d) vs oracle after %d steps: %.3e" % (path, prec, nsteps, e),
+              flush=True)
         assert e <= 1e-10
         if prec == FP64:
             assert max_ulp(got, Qo) == 0  # exact order, correctly rounded pow

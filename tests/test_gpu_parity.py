@@ -194,6 +194,34 @@ def test_fma_stage_parity(cuda, eqn, dim, N):
 
 
 @pytest.mark.parametrize("eqn,dim,N", CASES)
+def test_contracted_three_kernels(cuda, eqn, dim, N):
+    """The three drop-in kernels in the contracted form (CANARY_FP64_FMA):
+    volumerhs + facerhs within 1e-12 of the oracle's RHS, then
+    updatesolution within 1e-12 of its update."""
+    print("contracted three kernels eqn=%d dim=%d N=%d" % (eqn, dim, N), flush=True)
+    s = case_setup(eqn, dim, N, (True,) * dim if N == 3 else None)
+    ds = DeviceSetup(s)
+    Q = np.asfortranarray(state(s, seed=29 + N))
+    rhs0 = np.asfortranarray(np.random.default_rng(N).standard_normal(Q.shape) * 10)
+    if eqn == 1:
+        rhs0[:, 3, :] = 0.0
+    want = O.rhs_eval(s, Q, rhs0)
+    q, r = dev(Q), dev(rhs0)
+    K.volumerhs(dim, N, r, q, ds.vgeo, ds.D, s.nelem, eqn=eqn, phys=s.phys, prec=FP64_FMA)
+    K.facerhs(dim, N, r, q, ds.sgeo, s.nelem, ds.vmapM, ds.vmapP, ds.bndy, eqn=eqn, phys=s.phys,
+              prec=FP64_FMA)
+    K.check_errors()
+    got = host(r, Q.shape)
+    assert rel_l2(got, want) <= 1e-12
+    A, B, _ = P.rk_coefs()
+    Qo, ro = Q.copy(order="F"), got.copy(order="F")
+    O.updatesolution(s, ro, Qo, A[2], B[1], 0.01)
+    K.updatesolution(dim, N, r, q, ds.vgeo, s.nelem, A[2], B[1], 0.01, eqn=eqn, prec=FP64_FMA)
+    K.check_errors()
+    assert rel_l2(host(q, Q.shape), Qo) <= 1e-12 and rel_l2(host(r, Q.shape), ro) <= 1e-12
+
+
+@pytest.mark.parametrize("eqn,dim,N", CASES)
 def test_contracted_volume_term(cuda, eqn, dim, N):
     """canary_volumerhs in the contracted form (CANARY_FP64_FMA): within
     1e-12 of the oracle's volumerhs!, rhs_in zero and random; shallow

@@ -127,8 +127,9 @@ def test_contracted_loop_after_100_steps(cuda):
 
 @pytest.mark.parametrize("N", [2, 4, 7])
 def test_contracted_volume_term(cuda, N):
-    """canary_volumerhs in the contracted form (CANARY_FP64_FMA) reads the
-    node heights too: within 1e-12 of the oracle's volumerhs!."""
+    """canary_volumerhs and canary_facerhs_geo in the contracted form
+    (CANARY_FP64_FMA) read the node heights too: within 1e-12 of the
+    oracle's volumerhs! and of its RHS."""
     s = setup(N, (True, False), warped=N == 4)
     ds = DeviceSetup(s)
     Q = np.asfortranarray(smooth_te_state(s, seed=N))
@@ -140,6 +141,15 @@ def test_contracted_volume_term(cuda, N):
     K.check_errors()
     e = rel_l2(host(r, Q.shape), want)
     print("contracted total-energy volume term N=%d: rel L2 %.3e" % (N, e), flush=True)
+    assert e <= 1e-12
+    # and the face term (it reads the heights too), contracted
+    want = O.rhs_eval(s, Q, s.zeros_state())
+    q = dev(Q)
+    K.facerhs(2, N, r, q, ds.sgeo, s.nelem, ds.vmapM, ds.vmapP, ds.bndy, eqn=EULER_TE,
+              phys=s.phys, vgeo=ds.vgeo, prec=FP64_FMA)
+    K.check_errors()
+    e = rel_l2(host(r, Q.shape), want)
+    print("contracted total-energy RHS N=%d: rel L2 %.3e" % (N, e), flush=True)
     assert e <= 1e-12
 
 
