@@ -1210,7 +1210,7 @@ def summarize_trace(d, steps, ne, N):
         groups.setdefault(key, []).append((int(r["Dispatch_Id"]), t))
     print("%-96s %9s %4s %6s %9s %9s %9s" % ("kernel", "grid", "wg", "calls", "avg_us", "min_us",
                                              "max_us"))
-    for (name, grid, wg), v in sorted(groups.items(), key=lambda kv: -sum(t for _, t in kv[1]))[:30]:
+    for (name, grid, wg), v in sorted(groups.items(), key=lambda kv: -sum(t for _, t in kv[1]))[:40]:
         ts = [t for _, t in v]
         print("%-96s %9d %4d %6d %9.2f %9.2f %9.2f" % (_short(name), grid, wg, len(ts),
                                                        sum(ts) / len(ts), min(ts), max(ts)))
