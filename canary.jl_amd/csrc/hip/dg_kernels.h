@@ -293,7 +293,7 @@ struct Prepared {
 };
 
 // face_load over the prepared arrays: the face code goes to idP and is
-// turned into node ids by face_decode once it has arrived.
+// turned into node ids by face_decode_resolve once it has arrived.
 template <class Phys, int NQ>
 __device__ __forceinline__ void face_load_prep(FaceItem<Phys>& it,
                                                const typename Phys::Real* __restrict__ psgeo,
@@ -310,18 +310,28 @@ __device__ __forceinline__ void face_load_prep(FaceItem<Phys>& it,
   it.idP = pface[f + S::NFACE * e];
 }
 
+// face_decode + face_resolve over the prepared arrays, without building the
+// 1-based node ids and splitting them again: the code (an int32, as loaded)
+// gives the neighbour and its face directly.  The values are those the two
+// steps give (the M side is always in range; a neighbour beyond ntotal is
+// the same vmap error, with the same payload).
 template <class Phys, int NQ>
-__device__ __forceinline__ void face_decode(FaceItem<Phys>& it, int64_t e, int fi) {
+__device__ __forceinline__ void face_decode_resolve(FaceItem<Phys>& it, int64_t e, int fi,
+                                                    int64_t ntotal) {
   using S = Shape<Phys, NQ>;
   constexpr int DIM = Phys::DIM;
   const int f = fi / S::Nfp, fn = fi - f * S::Nfp;
-  const int64_t code = it.idP;
+  const int32_t code = (int32_t)it.idP;
   const bool wall = code < 0;
-  const int64_t eP = wall ? e : code / S::NFACE;
-  const int fP = wall ? f : (int)(code - eP * S::NFACE);
-  it.idM = (int64_t)S::Np * e + canon_vid<DIM, NQ>(f, fn) + 1;
-  it.idP = (int64_t)S::Np * eP + canon_vid<DIM, NQ>(fP, fn) + 1;
-  it.bcode = wall ? 1 : 0;
+  const int32_t eP = wall ? 0 : code / S::NFACE;
+  const int fP = wall ? f : code - eP * S::NFACE;
+  const int vP = canon_vid<DIM, NQ>(fP, fn);
+  const bool ok = wall || (int64_t)eP < ntotal;
+  it.bc = wall ? 1 : 0;
+  it.eP = wall || !ok ? 0 : eP;
+  it.vidP = wall || !ok ? 0 : vP;
+  it.vidM = ok ? canon_vid<DIM, NQ>(f, fn) : -1;  // -1: no contribution
+  if (!ok) record_error(5 /*CANARY_ERR_VMAP*/, f + 1, e + 1, (int64_t)S::Np * eP + vP + 1);
 }
 
 // The height y of node v of element e (vgeo field iY), for the equation
@@ -705,9 +715,11 @@ __device__ __forceinline__ void stage_body(
     const int item = t + k * NT;
     const int sl = item / NFN;
     if constexpr (PREP)
-      face_decode<Phys, NQ>(fit[k], fval[k] ? e0 + sl : e0, fval[k] ? item - sl * NFN : 0);
-    face_resolve<Phys, NQ>(fit[k], fval[k] ? e0 + sl : e0, fval[k] ? item - sl * NFN : 0,
-                           ntotal);
+      face_decode_resolve<Phys, NQ>(fit[k], fval[k] ? e0 + sl : e0,
+                                    fval[k] ? item - sl * NFN : 0, ntotal);
+    else
+      face_resolve<Phys, NQ>(fit[k], fval[k] ? e0 + sl : e0, fval[k] ? item - sl * NFN : 0,
+                             ntotal);
     face_gather_any<Phys, NQ>(fit[k], Qin, vgeo, fval[k], e0);
   }
 
@@ -790,8 +802,7 @@ __device__ __forceinline__ void stage_body(
     if constexpr (PREP) {
       FaceItem<Phys> it;
       face_load_prep<Phys, NQ>(it, sgeo, pface, e0 + sl, fi);
-      face_decode<Phys, NQ>(it, e0 + sl, fi);
-      face_resolve<Phys, NQ>(it, e0 + sl, fi, ntotal);
+      face_decode_resolve<Phys, NQ>(it, e0 + sl, fi, ntotal);
       face_gather<Phys, NQ>(it, Qin, vgeo);
       vid = face_finish<Phys>(it, p, qm_at, pm_at, c);
     } else {
