@@ -1,5 +1,5 @@
-"""The Julia ccall shim written out in INTEGRATION.md against the C ABI of
-include/canary_dg.h: every ccall names a declared entry point and passes as
+"""The Julia ccall shim of the repository's integration notes against the C
+ABI of include/canary_dg.h: every ccall names a declared entry point and passes as
 many arguments, of matching width, as the declaration takes (Julia cannot
 run here, so this keeps the shim's text in step with the header)."""
 import os
